@@ -1,0 +1,136 @@
+/*
+ * mps2_amd: C ABI of the MI355X block-sparse tensor engine behind the
+ * two-site finite vMPS (DMRG) hot path of GraceQ/MPS2.
+ *
+ * This is the boundary the host C++ (gqmps2-compatible) layer and any FFI
+ * caller bind to.  Each entry point replaces one GraceQ/tensor (gqten) call,
+ * or one gqmps2 routine, made on the hot path; the reference call sites are
+ * cited per function (paths relative to the GraceQ/MPS2 0.2-alpha.0 tree).
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a nonzero status
+ *    (MPS2_ERR_*); mps2_last_error() gives the message of the calling
+ *    thread's last failure.  The reference exits the process on solver
+ *    failure (lanczos_solver_impl.h:288-291); here the caller decides.
+ *  - Tensors are opaque device-resident handles owned by the caller
+ *    (mps2_ten_free).  One context = one GPU + one HIP stream; a context and
+ *    its tensors must be used from one host thread at a time.
+ *  - Flat tensor format used for import/export:
+ *      int32 meta[]: rank, qn_width, nblocks,
+ *                    per leg: dir (0 = IN, 1 = OUT), nsect,
+ *                             per sector: qn[qn_width], dim
+ *                    per block: sector coordinate of each leg
+ *                    (blocks unique, sorted lexicographically)
+ *      double data[]: the blocks in meta order, each dense row-major.
+ *    Block divergence = sum(qn of OUT legs) - sum(qn of IN legs).
+ *  - Precision for mps2_lanczos: MPS2_PREC_FP64 is the plain FP64 form (all
+ *    contractions FP64 on FP64 MFMA); MPS2_PREC_MIXED runs the matvec
+ *    contractions in FP32 on MFMA with FP64 residual correction.
+ */
+#ifndef MPS2_AMD_H
+#define MPS2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPS2_OK 0
+#define MPS2_ERR_ARG 1
+#define MPS2_ERR_HIP 2
+#define MPS2_ERR_OOM 3
+#define MPS2_ERR_NUMERIC 4
+#define MPS2_ERR_STATE 5
+#define MPS2_ERR_NO_DEVICE 6
+
+#define MPS2_PREC_FP64 0
+#define MPS2_PREC_MIXED 1
+
+#define MPS2_ORTH_LEFT 0
+#define MPS2_ORTH_RIGHT 1
+
+typedef struct mps2_ctx_s *mps2_ctx;
+typedef struct mps2_ten_s *mps2_ten;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mps2_version(void);
+/* Message of this thread's last failure ("" if none). */
+const char *mps2_last_error(void);
+/* Number of visible HIP devices (0 on a host without a GPU). */
+int mps2_device_count(int *n);
+
+/* Context: one device, one stream, caching allocator, plan cache. */
+int mps2_ctx_create(int device, mps2_ctx *out);
+int mps2_ctx_destroy(mps2_ctx ctx);
+int mps2_ctx_sync(mps2_ctx ctx);
+/* Counters: algorithmic GEMM flops issued, grouped-GEMM launches, live and
+ * peak device bytes held by the allocator. */
+int mps2_ctx_stats(mps2_ctx ctx, double *gemm_flops, int64_t *gemm_launches, int64_t *live_bytes,
+                   int64_t *peak_bytes);
+int mps2_ctx_reset_stats(mps2_ctx ctx);
+
+/* Tensors.  Replace GQTensor construction, element fill and the stream I/O
+ * used by TenVec::LoadTen/DumpTen (one_dim_tn/framework/ten_vec.h:50-85). */
+int mps2_ten_import(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_ten *out);
+int mps2_ten_export_sizes(mps2_ten t, int64_t *nmeta, int64_t *ndata);
+int mps2_ten_export(mps2_ten t, int32_t *meta, double *data);
+int mps2_ten_free(mps2_ten t);
+int mps2_ten_copy(mps2_ten t, mps2_ten *out);
+/* gqten Dag for real tensors: flip every index direction (data shared).
+ * Reference call sites: lanczos_solver_impl.h:135,197;
+ * two_site_update_finite_vmps_impl.h:119,128,291,298,309,316. */
+int mps2_ten_dag(mps2_ten t, mps2_ten *out);
+/* gqten Div: divergence QN (qn_width ints).
+ * Reference call site: two_site_update_finite_vmps_impl.h:260. */
+int mps2_ten_div(mps2_ten t, int32_t *qn);
+
+/* gqten Contract(a, b, {axes_a, axes_b}, out): output legs are the free legs
+ * of a then of b; contracted legs must be mutually inverse indexes.
+ * Reference call sites: lanczos_solver_impl.h:136,198,231-234,242-244,252-254;
+ * two_site_update_finite_vmps_impl.h:118,120,124,127,129,243,272,276,290-317. */
+int mps2_contract(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, mps2_ten *out);
+
+/* gqten LinearCombine: out = beta*out + sum_i coefs[i]*ts[i] (block sets
+ * are united).  Reference call sites: lanczos_solver_impl.h:151,153,173,215. */
+int mps2_linear_combine(int n, const double *coefs, const mps2_ten *ts, double beta, mps2_ten out);
+
+/* GQTensor::Normalize: scale to unit 2-norm, return the previous norm.
+ * Reference call sites: lanczos_solver_impl.h:120,160. */
+int mps2_normalize(mps2_ten t, double *norm_before);
+
+/* Full contraction <a|b> of two real tensors (Contract(a, Dag(b)) over all
+ * legs), as in lanczos_solver_impl.h:134-141,196-204. */
+int mps2_dot(mps2_ten a, mps2_ten b, double *out);
+
+/* gqten truncated SVD(t, ldims, ldiv, trunc_err, Dmin, Dmax, &u, &s, &vt,
+ * &actual_trunc_err, &D): legs [0, ldims) go to u (plus a new OUT leg, u
+ * carries divergence ldiv), s is the real diagonal (IN, OUT) on the new bond,
+ * vt has the new IN leg then the remaining legs.  orth_side selects which
+ * factor is built exactly orthonormal (MPS2_ORTH_LEFT: u; MPS2_ORTH_RIGHT:
+ * vt); the other is t projected on it and divided by s.
+ * Reference call sites: two_site_update_finite_vmps_impl.h:258-263;
+ * finite_mps.h:184,221 (through mock_gqten::SVD, utilities.h:29-55). */
+int mps2_svd(mps2_ten t, int ldims, const int32_t *ldiv, double trunc_err, int64_t dmin, int64_t dmax,
+             int orth_side, mps2_ten *u, mps2_ten *s, mps2_ten *vt, double *actual_trunc_err, int64_t *d_kept);
+
+/* Effective Hamiltonian times state: eff_ham_mul_state_{lend,cent,rend}
+ * (lanczos_solver_impl.h:228-256).  L (for lend) and R (for rend) may be
+ * NULL, as in the reference where they are unused. */
+int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where, mps2_ten *out);
+
+/* LanczosSolver({L, W1, W2, R}, init, {error, max_iterations}, where)
+ * (lanczos_solver_impl.h:79-225).  init is not consumed (the caller frees it).
+ * Returns iterations, ground energy and a new ground-state tensor. */
+int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init, const char *where, double error,
+                 int64_t max_iter, int precision, int64_t *iters, double *energy, mps2_ten *gs);
+
+/* TridiagGsSolver (lanczos_solver_impl.h:259-312): lowest eigenpair of the
+ * symmetric tridiagonal (a[0..n-1], b[0..n-2]), solved on the device.  vec may
+ * be NULL (jobz 'N'). */
+int mps2_tridiag_lowest(mps2_ctx ctx, int n, const double *a, const double *b, double *eval, double *vec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPS2_AMD_H */

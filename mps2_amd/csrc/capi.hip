@@ -1,0 +1,289 @@
+// extern "C" boundary of the engine (declared in include/mps2_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "../../include/mps2_amd.h"
+#include "runtime.h"
+
+struct mps2_ctx_s {
+  std::unique_ptr<mps2::Ctx> c;
+};
+struct mps2_ten_s {
+  mps2::TenP t;
+};
+
+namespace mps2 {
+void plan_cache_clear(Ctx *c);
+}
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guard(F f) {
+  try {
+    f();
+    return MPS2_OK;
+  } catch (const mps2::Error &e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_err = "host allocation failed";
+    return MPS2_ERR_OOM;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return MPS2_ERR_ARG;
+  }
+}
+
+mps2_ten wrap(mps2::TenP t) {
+  auto h = new mps2_ten_s;
+  h->t = std::move(t);
+  return h;
+}
+
+void need(const void *p, const char *what) {
+  if (!p) throw mps2::Error(mps2::kErrArg, std::string(what) + " is NULL");
+}
+
+mps2::TenP opt(mps2_ten h) { return h ? h->t : nullptr; }
+}  // namespace
+
+extern "C" {
+
+int mps2_version(void) { return 100; }
+
+const char *mps2_last_error(void) { return g_err.c_str(); }
+
+int mps2_device_count(int *n) {
+  return guard([&] {
+    need(n, "n");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      c = 0;
+    }
+    *n = c;
+  });
+}
+
+int mps2_ctx_create(int device, mps2_ctx *out) {
+  return guard([&] {
+    need(out, "out");
+    auto h = new mps2_ctx_s;
+    try {
+      h->c = std::make_unique<mps2::Ctx>(device);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int mps2_ctx_destroy(mps2_ctx ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    mps2::plan_cache_clear(ctx->c.get());
+    delete ctx;
+  });
+}
+
+int mps2_ctx_sync(mps2_ctx ctx) {
+  return guard([&] {
+    need(ctx, "ctx");
+    ctx->c->sync();
+  });
+}
+
+int mps2_ctx_stats(mps2_ctx ctx, double *gemm_flops, int64_t *gemm_launches, int64_t *live_bytes,
+                   int64_t *peak_bytes) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (gemm_flops) *gemm_flops = ctx->c->gemm_flops;
+    if (gemm_launches) *gemm_launches = ctx->c->gemm_launches;
+    if (live_bytes) *live_bytes = (int64_t)ctx->c->live_bytes;
+    if (peak_bytes) *peak_bytes = (int64_t)ctx->c->peak_bytes;
+  });
+}
+
+int mps2_ctx_reset_stats(mps2_ctx ctx) {
+  return guard([&] {
+    need(ctx, "ctx");
+    ctx->c->gemm_flops = 0;
+    ctx->c->gemm_launches = 0;
+    ctx->c->peak_bytes = ctx->c->live_bytes;
+  });
+}
+
+int mps2_ten_import(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_ten *out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(meta, "meta");
+    need(out, "out");
+    auto m = mps2::meta_from_flat(meta, nullptr);
+    if (m->total > 0) need(data, "data");
+    *out = wrap(mps2::ten_upload(ctx->c.get(), m, data));
+  });
+}
+
+int mps2_ten_export_sizes(mps2_ten t, int64_t *nmeta, int64_t *ndata) {
+  return guard([&] {
+    need(t, "t");
+    if (nmeta) *nmeta = mps2::meta_flat_size(*t->t->meta);
+    if (ndata) *ndata = t->t->meta->total;
+  });
+}
+
+int mps2_ten_export(mps2_ten t, int32_t *meta, double *data) {
+  return guard([&] {
+    need(t, "t");
+    if (meta) mps2::meta_to_flat(*t->t->meta, meta);
+    if (data) mps2::ten_download(*t->t, data);
+  });
+}
+
+int mps2_ten_free(mps2_ten t) {
+  return guard([&] { delete t; });
+}
+
+int mps2_ten_copy(mps2_ten t, mps2_ten *out) {
+  return guard([&] {
+    need(t, "t");
+    need(out, "out");
+    *out = wrap(mps2::ten_copy(*t->t));
+  });
+}
+
+int mps2_ten_dag(mps2_ten t, mps2_ten *out) {
+  return guard([&] {
+    need(t, "t");
+    need(out, "out");
+    *out = wrap(mps2::ten_dag(t->t));
+  });
+}
+
+int mps2_ten_div(mps2_ten t, int32_t *qn) {
+  return guard([&] {
+    need(t, "t");
+    need(qn, "qn");
+    mps2::QN d = t->t->meta->div();
+    for (int i = 0; i < t->t->meta->qw; ++i) qn[i] = d.v[i];
+  });
+}
+
+int mps2_contract(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, mps2_ten *out) {
+  return guard([&] {
+    need(a, "a");
+    need(b, "b");
+    need(out, "out");
+    if (nax > 0) {
+      need(axes_a, "axes_a");
+      need(axes_b, "axes_b");
+    }
+    std::vector<int> xa(axes_a, axes_a + nax), xb(axes_b, axes_b + nax);
+    *out = wrap(mps2::contract(a->t, b->t, xa, xb));
+  });
+}
+
+int mps2_linear_combine(int n, const double *coefs, const mps2_ten *ts, double beta, mps2_ten out) {
+  return guard([&] {
+    need(out, "out");
+    std::vector<double> c(coefs, coefs + n);
+    std::vector<mps2::TenP> v;
+    for (int i = 0; i < n; ++i) {
+      need(ts[i], "tensor");
+      v.push_back(ts[i]->t);
+    }
+    mps2::lincomb(c, v, beta, out->t);
+  });
+}
+
+int mps2_normalize(mps2_ten t, double *norm_before) {
+  return guard([&] {
+    need(t, "t");
+    double n = mps2::normalize(*t->t);
+    if (norm_before) *norm_before = n;
+  });
+}
+
+int mps2_dot(mps2_ten a, mps2_ten b, double *out) {
+  return guard([&] {
+    need(a, "a");
+    need(b, "b");
+    need(out, "out");
+    *out = mps2::dot(*a->t, *b->t);
+  });
+}
+
+int mps2_svd(mps2_ten t, int ldims, const int32_t *ldiv, double trunc_err, int64_t dmin, int64_t dmax,
+             int orth_side, mps2_ten *u, mps2_ten *s, mps2_ten *vt, double *actual_trunc_err, int64_t *d_kept) {
+  return guard([&] {
+    need(t, "t");
+    need(ldiv, "ldiv");
+    need(u, "u");
+    need(s, "s");
+    need(vt, "vt");
+    mps2::QN q;
+    for (int i = 0; i < t->t->meta->qw; ++i) q.v[i] = ldiv[i];
+    auto r = mps2::svd(t->t, ldims, q, trunc_err, dmin, dmax,
+                       orth_side == MPS2_ORTH_RIGHT ? mps2::kOrthRight : mps2::kOrthLeft);
+    *u = wrap(r.u);
+    *s = wrap(r.s);
+    *vt = wrap(r.vt);
+    if (actual_trunc_err) *actual_trunc_err = r.trunc_err;
+    if (d_kept) *d_kept = r.D;
+  });
+}
+
+int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                  mps2_ten *out) {
+  return guard([&] {
+    need(W1, "W1");
+    need(W2, "W2");
+    need(psi, "psi");
+    need(where, "where");
+    need(out, "out");
+    std::string w(where);
+    if (w == "cent" || w == "rend") need(L, "L");
+    if (w == "cent" || w == "lend") need(R, "R");
+    auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
+    auto o = mps2::ten_alloc(psi->t->ctx, full);
+    mps2::heff_mul(opt(L), W1->t, W2->t, opt(R), psi->t, w, o, mps2::kF64);
+    *out = wrap(o);
+  });
+}
+
+int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init, const char *where, double error,
+                 int64_t max_iter, int precision, int64_t *iters, double *energy, mps2_ten *gs) {
+  return guard([&] {
+    need(W1, "W1");
+    need(W2, "W2");
+    need(init, "init");
+    need(where, "where");
+    need(gs, "gs");
+    std::string w(where);
+    if (w == "cent" || w == "rend") need(L, "L");
+    if (w == "cent" || w == "lend") need(R, "R");
+    auto r = mps2::lanczos(opt(L), W1->t, W2->t, opt(R), init->t, w, error, max_iter,
+                           precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64);
+    if (iters) *iters = r.iters;
+    if (energy) *energy = r.energy;
+    *gs = wrap(r.gs);
+  });
+}
+
+int mps2_tridiag_lowest(mps2_ctx ctx, int n, const double *a, const double *b, double *eval, double *vec) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(a, "a");
+    need(eval, "eval");
+    if (n > 1) need(b, "b");
+    if (mps2::tridiag_lowest_dev(ctx->c.get(), n, a, b, eval, vec)) throw mps2::Error(mps2::kErrArg, "n must be >= 1");
+  });
+}
+
+}  // extern "C"

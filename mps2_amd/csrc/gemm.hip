@@ -1,0 +1,187 @@
+// Grouped block-sparse GEMM on CDNA4 matrix cores.
+//
+// One launch covers every output block of one contraction stage.  Each
+// workgroup owns one BMxBN tile of one output block and accumulates, in a
+// fixed order, the products of all block pairs that feed that block (owner
+// computes: no atomics, bit-reproducible).  Operands are addressed through
+// per-pair offsets; the A operand is read either row-major (M x K) or
+// transposed (stored K x M), likewise B, so contraction legs that lead or
+// trail a block need no permutation pass.
+//
+// Matrix core: v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32),
+// 4 wavefronts of 64 lanes in a 2x2 arrangement, LDS-staged BK=16 slabs.
+#include <hip/hip_runtime.h>
+
+#include "runtime.h"
+
+namespace mps2 {
+namespace {
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+constexpr int kThreads = 256;
+constexpr int kBK = 16;
+
+template <typename T, int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
+                                                                 T *__restrict__ C, const TileD *__restrict__ tiles,
+                                                                 const BlkD *__restrict__ blks,
+                                                                 const PairD *__restrict__ pairs, int ntiles) {
+  using V4 = typename Mfma<T>::v4;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int PA = BM + 1, PB = BN + 1;
+  __shared__ T As[kBK * PA];
+  __shared__ T Bs[kBK * PB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tix = blockIdx.x;
+  if (tix >= ntiles) return;
+  const TileD tile = tiles[tix];
+  const BlkD blk = blks[tile.blk];
+  const int M = blk.M, N = blk.N;
+  const int m0 = tile.m0, n0 = tile.n0;
+
+  V4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+  for (int p = blk.pb; p < blk.pe; ++p) {
+    const PairD pr = pairs[p];
+    const int K = pr.K;
+    const T *Ab = A + pr.a_off;
+    const T *Bb = B + pr.b_off;
+    for (int k0 = 0; k0 < K; k0 += kBK) {
+      // ---- stage A slab (BM x BK) into As[k][m]
+      if (!TA) {
+        constexpr int RSTEP = kThreads / kBK;
+        const int kk = tid % kBK;
+#pragma unroll
+        for (int i = 0; i < BM / RSTEP; ++i) {
+          const int mm = tid / kBK + i * RSTEP;
+          const int gm = m0 + mm, gk = k0 + kk;
+          T v = 0;
+          if (gm < M && gk < K) v = Ab[(int64_t)gm * K + gk];
+          As[kk * PA + mm] = v;
+        }
+      } else {
+        constexpr int KSTEP = kThreads / BM;
+        const int mm = tid % BM;
+#pragma unroll
+        for (int i = 0; i < kBK / KSTEP; ++i) {
+          const int kk = tid / BM + i * KSTEP;
+          const int gm = m0 + mm, gk = k0 + kk;
+          T v = 0;
+          if (gm < M && gk < K) v = Ab[(int64_t)gk * M + gm];
+          As[kk * PA + mm] = v;
+        }
+      }
+      // ---- stage B slab (BK x BN) into Bs[k][n]
+      if (!TB) {
+        constexpr int KSTEP = kThreads / BN;
+        const int nn = tid % BN;
+#pragma unroll
+        for (int i = 0; i < kBK / KSTEP; ++i) {
+          const int kk = tid / BN + i * KSTEP;
+          const int gn = n0 + nn, gk = k0 + kk;
+          T v = 0;
+          if (gn < N && gk < K) v = Bb[(int64_t)gk * N + gn];
+          Bs[kk * PB + nn] = v;
+        }
+      } else {
+        constexpr int RSTEP = kThreads / kBK;
+        const int kk = tid % kBK;
+#pragma unroll
+        for (int i = 0; i < BN / RSTEP; ++i) {
+          const int nn = tid / kBK + i * RSTEP;
+          const int gn = n0 + nn, gk = k0 + kk;
+          T v = 0;
+          if (gn < N && gk < K) v = Bb[(int64_t)gn * K + gk];
+          Bs[kk * PB + nn] = v;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < kBK; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        T af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = As[kr * PA + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = Bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+      }
+      __syncthreads();
+    }
+  }
+  // ---- epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 fragment
+  T *Cb = C + blk.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        if (gm < M && gn < N) Cb[(int64_t)gm * N + gn] = acc[i][j][r];
+      }
+    }
+}
+
+template <typename T, int BM, int BN>
+void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                  const BlkD *blks, const PairD *pairs) {
+  dim3 grid(ntiles), block(kThreads);
+  if (la == 0 && lb == 0)
+    grouped_gemm_kernel<T, BM, BN, false, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+  else if (la == 0 && lb == 1)
+    grouped_gemm_kernel<T, BM, BN, false, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+  else if (la == 1 && lb == 0)
+    grouped_gemm_kernel<T, BM, BN, true, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+  else
+    grouped_gemm_kernel<T, BM, BN, true, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n) {
+  if (ntiles <= 0) return;
+  if (tile_m == 128 && tile_n == 128) launch_tiles<T, 128, 128>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs);
+  else launch_tiles<T, 64, 64>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs);
+  c->gemm_launches++;
+}
+
+template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *,
+                                          int, const BlkD *, const PairD *, int, int);
+template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
+                                         const BlkD *, const PairD *, int, int);
+
+}  // namespace mps2

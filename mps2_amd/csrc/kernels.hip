@@ -1,0 +1,329 @@
+// Runtime (context, caching allocator, device tensors) and the HBM-bound
+// helper kernels of the engine: fills, axpby, casts, deterministic
+// reductions, block permutations and strided block copies.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "runtime.h"
+
+namespace mps2 {
+
+// ------------------------------------------------------------------ runtime
+constexpr int kRedBlocks = 1024;
+constexpr int kRedThreads = 256;
+
+Buf::~Buf() {
+  if (ctx && p) ctx->release(p, bytes);
+}
+
+Ctx::Ctx(int dev) : device(dev) {
+  int n = 0;
+  MPS2_HIP(hipGetDeviceCount(&n));
+  if (n <= 0) throw Error(kErrNoDevice, "no HIP device visible");
+  if (dev < 0 || dev >= n) throw Error(kErrArg, "device index out of range");
+  MPS2_HIP(hipSetDevice(dev));
+  MPS2_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  MPS2_HIP(hipGetDeviceProperties(&prop, dev));
+  num_cu = prop.multiProcessorCount;
+  MPS2_HIP(hipHostMalloc((void **)&host_scalars, 256 * sizeof(double), hipHostMallocDefault));
+  red_partial = alloc(kRedBlocks * 8 * sizeof(double));
+  dev_scalars = alloc(256 * sizeof(double));
+}
+
+Ctx::~Ctx() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  red_partial.reset();
+  dev_scalars.reset();
+  for (auto &kv : free_)
+    for (void *p : kv.second) (void)hipFree(p);
+  free_.clear();
+  if (host_scalars) (void)hipHostFree(host_scalars);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+size_t Ctx::round_bytes(size_t b) {
+  if (b == 0) b = 1;
+  if (b <= (1u << 20)) {
+    size_t r = 256;
+    while (r < b) r <<= 1;
+    return r;
+  }
+  const size_t g = 2u << 20;
+  return (b + g - 1) / g * g;
+}
+
+BufP Ctx::alloc(size_t bytes) {
+  size_t rb = round_bytes(bytes);
+  void *p = nullptr;
+  auto it = free_.find(rb);
+  if (it != free_.end() && !it->second.empty()) {
+    p = it->second.back();
+    it->second.pop_back();
+    cached_bytes -= rb;
+  } else {
+    hipError_t e = hipMalloc(&p, rb);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, rb);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(kErrOom, "hipMalloc of " + std::to_string(rb) + " bytes failed");
+      }
+    }
+  }
+  live_bytes += rb;
+  peak_bytes = std::max(peak_bytes, live_bytes);
+  auto b = std::make_shared<Buf>();
+  b->ctx = this;
+  b->p = p;
+  b->bytes = rb;
+  return b;
+}
+
+void Ctx::release(void *p, size_t bytes) {
+  free_[bytes].push_back(p);
+  cached_bytes += bytes;
+  live_bytes -= bytes;
+}
+
+void Ctx::trim() {
+  (void)hipStreamSynchronize(stream);
+  for (auto &kv : free_)
+    for (void *p : kv.second) (void)hipFree(p);
+  free_.clear();
+  cached_bytes = 0;
+}
+
+void Ctx::sync() { MPS2_HIP(hipStreamSynchronize(stream)); }
+
+TenP ten_alloc(Ctx *c, MetaP m, Prec p) {
+  auto t = std::make_shared<Ten>();
+  t->ctx = c;
+  t->meta = std::move(m);
+  t->prec = p;
+  size_t es = p == kF64 ? sizeof(double) : sizeof(float);
+  t->buf = c->alloc(std::max<int64_t>(t->meta->total, 1) * es);
+  return t;
+}
+
+TenP ten_zeros(Ctx *c, MetaP m, Prec p) {
+  auto t = ten_alloc(c, std::move(m), p);
+  if (p == kF64) k_fill(c, t->d(), std::max<int64_t>(t->n(), 1), 0.0);
+  else k_fillf(c, t->f(), std::max<int64_t>(t->n(), 1), 0.0f);
+  return t;
+}
+
+TenP ten_upload(Ctx *c, MetaP m, const double *host) {
+  auto t = ten_alloc(c, std::move(m));
+  if (t->n() > 0) {
+    MPS2_HIP(hipMemcpyAsync(t->d(), host, t->n() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->sync();
+  }
+  return t;
+}
+
+void ten_download(const Ten &t, double *host) {
+  if (t.n() == 0) return;
+  if (t.prec != kF64) throw Error(kErrState, "download of a non-FP64 tensor");
+  MPS2_HIP(hipMemcpyAsync(host, t.d(), t.n() * sizeof(double), hipMemcpyDeviceToHost, t.ctx->stream));
+  t.ctx->sync();
+}
+
+TenP ten_copy(const Ten &t) {
+  auto r = ten_alloc(t.ctx, t.meta, t.prec);
+  size_t es = t.prec == kF64 ? 8 : 4;
+  if (t.n() > 0)
+    MPS2_HIP(hipMemcpyAsync(r->buf->p, t.buf->p, t.n() * es, hipMemcpyDeviceToDevice, t.ctx->stream));
+  return r;
+}
+
+TenP ten_dag(const TenP &t) {
+  auto m = std::make_shared<Meta>(*t->meta);
+  for (auto &l : m->legs) l.dir = l.dir == kIn ? kOut : kIn;
+  auto r = std::make_shared<Ten>(*t);
+  r->meta = m;
+  return r;
+}
+
+// ------------------------------------------------------------------ elementwise
+namespace {
+
+inline int grid_for(int64_t n, int threads, int cap = 65536) {
+  int64_t g = (n + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (int)g;
+}
+
+__global__ void fill_kernel(double *p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fillf_kernel(float *p, int64_t n, float v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void axpby_kernel(int64_t n, double a, const double *x, double b, double *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+__global__ void scale_kernel(int64_t n, double a, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= a;
+}
+__global__ void d2f_kernel(int64_t n, const double *x, float *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+__global__ void f2d_kernel(int64_t n, const float *x, double *y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+
+__device__ inline double block_sum(double v, double *sh) {
+  // 256 threads = 4 wavefronts of 64
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r += sh[i];
+  }
+  return r;
+}
+
+__global__ __launch_bounds__(kRedThreads) void dot_partial_kernel(int64_t n, const double *x, const double *y,
+                                                                  double *partial) {
+  __shared__ double sh[8];
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i] * y[i];
+  double r = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double *partial, int np, double *out) {
+  __shared__ double sh[8];
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[i];
+  double r = block_sum(s, sh);
+  if (threadIdx.x == 0) *out = r;
+}
+
+__global__ __launch_bounds__(kRedThreads) void block_maxabs_kernel(const double *x, const int64_t *off,
+                                                                   const int64_t *size, double *out) {
+  __shared__ double sh[8];
+  int b = blockIdx.x;
+  const double *p = x + off[b];
+  int64_t n = size[b];
+  double m = 0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(p[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
+    out[b] = r;
+  }
+}
+
+__global__ void lincomb_kernel(int64_t n, int nv, const double *const *vecs, const double *coef, double *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int v = 0; v < nv; ++v) s += coef[v] * vecs[v][i];
+    out[i] = s;
+  }
+}
+
+// Each thread writes one destination element; its block is found by binary
+// search over the destination start offsets.
+template <typename T>
+__global__ void permute_kernel(const T *src, T *dst, const PermDesc *desc, const int64_t *dst_start, int nblk,
+                               int64_t total) {
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < total;
+       g += (int64_t)gridDim.x * blockDim.x) {
+    int lo = 0, hi = nblk - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (dst_start[mid] <= g) lo = mid;
+      else hi = mid - 1;
+    }
+    const PermDesc &d = desc[lo];
+    int64_t loc = g - d.dst_off;
+    if (loc < 0 || loc >= d.size) continue;
+    int64_t so = d.src_off;
+    for (int i = d.rank - 1; i >= 0; --i) {
+      int64_t di = d.dims[i];
+      int64_t q = loc / di;
+      so += (loc - q * di) * d.sstride[i];
+      loc = q;
+    }
+    dst[g] = src[so];
+  }
+}
+
+// blockIdx.x selects the descriptor, blockIdx.y strides over its elements.
+__global__ void copy2d_kernel(const double *src, double *dst, const Copy2D *desc) {
+  const Copy2D &d = desc[blockIdx.x];
+  int64_t n = (int64_t)d.rows * d.cols;
+  for (int64_t e = blockIdx.y * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.y * blockDim.x) {
+    int64_t r = e / d.cols, cc = e - r * d.cols;
+    dst[d.dst_off + r * d.ldd + cc] = d.scale * src[d.src_off + r * d.lds + cc];
+  }
+}
+
+}  // namespace
+
+void k_fill(Ctx *c, double *p, int64_t n, double v) {
+  fill_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
+}
+void k_fillf(Ctx *c, float *p, int64_t n, float v) {
+  fillf_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
+}
+void k_axpby(Ctx *c, int64_t n, double a, const double *x, double b, double *y) {
+  if (n > 0) axpby_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, a, x, b, y);
+}
+void k_scale(Ctx *c, int64_t n, double a, double *x) {
+  if (n > 0) scale_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, a, x);
+}
+void k_cast_d2f(Ctx *c, int64_t n, const double *x, float *y) {
+  if (n > 0) d2f_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, x, y);
+}
+void k_cast_f2d(Ctx *c, int64_t n, const float *x, double *y) {
+  if (n > 0) f2d_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, x, y);
+}
+void k_dot(Ctx *c, int64_t n, const double *x, const double *y, int slot) {
+  double *part = static_cast<double *>(c->red_partial->p);
+  dot_partial_kernel<<<kRedBlocks, kRedThreads, 0, c->stream>>>(n, x, y, part);
+  reduce_final_kernel<<<1, kRedThreads, 0, c->stream>>>(part, kRedBlocks,
+                                                        static_cast<double *>(c->dev_scalars->p) + slot);
+}
+void k_block_maxabs(Ctx *c, const double *x, const int64_t *d_off, const int64_t *d_size, int nblk, double *d_out) {
+  if (nblk > 0) block_maxabs_kernel<<<nblk, 256, 0, c->stream>>>(x, d_off, d_size, d_out);
+}
+void k_lincomb(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *d_coef, double *out) {
+  if (n > 0) lincomb_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, nv, d_vecs, d_coef, out);
+}
+template <typename T>
+void k_permute(Ctx *c, const T *src, T *dst, const PermDesc *d_desc, const int64_t *d_dst_start, int nblk,
+               int64_t total) {
+  if (total > 0 && nblk > 0)
+    permute_kernel<T><<<grid_for(total, 256, 16384), 256, 0, c->stream>>>(src, dst, d_desc, d_dst_start, nblk, total);
+}
+template void k_permute<double>(Ctx *, const double *, double *, const PermDesc *, const int64_t *, int, int64_t);
+template void k_permute<float>(Ctx *, const float *, float *, const PermDesc *, const int64_t *, int, int64_t);
+
+void k_copy2d(Ctx *c, const double *src, double *dst, const Copy2D *d_desc, int n) {
+  if (n <= 0) return;
+  dim3 g(n, 32);
+  copy2d_kernel<<<g, 256, 0, c->stream>>>(src, dst, d_desc);
+}
+
+}  // namespace mps2

@@ -1,0 +1,337 @@
+// Block-sparse tensor operations on the device: Contract (permute + grouped
+// GEMM over a cached plan), LinearCombine, dot / norm / Normalize, layout
+// embedding, and the effective-Hamiltonian matvec.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "runtime.h"
+
+namespace mps2 {
+
+namespace {
+
+BufP upload_bytes(Ctx *c, const void *host, size_t bytes) {
+  auto b = c->alloc(std::max<size_t>(bytes, 16));
+  if (bytes) {
+    MPS2_HIP(hipMemcpyAsync(b->p, host, bytes, hipMemcpyHostToDevice, c->stream));
+    c->sync();   // the host vector may be freed right after
+  }
+  return b;
+}
+
+using PlanKey = std::tuple<Ctx *, const Meta *, const Meta *, std::vector<int>, std::vector<int>, const Meta *>;
+
+struct PlanEntry {
+  std::shared_ptr<DevPlan> plan;
+  MetaP keep_out;   // retarget layout kept alive while cached
+};
+
+std::map<PlanKey, PlanEntry> g_plans;
+std::vector<PlanKey> g_plan_order;
+constexpr size_t kPlanCacheCap = 512;
+
+void perm_descs(const Meta &m, const std::vector<int> &perm, std::vector<PermDesc> &out, std::vector<int64_t> &starts) {
+  int r = m.rank();
+  out.resize(m.coords.size());
+  starts.resize(m.coords.size());
+  for (size_t b = 0; b < m.coords.size(); ++b) {
+    PermDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.src_off = m.offset[b];
+    d.dst_off = m.offset[b];
+    d.size = m.bsize[b];
+    d.rank = r;
+    int64_t st[kMaxRank];
+    int64_t s = 1;
+    for (int i = r - 1; i >= 0; --i) {
+      st[i] = s;
+      s *= m.legs[i].dim[m.coords[b][i]];
+    }
+    for (int i = 0; i < r; ++i) {
+      d.dims[i] = m.legs[perm[i]].dim[m.coords[b][perm[i]]];
+      d.sstride[i] = st[perm[i]];
+    }
+    out[b] = d;
+    starts[b] = d.dst_off;
+  }
+}
+
+std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPlan> &hp, const Meta *target) {
+  auto dp = std::make_shared<DevPlan>();
+  dp->hp = hp;
+  // Tile shape: large tiles when most flops sit in blocks >= 96 on both sides.
+  double big = 0;
+  for (auto &b : hp->blks) {
+    double f = 0;
+    for (int p = b.pair_begin; p < b.pair_end; ++p) f += 2.0 * b.M * b.N * hp->pairs[p].K;
+    if (b.M >= 96 && b.N >= 96) big += f;
+  }
+  int ts = (hp->flops > 0 && big > 0.5 * hp->flops) ? 128 : 64;
+  dp->tile_m = dp->tile_n = ts;
+  std::vector<BlkD> blks(hp->blks.size());
+  std::vector<PairD> pairs(hp->pairs.size());
+  for (size_t i = 0; i < hp->pairs.size(); ++i) {
+    pairs[i].a_off = hp->pairs[i].a_off;
+    pairs[i].b_off = hp->pairs[i].b_off;
+    pairs[i].K = hp->pairs[i].K;
+    pairs[i].pad = 0;
+  }
+  struct TC {
+    TileD t;
+    int64_t cost;
+  };
+  std::vector<TC> tl;
+  for (size_t i = 0; i < hp->blks.size(); ++i) {
+    const BlkH &b = hp->blks[i];
+    blks[i].c_off = b.c_off;
+    if (target) {
+      const Coord &oc = hp->out->coords[i];
+      int32_t tb = target->find(oc);
+      if (tb < 0) throw Error(kErrArg, "contract_into: target layout lacks an output block");
+      blks[i].c_off = target->offset[tb];
+    }
+    blks[i].M = b.M;
+    blks[i].N = b.N;
+    blks[i].pb = b.pair_begin;
+    blks[i].pe = b.pair_end;
+    int64_t kc = 0;
+    for (int p = b.pair_begin; p < b.pair_end; ++p) kc += hp->pairs[p].K;
+    for (int m0 = 0; m0 < b.M; m0 += ts)
+      for (int n0 = 0; n0 < b.N; n0 += ts) tl.push_back({TileD{(int32_t)i, m0, n0, 0}, kc});
+  }
+  std::stable_sort(tl.begin(), tl.end(), [](const TC &x, const TC &y) { return x.cost > y.cost; });
+  std::vector<TileD> tiles(tl.size());
+  for (size_t i = 0; i < tl.size(); ++i) tiles[i] = tl[i].t;
+  dp->ntiles = (int)tiles.size();
+  dp->tiles = upload_bytes(c, tiles.data(), tiles.size() * sizeof(TileD));
+  dp->blks = upload_bytes(c, blks.data(), blks.size() * sizeof(BlkD));
+  dp->pairs = upload_bytes(c, pairs.data(), pairs.size() * sizeof(PairD));
+  if (hp->permute_a) {
+    std::vector<PermDesc> d;
+    std::vector<int64_t> s;
+    perm_descs(*hp->a, hp->aperm, d, s);
+    dp->perm_a = upload_bytes(c, d.data(), d.size() * sizeof(PermDesc));
+    dp->perm_a_start = upload_bytes(c, s.data(), s.size() * sizeof(int64_t));
+    dp->nperm_a = (int)d.size();
+  }
+  if (hp->permute_b) {
+    std::vector<PermDesc> d;
+    std::vector<int64_t> s;
+    perm_descs(*hp->b, hp->bperm, d, s);
+    dp->perm_b = upload_bytes(c, d.data(), d.size() * sizeof(PermDesc));
+    dp->perm_b_start = upload_bytes(c, s.data(), s.size() * sizeof(int64_t));
+    dp->nperm_b = (int)d.size();
+  }
+  return dp;
+}
+
+std::shared_ptr<DevPlan> get_plan_t(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
+                                    const std::vector<int> &bx, const MetaP &target) {
+  PlanKey key{c, a.get(), b.get(), ax, bx, target.get()};
+  auto it = g_plans.find(key);
+  if (it != g_plans.end() && it->second.plan->hp->a == a && it->second.plan->hp->b == b) return it->second.plan;
+  auto hp = plan_contract(a, b, ax, bx);
+  auto dp = build_dev_plan(c, hp, target.get());
+  if (g_plans.size() >= kPlanCacheCap) {
+    g_plans.erase(g_plan_order.front());
+    g_plan_order.erase(g_plan_order.begin());
+  }
+  g_plans[key] = PlanEntry{dp, target};
+  g_plan_order.push_back(key);
+  return dp;
+}
+
+template <typename T>
+T *ptr(const TenP &t) {
+  return static_cast<T *>(t->buf->p);
+}
+
+template <typename T>
+void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
+  const T *A = ptr<T>(a);
+  const T *B = ptr<T>(b);
+  BufP pa, pb;
+  if (dp.hp->permute_a) {
+    pa = c->alloc(std::max<int64_t>(a->n(), 1) * sizeof(T));
+    k_permute<T>(c, A, static_cast<T *>(pa->p), static_cast<const PermDesc *>(dp.perm_a->p),
+                 static_cast<const int64_t *>(dp.perm_a_start->p), dp.nperm_a, a->n());
+    A = static_cast<const T *>(pa->p);
+  }
+  if (dp.hp->permute_b) {
+    pb = c->alloc(std::max<int64_t>(b->n(), 1) * sizeof(T));
+    k_permute<T>(c, B, static_cast<T *>(pb->p), static_cast<const PermDesc *>(dp.perm_b->p),
+                 static_cast<const int64_t *>(dp.perm_b_start->p), dp.nperm_b, b->n());
+    B = static_cast<const T *>(pb->p);
+  }
+  launch_grouped_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, static_cast<const TileD *>(dp.tiles->p), dp.ntiles,
+                         static_cast<const BlkD *>(dp.blks->p), static_cast<const PairD *>(dp.pairs->p), dp.tile_m,
+                         dp.tile_n);
+  c->gemm_flops += dp.hp->flops;
+}
+
+}  // namespace
+
+void plan_cache_clear(Ctx *c) {
+  for (auto it = g_plans.begin(); it != g_plans.end();) {
+    if (std::get<0>(it->first) == c) it = g_plans.erase(it);
+    else ++it;
+  }
+  std::vector<PlanKey> keep;
+  for (auto &k : g_plan_order)
+    if (std::get<0>(k) != c) keep.push_back(k);
+  g_plan_order.swap(keep);
+}
+
+std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
+                                  const std::vector<int> &bx) {
+  return get_plan_t(c, a, b, ax, bx, nullptr);
+}
+
+TenP contract(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx) {
+  MPS2_REQUIRE(a->ctx == b->ctx, "Contract: tensors live on different contexts");
+  MPS2_REQUIRE(a->prec == b->prec, "Contract: operand precisions differ");
+  Ctx *c = a->ctx;
+  auto dp = get_plan(c, a->meta, b->meta, ax, bx);
+  auto out = ten_alloc(c, dp->hp->out, a->prec);
+  if (dp->hp->out->total == 0) return out;
+  if (a->prec == kF64) run_plan<double>(c, *dp, a, b, out->d());
+  else run_plan<float>(c, *dp, a, b, out->f());
+  return out;
+}
+
+void contract_into(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx,
+                   const TenP &out) {
+  MPS2_REQUIRE(a->prec == b->prec && a->prec == out->prec, "Contract: operand precisions differ");
+  Ctx *c = a->ctx;
+  auto dp = get_plan_t(c, a->meta, b->meta, ax, bx, out->meta);
+  if (!(dp->hp->out->coords == out->meta->coords)) {
+    if (out->prec == kF64) k_fill(c, out->d(), std::max<int64_t>(out->n(), 1), 0.0);
+    else k_fillf(c, out->f(), std::max<int64_t>(out->n(), 1), 0.0f);
+  }
+  if (dp->hp->out->total == 0) return;
+  if (a->prec == kF64) run_plan<double>(c, *dp, a, b, out->d());
+  else run_plan<float>(c, *dp, a, b, out->f());
+}
+
+namespace {
+MetaP union_meta(const Meta &a, const Meta &b) {
+  MPS2_REQUIRE(a.legs == b.legs, "layouts with different indexes cannot be combined");
+  auto m = std::make_shared<Meta>();
+  m->qw = a.qw;
+  m->legs = a.legs;
+  m->coords = a.coords;
+  m->coords.insert(m->coords.end(), b.coords.begin(), b.coords.end());
+  m->finalize();
+  return m;
+}
+}  // namespace
+
+TenP embed(const TenP &t, const MetaP &m) {
+  MPS2_REQUIRE(t->prec == kF64, "embed: FP64 tensors only");
+  Ctx *c = t->ctx;
+  if (t->meta->same_layout(*m)) {
+    auto r = ten_copy(*t);
+    r->meta = m;
+    return r;
+  }
+  MPS2_REQUIRE(t->meta->legs == m->legs, "embed: indexes differ");
+  auto r = ten_zeros(c, m);
+  std::vector<Copy2D> d;
+  for (size_t b = 0; b < t->meta->coords.size(); ++b) {
+    int32_t tb = m->find(t->meta->coords[b]);
+    MPS2_REQUIRE(tb >= 0, "embed: target layout lacks a block");
+    int64_t sz = t->meta->bsize[b];
+    if (sz == 0) continue;
+    d.push_back(Copy2D{t->meta->offset[b], m->offset[tb], 1, (int32_t)sz, (int32_t)sz, (int32_t)sz, 1.0});
+  }
+  if (!d.empty()) {
+    auto dd = upload_bytes(c, d.data(), d.size() * sizeof(Copy2D));
+    k_copy2d(c, t->d(), r->d(), static_cast<const Copy2D *>(dd->p), (int)d.size());
+  }
+  return r;
+}
+
+double dot(const Ten &a, const Ten &b) {
+  MPS2_REQUIRE(a.prec == kF64 && b.prec == kF64, "dot: FP64 tensors only");
+  Ctx *c = a.ctx;
+  if (!a.meta->same_layout(*b.meta)) {
+    auto m = union_meta(*a.meta, *b.meta);
+    auto ta = std::make_shared<Ten>(a), tb = std::make_shared<Ten>(b);
+    auto ea = embed(ta, m), eb = embed(tb, m);
+    return dot(*ea, *eb);
+  }
+  if (a.n() == 0) return 0.0;
+  k_dot(c, a.n(), a.d(), b.d(), 0);
+  MPS2_HIP(hipMemcpyAsync(c->host_scalars, c->dev_scalars->p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  return c->host_scalars[0];
+}
+
+double norm(const Ten &a) { return std::sqrt(dot(a, a)); }
+
+double normalize(Ten &a) {
+  double n = norm(a);
+  if (n > 0) k_scale(a.ctx, a.n(), 1.0 / n, a.d());
+  return n;
+}
+
+void lincomb(const std::vector<double> &coefs, const std::vector<TenP> &ts, double beta, const TenP &out) {
+  MPS2_REQUIRE(coefs.size() == ts.size(), "LinearCombine: coefficient count differs from tensor count");
+  Ctx *c = out->ctx;
+  // Grow `out` to the union of all block sets (gqten LinearCombine semantics).
+  MetaP m = out->meta;
+  for (auto &t : ts)
+    if (!t->meta->same_layout(*m)) {
+      bool sub = t->meta->legs == m->legs;
+      if (sub)
+        for (auto &cc : t->meta->coords)
+          if (m->find(cc) < 0) {
+            sub = false;
+            break;
+          }
+      if (!sub) m = union_meta(*m, *t->meta);
+    }
+  if (m != out->meta) {
+    auto grown = embed(out, m);
+    out->meta = grown->meta;
+    out->buf = grown->buf;
+  }
+  if (beta != 1.0) k_scale(c, out->n(), beta, out->d());
+  for (size_t i = 0; i < ts.size(); ++i) {
+    TenP t = ts[i];
+    if (!t->meta->same_layout(*out->meta)) t = embed(t, out->meta);
+    k_axpby(c, out->n(), coefs[i], t->d(), 1.0, out->d());
+  }
+}
+
+void heff_mul(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &psi,
+              const std::string &where, const TenP &out, Prec prec) {
+  (void)prec;
+  if (where == "cent") {
+    auto t1 = contract(L, psi, {0}, {0});
+    auto t2 = contract(t1, W1, {0, 2}, {0, 1});
+    t1.reset();
+    auto t3 = contract(t2, W2, {4, 1}, {0, 1});
+    t2.reset();
+    contract_into(t3, R, {4, 1}, {1, 0}, out);
+  } else if (where == "lend") {
+    auto t1 = contract(psi, W1, {0}, {0});
+    auto t2 = contract(t1, W2, {0, 2}, {1, 0});
+    t1.reset();
+    contract_into(t2, R, {0, 3}, {0, 1}, out);
+  } else if (where == "rend") {
+    auto t1 = contract(psi, L, {0}, {0});
+    auto t2 = contract(t1, W1, {2, 0}, {0, 1});
+    t1.reset();
+    contract_into(t2, W2, {3, 0}, {1, 0}, out);
+  } else {
+    throw Error(kErrArg, "where must be lend, cent or rend");
+  }
+}
+
+}  // namespace mps2

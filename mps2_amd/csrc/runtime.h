@@ -1,0 +1,215 @@
+// Device runtime of the block-sparse engine: one context per GPU holding one
+// HIP stream, a stream-ordered caching allocator and a plan cache.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "meta.h"
+
+namespace mps2 {
+
+enum Status : int {
+  kOk = 0,
+  kErrArg = 1,
+  kErrHip = 2,
+  kErrOom = 3,
+  kErrNumeric = 4,
+  kErrState = 5,
+  kErrNoDevice = 6,
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define MPS2_HIP(expr)                                                                              \
+  do {                                                                                              \
+    hipError_t e_ = (expr);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      throw ::mps2::Error(e_ == hipErrorOutOfMemory ? ::mps2::kErrOom : ::mps2::kErrHip,           \
+                          std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+#define MPS2_REQUIRE(cond, msg)                                       \
+  do {                                                                \
+    if (!(cond)) throw ::mps2::Error(::mps2::kErrArg, (msg));         \
+  } while (0)
+
+struct Ctx;
+
+// Device allocation owned through shared_ptr; returned to the context cache
+// on release (stream order makes reuse safe: all work runs on ctx->stream).
+struct Buf {
+  Ctx *ctx = nullptr;
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~Buf();
+};
+using BufP = std::shared_ptr<Buf>;
+
+enum Prec : int { kF64 = 0, kF32 = 1 };
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cu = 0;
+  // caching allocator
+  std::map<size_t, std::vector<void *>> free_;
+  size_t cached_bytes = 0, live_bytes = 0, peak_bytes = 0;
+  // pinned host staging for scalars
+  double *host_scalars = nullptr;   // 256 doubles
+  BufP red_partial;                 // reduction partials (device)
+  BufP dev_scalars;                 // 256 doubles on device
+  // statistics
+  double gemm_flops = 0;
+  int64_t gemm_launches = 0;
+  std::string last_kernel;
+
+  explicit Ctx(int dev);
+  ~Ctx();
+  BufP alloc(size_t bytes);
+  void release(void *p, size_t bytes);
+  void trim();
+  void sync();
+  static size_t round_bytes(size_t b);
+};
+
+// A device-resident block-sparse tensor.  Data is FP64 unless prec == kF32
+// (mixed-precision working copies).  Tensors sharing a Buf (Dag views) never
+// write through it.
+struct Ten {
+  Ctx *ctx = nullptr;
+  MetaP meta;
+  BufP buf;
+  Prec prec = kF64;
+  double *d() const { return static_cast<double *>(buf->p); }
+  float *f() const { return static_cast<float *>(buf->p); }
+  int64_t n() const { return meta->total; }
+};
+using TenP = std::shared_ptr<Ten>;
+
+TenP ten_alloc(Ctx *c, MetaP m, Prec p = kF64);
+TenP ten_zeros(Ctx *c, MetaP m, Prec p = kF64);
+TenP ten_upload(Ctx *c, MetaP m, const double *host);
+void ten_download(const Ten &t, double *host);
+TenP ten_copy(const Ten &t);
+TenP ten_dag(const TenP &t);   // real tensors: flip directions, share data
+
+// ------------------------------------------------------------------ kernels (kernels.hip)
+void k_fill(Ctx *c, double *p, int64_t n, double v);
+void k_fillf(Ctx *c, float *p, int64_t n, float v);
+void k_axpby(Ctx *c, int64_t n, double a, const double *x, double b, double *y);     // y = a x + b y
+void k_scale(Ctx *c, int64_t n, double a, double *x);
+void k_cast_d2f(Ctx *c, int64_t n, const double *x, float *y);
+void k_cast_f2d(Ctx *c, int64_t n, const float *x, double *y);
+// Deterministic reductions; result left in device scalar slot `slot` of ctx->dev_scalars.
+void k_dot(Ctx *c, int64_t n, const double *x, const double *y, int slot);
+void k_block_maxabs(Ctx *c, const double *x, const int64_t *d_off, const int64_t *d_size, int nblk, double *d_out);
+// out = sum_i coef[i] * vec[i] (device pointer table, device coefficients)
+void k_lincomb(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *d_coef, double *out);
+// Block-wise permutation / copy (descriptors on device).
+struct PermDesc {
+  int64_t src_off, dst_off, size;
+  int32_t rank, pad;
+  int64_t dims[kMaxRank];      // dst dims
+  int64_t sstride[kMaxRank];   // src stride for each dst axis
+};
+template <typename T>
+void k_permute(Ctx *c, const T *src, T *dst, const PermDesc *d_desc, const int64_t *d_dst_start, int nblk,
+               int64_t total);
+// Strided 2D copies: dst[dr*ldd + dc] = src[sr*lds + sc] over (rows x cols).
+struct Copy2D {
+  int64_t src_off, dst_off;
+  int32_t rows, cols;
+  int32_t lds, ldd;
+  double scale;   // multiply by scale (1 for plain copies)
+};
+void k_copy2d(Ctx *c, const double *src, double *dst, const Copy2D *d_desc, int n);
+
+// Grouped GEMM over a contraction plan (gemm.hip).
+struct TileD {
+  int32_t blk;
+  int32_t m0, n0;
+  int32_t pad;
+};
+struct BlkD {
+  int64_t c_off;
+  int32_t M, N;
+  int32_t pb, pe;
+};
+struct PairD {
+  int64_t a_off, b_off;
+  int32_t K, pad;
+};
+template <typename T>
+void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n);
+
+// ------------------------------------------------------------------ ops (ops.hip)
+// Device-side contraction plan (descriptors uploaded once, cached by layout).
+struct DevPlan {
+  std::shared_ptr<ContractPlan> hp;
+  BufP tiles, blks, pairs;
+  int ntiles = 0;
+  int tile_m = 64, tile_n = 64;
+  BufP perm_a, perm_b, perm_a_start, perm_b_start;   // permutation descriptors
+  int nperm_a = 0, nperm_b = 0;
+};
+
+std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
+                                  const std::vector<int> &bx);
+TenP contract(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx);
+// Contract writing into `out` whose meta must contain every output block of the
+// plan; blocks of `out` not produced are zero-filled.
+void contract_into(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx,
+                   const TenP &out);
+double dot(const Ten &a, const Ten &b);
+double norm(const Ten &a);
+double normalize(Ten &a);
+// out = beta*out + sum c_i t_i; all tensors must share out's layout or be
+// embeddable into it (missing blocks read as zero).
+void lincomb(const std::vector<double> &coefs, const std::vector<TenP> &ts, double beta, const TenP &out);
+// Copy t into the layout `m` (blocks of m absent from t are zero; blocks of t
+// absent from m are an error).
+TenP embed(const TenP &t, const MetaP &m);
+
+// Effective-Hamiltonian matvec (where = lend | cent | rend) with the
+// reference's contraction sequence; result embedded into `out` (full layout).
+void heff_mul(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &psi,
+              const std::string &where, const TenP &out, Prec prec);
+
+// Truncated SVD (svd.hip).
+struct SvdOut {
+  TenP u, s, vt;
+  double trunc_err = 0;
+  int64_t D = 0;
+  std::vector<int64_t> kept;   // per new-bond sector
+};
+enum OrthSide : int { kOrthLeft = 0, kOrthRight = 1 };
+SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax,
+           OrthSide side);
+
+// Symmetric tridiagonal lowest eigenpair on the device (tridiag.hip).
+int tridiag_lowest_dev(Ctx *c, int n, const double *a, const double *b, double *eval, double *vec);
+
+// Lanczos (lanczos.hip).
+struct LanczosOut {
+  int64_t iters = 0;
+  double energy = 0;
+  TenP gs;
+  int64_t matvecs_lo = 0, matvecs_hi = 0;   // low-precision / FP64 matvecs
+};
+LanczosOut lanczos(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
+                   const std::string &where, double error, int64_t max_iter, Prec prec);
+
+}  // namespace mps2

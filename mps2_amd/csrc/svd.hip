@@ -1,0 +1,736 @@
+// Truncated SVD of a block-sparse tensor on the device, the counterpart of
+// gqten's SVD as called by TwoSiteFiniteVMPSUpdate
+// (include/gqmps2/algorithm/vmps/two_site_update_finite_vmps_impl.h:258-263)
+// and FiniteMPS::{Left,Right}CanonicalizeTen_ (finite_mps.h:172-243).
+//
+// Per new-bond QN sector q the state is a dense matrix M_q (rows: left-leg
+// tuples, columns: right-leg tuples, taken from blocks holding a nonzero
+// element, both in lexicographic tuple order).  The side that must come out
+// orthonormal (left for a left-canonical U, right for a right-canonical Vt)
+// is found from the eigen-decomposition of its Gram matrix (M M^T or M^T M,
+// computed with the grouped MFMA GEMM); singular values are sqrt(eigenvalue).
+// The other factor is M projected on that basis (S Vt = U^T M, or U S = M V),
+// so both factors of the product used by the sweep are exact even for
+// (near-)zero singular values.
+//
+// Eigen-decomposition: batched block Jacobi.  Each sector's Gram is padded to
+// nb*b (b = 32, nb even); per round of a round-robin tournament over the nb
+// column blocks, every block pair (I, J) is diagonalised in LDS by cyclic
+// Jacobi (64 x 64), then the rotations are applied to rows, then to columns
+// and to the accumulated eigenvectors.  Sweeps repeat until a sweep performs
+// no rotation above 1e-15 * trace(G_q).
+//
+// Truncation (host, on the singular values only):
+//  1. all singular values, sorted descending, ties by (sector, index);
+//  2. D_err = smallest D >= 1 with sum_{i>=D} s_i^2 <= trunc_err * sum s_i^2;
+//  3. D = min(max(D_err, Dmin), Dmax, count);
+//  4. values within relative 1e-8 of the last kept value form the cut cluster;
+//     the cut keeps as many of them as step 3 does, chosen by (sector, index).
+// New-bond sectors are ordered by ascending QN; states in a sector by
+// descending singular value.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+
+#include "runtime.h"
+
+namespace mps2 {
+
+namespace {
+
+constexpr int kB = 32;        // Jacobi block width
+constexpr int kP = 2 * kB;    // subproblem size
+constexpr double kDegTol = 1e-8;
+
+BufP upload(Ctx *c, const void *h, size_t bytes) {
+  auto b = c->alloc(std::max<size_t>(bytes, 16));
+  if (bytes) {
+    MPS2_HIP(hipMemcpyAsync(b->p, h, bytes, hipMemcpyHostToDevice, c->stream));
+    c->sync();
+  }
+  return b;
+}
+
+// Grouped GEMM from host-side block / pair lists (not cached).
+void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *C, const std::vector<BlkD> &blks,
+              const std::vector<PairD> &pairs) {
+  std::vector<TileD> tiles;
+  for (size_t i = 0; i < blks.size(); ++i)
+    for (int m0 = 0; m0 < blks[i].M; m0 += 64)
+      for (int n0 = 0; n0 < blks[i].N; n0 += 64) tiles.push_back(TileD{(int32_t)i, m0, n0, 0});
+  if (tiles.empty()) return;
+  auto dt = upload(c, tiles.data(), tiles.size() * sizeof(TileD));
+  auto db = upload(c, blks.data(), blks.size() * sizeof(BlkD));
+  auto dp = upload(c, pairs.data(), pairs.size() * sizeof(PairD));
+  launch_grouped_gemm<double>(c, la, lb, A, B, C, static_cast<const TileD *>(dt->p), (int)tiles.size(),
+                              static_cast<const BlkD *>(db->p), static_cast<const PairD *>(dp->p), 64, 64);
+  c->sync();
+}
+
+struct SecDesc {     // per sector, device
+  int64_t g_off;     // offset of padded Gram / eigenvector matrices
+  int32_t np;        // padded size nb * kB
+  int32_t nb;        // number of blocks (even)
+  double tau;        // rotation threshold
+};
+
+struct PairWork {    // one (sector, I, J) subproblem
+  int32_t sec, I, J, slot;   // slot: Q storage index
+};
+
+struct ApplyWork {   // one (pair, other block) tile of the row / column update
+  int32_t pair;      // index into the round's PairWork list (absolute)
+  int32_t blk;
+};
+
+// Tournament (circle method) pairing of nb players in round r (nb even).
+void tournament(int nb, int r, std::vector<std::pair<int, int>> &out) {
+  out.clear();
+  int m = nb - 1;
+  for (int k = 0; k < nb / 2; ++k) {
+    int p, q;
+    if (k == 0) {
+      p = nb - 1;
+      q = r % m;
+    } else {
+      p = (r + k) % m;
+      q = (r - k + m) % m;
+    }
+    out.push_back({std::min(p, q), std::max(p, q)});
+  }
+}
+
+__device__ inline void jacobi_rot(double app, double aqq, double apq, double &c, double &s) {
+  double zeta = (aqq - app) / (2.0 * apq);
+  double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  c = 1.0 / sqrt(1.0 + t * t);
+  s = t * c;
+}
+
+// Diagonalise the 64x64 subproblem [[G_II, G_IJ], [G_IJ^T, G_JJ]] by cyclic
+// Jacobi in LDS; write the accumulated rotation Q (64x64, row-major).
+__global__ __launch_bounds__(256) void pair_eig_kernel(const double *G, const SecDesc *secs, const PairWork *work,
+                                                       double *Qout, int *rotated) {
+  __shared__ double S[kP][kP + 1];
+  __shared__ double Q[kP][kP + 1];
+  __shared__ double cs[kP / 2], sn[kP / 2];
+  __shared__ int pp[kP / 2], qq[kP / 2];
+  __shared__ int any_rot, any_sweep;
+  const PairWork w = work[blockIdx.x];
+  const SecDesc sd = secs[w.sec];
+  const double *g = G + sd.g_off;
+  const int np = sd.np;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < kP * kP; e += 256) {
+    int i = e / kP, j = e % kP;
+    int gi = (i < kB ? w.I * kB + i : w.J * kB + (i - kB));
+    int gj = (j < kB ? w.J * kB * 0 + w.I * kB + j : w.J * kB + (j - kB));
+    double v1 = g[(int64_t)gi * np + gj], v2 = g[(int64_t)gj * np + gi];
+    S[i][j] = 0.5 * (v1 + v2);
+    Q[i][j] = (i == j) ? 1.0 : 0.0;
+  }
+  if (tid == 0) any_rot = 0;
+  __syncthreads();
+  const double tau = sd.tau;
+  for (int sweep = 0; sweep < 15; ++sweep) {
+    if (tid == 0) any_sweep = 0;
+    __syncthreads();
+    for (int r = 0; r < kP - 1; ++r) {
+      if (tid < kP / 2) {
+        int k = tid, m = kP - 1, p, q;
+        if (k == 0) {
+          p = kP - 1;
+          q = r;
+        } else {
+          p = (r + k) % m;
+          q = (r - k + m) % m;
+        }
+        if (p > q) {
+          int t = p;
+          p = q;
+          q = t;
+        }
+        pp[k] = p;
+        qq[k] = q;
+        double apq = S[p][q];
+        if (fabs(apq) > tau) {
+          double c, s;
+          jacobi_rot(S[p][p], S[q][q], apq, c, s);
+          cs[k] = c;
+          sn[k] = s;
+          any_sweep = 1;
+        } else {
+          cs[k] = 1.0;
+          sn[k] = 0.0;
+        }
+      }
+      __syncthreads();
+      // rows p, q of S
+      for (int e = tid; e < (kP / 2) * kP; e += 256) {
+        int k = e / kP, j = e % kP;
+        double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        int p = pp[k], q = qq[k];
+        double x = S[p][j], y = S[q][j];
+        S[p][j] = c * x - s * y;
+        S[q][j] = s * x + c * y;
+      }
+      __syncthreads();
+      // columns p, q of S and Q
+      for (int e = tid; e < (kP / 2) * kP; e += 256) {
+        int k = e / kP, i = e % kP;
+        double c = cs[k], s = sn[k];
+        if (s == 0.0) continue;
+        int p = pp[k], q = qq[k];
+        double x = S[i][p], y = S[i][q];
+        S[i][p] = c * x - s * y;
+        S[i][q] = s * x + c * y;
+        double u = Q[i][p], v = Q[i][q];
+        Q[i][p] = c * u - s * v;
+        Q[i][q] = s * u + c * v;
+      }
+      __syncthreads();
+    }
+    int done = !any_sweep;
+    if (!done && tid == 0) any_rot = 1;
+    __syncthreads();
+    if (done) break;
+  }
+  double *qo = Qout + (int64_t)w.slot * kP * kP;
+  for (int e = tid; e < kP * kP; e += 256) qo[e] = Q[e / kP][e % kP];
+  if (tid == 0 && any_rot) atomicOr(rotated + w.sec, 1);
+}
+
+// Rows: [G_I*; G_J*] <- Q^T [G_I*; G_J*] for one column block.
+__global__ __launch_bounds__(256) void apply_rows_kernel(double *G, const SecDesc *secs, const PairWork *pw,
+                                                         const ApplyWork *aw, const double *Qall) {
+  __shared__ double Q[kP][kP + 1];
+  __shared__ double X[kP][kB + 1];
+  const ApplyWork a = aw[blockIdx.x];
+  const PairWork w = pw[a.pair];
+  const SecDesc sd = secs[w.sec];
+  double *g = G + sd.g_off;
+  const int np = sd.np, tid = threadIdx.x;
+  const double *qs = Qall + (int64_t)w.slot * kP * kP;
+  for (int e = tid; e < kP * kP; e += 256) Q[e / kP][e % kP] = qs[e];
+  for (int e = tid; e < kP * kB; e += 256) {
+    int i = e / kB, j = e % kB;
+    int gi = i < kB ? w.I * kB + i : w.J * kB + (i - kB);
+    X[i][j] = g[(int64_t)gi * np + a.blk * kB + j];
+  }
+  __syncthreads();
+  for (int e = tid; e < kP * kB; e += 256) {
+    int i = e / kB, j = e % kB;
+    double s = 0;
+    for (int k = 0; k < kP; ++k) s += Q[k][i] * X[k][j];
+    int gi = i < kB ? w.I * kB + i : w.J * kB + (i - kB);
+    g[(int64_t)gi * np + a.blk * kB + j] = s;
+  }
+}
+
+// Columns: [G_*I, G_*J] <- [G_*I, G_*J] Q and the same for V, one row block.
+__global__ __launch_bounds__(256) void apply_cols_kernel(double *G, double *V, const SecDesc *secs,
+                                                         const PairWork *pw, const ApplyWork *aw, const double *Qall) {
+  __shared__ double Q[kP][kP + 1];
+  __shared__ double X[kB][kP + 1];
+  const ApplyWork a = aw[blockIdx.x];
+  const PairWork w = pw[a.pair];
+  const SecDesc sd = secs[w.sec];
+  const int np = sd.np, tid = threadIdx.x;
+  const double *qs = Qall + (int64_t)w.slot * kP * kP;
+  for (int e = tid; e < kP * kP; e += 256) Q[e / kP][e % kP] = qs[e];
+  for (int mat = 0; mat < 2; ++mat) {
+    double *g = (mat == 0 ? G : V) + sd.g_off;
+    __syncthreads();
+    for (int e = tid; e < kB * kP; e += 256) {
+      int i = e / kP, j = e % kP;
+      int gj = j < kB ? w.I * kB + j : w.J * kB + (j - kB);
+      X[i][j] = g[(int64_t)(a.blk * kB + i) * np + gj];
+    }
+    __syncthreads();
+    for (int e = tid; e < kB * kP; e += 256) {
+      int i = e / kP, j = e % kP;
+      double s = 0;
+      for (int k = 0; k < kP; ++k) s += X[i][k] * Q[k][j];
+      int gj = j < kB ? w.I * kB + j : w.J * kB + (j - kB);
+      g[(int64_t)(a.blk * kB + i) * np + gj] = s;
+    }
+  }
+}
+
+// Copy the n x n Gram into the padded np x np layout; set V = I.
+__global__ void pad_init_kernel(const double *Gsrc, int64_t src_off, int n, double *G, double *V, int64_t g_off,
+                                int np) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)np * np;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(e / np), j = (int)(e % np);
+    G[g_off + e] = (i < n && j < n) ? Gsrc[src_off + (int64_t)i * n + j] : 0.0;
+    V[g_off + e] = (i == j) ? 1.0 : 0.0;
+  }
+}
+
+// out[k] = diag entry, and gather selected eigenvector columns:
+// E[i * kq + t] = V[i * np + col[t]] for i < n.
+__global__ void gather_cols_kernel(const double *V, int64_t g_off, int np, int n, const int32_t *cols, int kq,
+                                   double *E) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)n * kq;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(e / kq), t = (int)(e % kq);
+    E[e] = V[g_off + (int64_t)i * np + cols[t]];
+  }
+}
+
+__global__ void diag_kernel(const double *G, int64_t g_off, int np, int n, double *out) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    out[i] = G[g_off + (int64_t)i * np + i];
+}
+
+// Scale rows (mode 0) or columns (mode 1) of a rows x cols matrix.
+__global__ void scale_rc_kernel(double *X, int rows, int cols, const double *sc, int mode) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)rows * cols;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int i = (int)(e / cols), j = (int)(e % cols);
+    X[e] *= mode == 0 ? sc[i] : sc[j];
+  }
+}
+
+// Transposed block copy: dst[c * ldd + r] = src[r * lds + c] for r < rows, c < cols.
+__global__ void copy_t_kernel(const double *src, int64_t src_off, int lds, double *dst, int64_t dst_off, int ldd,
+                              int rows, int cols) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)rows * cols;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(e / cols), cc = (int)(e % cols);
+    dst[dst_off + (int64_t)cc * ldd + r] = src[src_off + (int64_t)r * lds + cc];
+  }
+}
+
+inline int gridn(int64_t n) { return (int)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 4096); }
+
+// Batched symmetric eigen-decomposition of Gram matrices (contiguous n_q x n_q
+// row-major at goffs[q] in Gsrc).  Returns eigenvalues (unsorted, per sector)
+// and leaves padded eigenvectors in V (np_q x np_q at the returned offsets).
+struct EigOut {
+  BufP Gp, V;
+  std::vector<SecDesc> secs;
+  std::vector<std::vector<double>> evals;
+};
+
+EigOut batched_eigh(Ctx *c, const double *Gsrc, const std::vector<int64_t> &goffs, const std::vector<int> &ns,
+                    const std::vector<double> &traces) {
+  int nsec = (int)ns.size();
+  EigOut eo;
+  eo.secs.resize(nsec);
+  int64_t tot = 0;
+  int maxnb = 0;
+  for (int q = 0; q < nsec; ++q) {
+    int nb = (ns[q] + kB - 1) / kB;
+    if (nb < 2) nb = 2;
+    if (nb & 1) nb++;
+    eo.secs[q].g_off = tot;
+    eo.secs[q].nb = nb;
+    eo.secs[q].np = nb * kB;
+    eo.secs[q].tau = std::max(1e-15 * traces[q], 1e-300);
+    tot += (int64_t)nb * kB * nb * kB;
+    maxnb = std::max(maxnb, nb);
+  }
+  eo.Gp = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
+  eo.V = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
+  double *G = static_cast<double *>(eo.Gp->p), *V = static_cast<double *>(eo.V->p);
+  for (int q = 0; q < nsec; ++q)
+    pad_init_kernel<<<gridn((int64_t)eo.secs[q].np * eo.secs[q].np), 256, 0, c->stream>>>(
+        Gsrc, goffs[q], ns[q], G, V, eo.secs[q].g_off, eo.secs[q].np);
+  auto dsecs = upload(c, eo.secs.data(), nsec * sizeof(SecDesc));
+  // Work lists for every round of one sweep.
+  std::vector<PairWork> pw;
+  std::vector<ApplyWork> aw;
+  std::vector<int> round_p0(maxnb, 0), round_p1(maxnb, 0), round_a0(maxnb, 0), round_a1(maxnb, 0);
+  std::vector<std::pair<int, int>> tp;
+  int slots = 0;
+  for (int r = 0; r < maxnb - 1; ++r) {
+    round_p0[r] = (int)pw.size();
+    round_a0[r] = (int)aw.size();
+    int slot = 0;
+    for (int q = 0; q < nsec; ++q) {
+      int nb = eo.secs[q].nb;
+      if (r >= nb - 1) continue;
+      tournament(nb, r, tp);
+      for (auto &pq : tp) {
+        int pidx = (int)pw.size();
+        pw.push_back(PairWork{q, pq.first, pq.second, slot++});
+        for (int bk = 0; bk < nb; ++bk) aw.push_back(ApplyWork{pidx, bk});
+      }
+    }
+    slots = std::max(slots, slot);
+    round_p1[r] = (int)pw.size();
+    round_a1[r] = (int)aw.size();
+  }
+  auto dpw = upload(c, pw.data(), pw.size() * sizeof(PairWork));
+  auto daw = upload(c, aw.data(), aw.size() * sizeof(ApplyWork));
+  auto dq = c->alloc((int64_t)std::max(slots, 1) * kP * kP * sizeof(double));
+  auto drot = c->alloc(std::max(nsec, 1) * sizeof(int));
+  const PairWork *PW = static_cast<const PairWork *>(dpw->p);
+  const ApplyWork *AW = static_cast<const ApplyWork *>(daw->p);
+  const SecDesc *SD = static_cast<const SecDesc *>(dsecs->p);
+  double *Q = static_cast<double *>(dq->p);
+  int *ROT = static_cast<int *>(drot->p);
+  std::vector<int> hrot(nsec);
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    MPS2_HIP(hipMemsetAsync(ROT, 0, nsec * sizeof(int), c->stream));
+    for (int r = 0; r < maxnb - 1; ++r) {
+      int np_ = round_p1[r] - round_p0[r], na = round_a1[r] - round_a0[r];
+      if (np_ == 0) continue;
+      pair_eig_kernel<<<np_, 256, 0, c->stream>>>(G, SD, PW + round_p0[r], Q, ROT);
+      // ApplyWork.pair is absolute; Q slots are per round -> kernels index PW absolutely
+      apply_rows_kernel<<<na, 256, 0, c->stream>>>(G, SD, PW, AW + round_a0[r], Q);
+      apply_cols_kernel<<<na, 256, 0, c->stream>>>(G, V, SD, PW, AW + round_a0[r], Q);
+    }
+    MPS2_HIP(hipMemcpyAsync(hrot.data(), ROT, nsec * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    bool any = false;
+    for (int q = 0; q < nsec; ++q) any |= hrot[q] != 0;
+    if (!any) break;
+  }
+  // eigenvalues
+  eo.evals.resize(nsec);
+  auto dd = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
+  for (int q = 0; q < nsec; ++q)
+    diag_kernel<<<gridn(ns[q]), 256, 0, c->stream>>>(G, eo.secs[q].g_off, eo.secs[q].np, ns[q],
+                                                    static_cast<double *>(dd->p) + eo.secs[q].g_off);
+  for (int q = 0; q < nsec; ++q) {
+    eo.evals[q].resize(ns[q]);
+    if (ns[q])
+      MPS2_HIP(hipMemcpyAsync(eo.evals[q].data(), static_cast<double *>(dd->p) + eo.secs[q].g_off,
+                              ns[q] * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  }
+  c->sync();
+  return eo;
+}
+
+struct SvInfo {
+  double v;
+  int sec;
+  int j;
+};
+
+void truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err, int64_t dmin, int64_t dmax,
+                   std::vector<int64_t> &kept, double *err, int64_t *D) {
+  std::vector<SvInfo> all;
+  for (int q = 0; q < (int)sv.size(); ++q)
+    for (int j = 0; j < (int)sv[q].size(); ++j) all.push_back({sv[q][j], q, j});
+  std::sort(all.begin(), all.end(), [](const SvInfo &x, const SvInfo &y) {
+    if (x.v != y.v) return x.v > y.v;
+    if (x.sec != y.sec) return x.sec < y.sec;
+    return x.j < y.j;
+  });
+  int64_t cnt = (int64_t)all.size();
+  double total = 0;
+  for (auto &x : all) total += x.v * x.v;
+  int64_t derr = cnt;
+  {
+    double tail = 0;
+    for (int64_t i = cnt - 1; i >= 1; --i) {
+      tail += all[i].v * all[i].v;
+      if (tail <= trunc_err * total) derr = i;
+      else break;
+    }
+  }
+  int64_t d = std::max(derr, dmin);
+  d = std::min(d, dmax);
+  d = std::min(d, cnt);
+  if (d < 1 && cnt > 0) d = 1;
+  std::vector<char> keep(cnt, 0);
+  for (int64_t i = 0; i < d; ++i) keep[i] = 1;
+  if (d > 0 && d < cnt) {
+    double v = all[d - 1].v;
+    std::vector<int64_t> cl;
+    int64_t kin = 0;
+    for (int64_t i = 0; i < cnt; ++i)
+      if (std::fabs(all[i].v - v) <= kDegTol * v) {
+        cl.push_back(i);
+        if (i < d) ++kin;
+      }
+    std::sort(cl.begin(), cl.end(), [&](int64_t x, int64_t y) {
+      if (all[x].sec != all[y].sec) return all[x].sec < all[y].sec;
+      return all[x].j < all[y].j;
+    });
+    for (int64_t t = 0; t < (int64_t)cl.size(); ++t) keep[cl[t]] = t < kin ? 1 : 0;
+  }
+  kept.assign(sv.size(), 0);
+  double disc = 0;
+  for (int64_t i = 0; i < cnt; ++i) {
+    if (keep[i]) kept[all[i].sec]++;
+    else disc += all[i].v * all[i].v;
+  }
+  *err = total > 0 ? disc / total : 0.0;
+  *D = d;
+}
+
+}  // namespace
+
+SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax, OrthSide side) {
+  MPS2_REQUIRE(t->prec == kF64, "SVD: FP64 tensors only");
+  Ctx *c = t->ctx;
+  const Meta &m = *t->meta;
+  int r = m.rank();
+  MPS2_REQUIRE(ldims >= 1 && ldims < r, "SVD: ldims out of range");
+  int nblk = (int)m.coords.size();
+  // 1. nonzero blocks
+  std::vector<double> bmax(nblk, 0.0);
+  if (nblk) {
+    auto doff = upload(c, m.offset.data(), nblk * sizeof(int64_t));
+    auto dsz = upload(c, m.bsize.data(), nblk * sizeof(int64_t));
+    auto dout = c->alloc(nblk * sizeof(double));
+    k_block_maxabs(c, t->d(), static_cast<const int64_t *>(doff->p), static_cast<const int64_t *>(dsz->p), nblk,
+                   static_cast<double *>(dout->p));
+    MPS2_HIP(hipMemcpyAsync(bmax.data(), dout->p, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+  }
+  // 2. sector structure
+  struct Sec {
+    QN qn;
+    std::map<Coord, int64_t> rows, cols;
+    int64_t m = 0, n = 0;
+  };
+  std::map<QN, Sec> smap;
+  auto kqn = [&](const Coord &cc) {
+    QN q = ldiv;
+    for (int i = 0; i < ldims; ++i) q = m.legs[i].dir == kOut ? q - m.legs[i].qn[cc[i]] : q + m.legs[i].qn[cc[i]];
+    return q;
+  };
+  for (int b = 0; b < nblk; ++b) {
+    if (!(bmax[b] > 0.0)) continue;
+    const Coord &cc = m.coords[b];
+    Sec &s = smap[kqn(cc)];
+    s.qn = kqn(cc);
+    s.rows[Coord(cc.begin(), cc.begin() + ldims)] = 0;
+    s.cols[Coord(cc.begin() + ldims, cc.end())] = 0;
+  }
+  std::vector<Sec> secs;
+  for (auto &kv : smap) secs.push_back(kv.second);
+  int ns = (int)secs.size();
+  int64_t moff_tot = 0;
+  std::vector<int64_t> moff(ns);
+  for (int q = 0; q < ns; ++q) {
+    auto &s = secs[q];
+    for (auto &kv : s.rows) {
+      kv.second = s.m;
+      int64_t d = 1;
+      for (int i = 0; i < ldims; ++i) d *= m.legs[i].dim[kv.first[i]];
+      s.m += d;
+    }
+    for (auto &kv : s.cols) {
+      kv.second = s.n;
+      int64_t d = 1;
+      for (int i = ldims; i < r; ++i) d *= m.legs[i].dim[kv.first[i - ldims]];
+      s.n += d;
+    }
+    moff[q] = moff_tot;
+    moff_tot += s.m * s.n;
+  }
+  // 3. assemble sector matrices
+  auto Mb = c->alloc(std::max<int64_t>(moff_tot, 1) * sizeof(double));
+  double *Md = static_cast<double *>(Mb->p);
+  k_fill(c, Md, std::max<int64_t>(moff_tot, 1), 0.0);
+  {
+    std::vector<Copy2D> cp;
+    for (int b = 0; b < nblk; ++b) {
+      if (!(bmax[b] > 0.0)) continue;
+      const Coord &cc = m.coords[b];
+      int q = (int)std::distance(smap.begin(), smap.find(kqn(cc)));
+      auto &s = secs[q];
+      int64_t ro = s.rows.at(Coord(cc.begin(), cc.begin() + ldims));
+      int64_t co = s.cols.at(Coord(cc.begin() + ldims, cc.end()));
+      int64_t bm = 1, bn = 1;
+      for (int i = 0; i < ldims; ++i) bm *= m.legs[i].dim[cc[i]];
+      for (int i = ldims; i < r; ++i) bn *= m.legs[i].dim[cc[i]];
+      cp.push_back(Copy2D{m.offset[b], moff[q] + ro * s.n + co, (int32_t)bm, (int32_t)bn, (int32_t)bn,
+                          (int32_t)s.n, 1.0});
+    }
+    if (!cp.empty()) {
+      auto dcp = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
+      k_copy2d(c, t->d(), Md, static_cast<const Copy2D *>(dcp->p), (int)cp.size());
+    }
+  }
+  // 4. Gram matrices of the orthonormal side
+  std::vector<int> gn(ns);
+  std::vector<int64_t> goff(ns);
+  int64_t gtot = 0;
+  std::vector<BlkD> gb;
+  std::vector<PairD> gp;
+  for (int q = 0; q < ns; ++q) {
+    int64_t n_ = side == kOrthLeft ? secs[q].m : secs[q].n;
+    int64_t k_ = side == kOrthLeft ? secs[q].n : secs[q].m;
+    gn[q] = (int)n_;
+    goff[q] = gtot;
+    gb.push_back(BlkD{gtot, (int32_t)n_, (int32_t)n_, (int32_t)gp.size(), (int32_t)gp.size() + 1});
+    gp.push_back(PairD{moff[q], moff[q], (int32_t)k_, 0});
+    gtot += n_ * n_;
+  }
+  auto Gb = c->alloc(std::max<int64_t>(gtot, 1) * sizeof(double));
+  double *Gd = static_cast<double *>(Gb->p);
+  // left: G = M M^T (A = M normal, B = M^T stored as M -> trans)
+  // right: G = M^T M (A = M^T stored as M -> trans, B = M normal)
+  if (side == kOrthLeft) raw_gemm(c, kNormal, kTrans, Md, Md, Gd, gb, gp);
+  else raw_gemm(c, kTrans, kNormal, Md, Md, Gd, gb, gp);
+  // traces for the rotation threshold
+  std::vector<double> traces(ns, 0.0);
+  {
+    auto dd = c->alloc(std::max<int64_t>(gtot, 1) * sizeof(double));
+    std::vector<std::vector<double>> dg(ns);
+    for (int q = 0; q < ns; ++q) {
+      diag_kernel<<<gridn(gn[q]), 256, 0, c->stream>>>(Gd, goff[q], gn[q], gn[q], static_cast<double *>(dd->p) + goff[q]);
+      dg[q].resize(gn[q]);
+    }
+    for (int q = 0; q < ns; ++q)
+      if (gn[q])
+        MPS2_HIP(hipMemcpyAsync(dg[q].data(), static_cast<double *>(dd->p) + goff[q], gn[q] * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    for (int q = 0; q < ns; ++q)
+      for (double x : dg[q]) traces[q] += x;
+  }
+  EigOut eo = batched_eigh(c, Gd, goff, gn, traces);
+  // 5. singular values: top min(m, n) eigenvalues per sector, descending
+  std::vector<std::vector<double>> sv(ns);
+  std::vector<std::vector<int32_t>> order(ns);
+  for (int q = 0; q < ns; ++q) {
+    std::vector<int32_t> idx(gn[q]);
+    for (int i = 0; i < gn[q]; ++i) idx[i] = i;
+    const auto &ev = eo.evals[q];
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return ev[x] > ev[y]; });
+    int64_t rq = std::min(secs[q].m, secs[q].n);
+    idx.resize(rq);
+    order[q] = idx;
+    sv[q].resize(rq);
+    for (int64_t j = 0; j < rq; ++j) sv[q][j] = std::sqrt(std::max(ev[idx[j]], 0.0));
+  }
+  SvdOut out;
+  truncate_rule(sv, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
+  // 6. new bond index
+  Leg knew;
+  knew.dir = kOut;
+  std::vector<int> secpos(ns, -1);
+  for (int q = 0; q < ns; ++q)
+    if (out.kept[q] > 0) {
+      secpos[q] = (int)knew.qn.size();
+      knew.qn.push_back(secs[q].qn);
+      knew.dim.push_back(out.kept[q]);
+    }
+  auto um = std::make_shared<Meta>(), sm = std::make_shared<Meta>(), vm = std::make_shared<Meta>();
+  um->qw = sm->qw = vm->qw = m.qw;
+  for (int i = 0; i < ldims; ++i) um->legs.push_back(m.legs[i]);
+  um->legs.push_back(knew);
+  sm->legs = {knew.inverse(), knew};
+  vm->legs.push_back(knew.inverse());
+  for (int i = ldims; i < r; ++i) vm->legs.push_back(m.legs[i]);
+  for (int q = 0; q < ns; ++q) {
+    if (secpos[q] < 0) continue;
+    for (auto &kv : secs[q].rows) {
+      Coord cc = kv.first;
+      cc.push_back(secpos[q]);
+      um->coords.push_back(cc);
+    }
+    for (auto &kv : secs[q].cols) {
+      Coord cc{secpos[q]};
+      cc.insert(cc.end(), kv.first.begin(), kv.first.end());
+      vm->coords.push_back(cc);
+    }
+    sm->coords.push_back(Coord{secpos[q], secpos[q]});
+  }
+  um->finalize();
+  sm->finalize();
+  vm->finalize();
+  out.u = ten_zeros(c, um);
+  out.vt = ten_zeros(c, vm);
+  // S on the host
+  {
+    std::vector<double> sh(sm->total, 0.0);
+    for (int q = 0; q < ns; ++q) {
+      if (secpos[q] < 0) continue;
+      int32_t b = sm->find(Coord{secpos[q], secpos[q]});
+      int64_t kq = out.kept[q];
+      for (int64_t k = 0; k < kq; ++k) sh[sm->offset[b] + k * kq + k] = sv[q][k];
+    }
+    out.s = ten_upload(c, sm, sh.data());
+  }
+  // 7. factors per sector: E (orthonormal side, n x kq) and F (other side)
+  for (int q = 0; q < ns; ++q) {
+    if (secpos[q] < 0) continue;
+    auto &s = secs[q];
+    int64_t kq = out.kept[q];
+    int n_ = gn[q];
+    std::vector<int32_t> cols(order[q].begin(), order[q].begin() + kq);
+    auto dcols = upload(c, cols.data(), kq * sizeof(int32_t));
+    auto E = c->alloc(std::max<int64_t>(n_ * kq, 1) * sizeof(double));
+    gather_cols_kernel<<<gridn(n_ * kq), 256, 0, c->stream>>>(static_cast<double *>(eo.V->p), eo.secs[q].g_off,
+                                                             eo.secs[q].np, n_, static_cast<int32_t *>(dcols->p),
+                                                             (int)kq, static_cast<double *>(E->p));
+    std::vector<double> inv(kq);
+    for (int64_t k = 0; k < kq; ++k) inv[k] = sv[q][k] > 0 ? 1.0 / sv[q][k] : 0.0;
+    auto dinv = upload(c, inv.data(), kq * sizeof(double));
+    if (side == kOrthLeft) {
+      // U rows = E (m x kq); Vt = S^-1 E^T M (kq x n)
+      auto F = c->alloc(std::max<int64_t>(kq * s.n, 1) * sizeof(double));
+      std::vector<BlkD> fb{BlkD{0, (int32_t)kq, (int32_t)s.n, 0, 1}};
+      std::vector<PairD> fp{PairD{0, moff[q], (int32_t)s.m, 0}};
+      raw_gemm(c, kTrans, kNormal, static_cast<double *>(E->p), Md, static_cast<double *>(F->p), fb, fp);
+      scale_rc_kernel<<<gridn(kq * s.n), 256, 0, c->stream>>>(static_cast<double *>(F->p), (int)kq, (int)s.n,
+                                                             static_cast<double *>(dinv->p), 0);
+      std::vector<Copy2D> cp;
+      for (auto &kv : s.rows) {
+        Coord cc = kv.first;
+        cc.push_back(secpos[q]);
+        int32_t b = um->find(cc);
+        int64_t bm = um->bsize[b] / kq;
+        cp.push_back(Copy2D{kv.second * kq, um->offset[b], (int32_t)bm, (int32_t)kq, (int32_t)kq, (int32_t)kq, 1.0});
+      }
+      auto d1 = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
+      k_copy2d(c, static_cast<double *>(E->p), out.u->d(), static_cast<const Copy2D *>(d1->p), (int)cp.size());
+      cp.clear();
+      for (auto &kv : s.cols) {
+        Coord cc{secpos[q]};
+        cc.insert(cc.end(), kv.first.begin(), kv.first.end());
+        int32_t b = vm->find(cc);
+        int64_t bn = vm->bsize[b] / kq;
+        cp.push_back(Copy2D{kv.second, vm->offset[b], (int32_t)kq, (int32_t)bn, (int32_t)s.n, (int32_t)bn, 1.0});
+      }
+      auto d2 = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
+      k_copy2d(c, static_cast<double *>(F->p), out.vt->d(), static_cast<const Copy2D *>(d2->p), (int)cp.size());
+      c->sync();
+    } else {
+      // Vt = E^T (kq x n); U = M E S^-1 (m x kq)
+      auto F = c->alloc(std::max<int64_t>(s.m * kq, 1) * sizeof(double));
+      std::vector<BlkD> fb{BlkD{0, (int32_t)s.m, (int32_t)kq, 0, 1}};
+      std::vector<PairD> fp{PairD{moff[q], 0, (int32_t)s.n, 0}};
+      raw_gemm(c, kNormal, kNormal, Md, static_cast<double *>(E->p), static_cast<double *>(F->p), fb, fp);
+      scale_rc_kernel<<<gridn(s.m * kq), 256, 0, c->stream>>>(static_cast<double *>(F->p), (int)s.m, (int)kq,
+                                                             static_cast<double *>(dinv->p), 1);
+      std::vector<Copy2D> cp;
+      for (auto &kv : s.rows) {
+        Coord cc = kv.first;
+        cc.push_back(secpos[q]);
+        int32_t b = um->find(cc);
+        int64_t bm = um->bsize[b] / kq;
+        cp.push_back(Copy2D{kv.second * kq, um->offset[b], (int32_t)bm, (int32_t)kq, (int32_t)kq, (int32_t)kq, 1.0});
+      }
+      auto d1 = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
+      k_copy2d(c, static_cast<double *>(F->p), out.u->d(), static_cast<const Copy2D *>(d1->p), (int)cp.size());
+      for (auto &kv : s.cols) {
+        Coord cc{secpos[q]};
+        cc.insert(cc.end(), kv.first.begin(), kv.first.end());
+        int32_t b = vm->find(cc);
+        int64_t bn = vm->bsize[b] / kq;
+        // Vt block (kq x bn) = E[kv.second : kv.second + bn, :]^T
+        copy_t_kernel<<<gridn(bn * kq), 256, 0, c->stream>>>(static_cast<double *>(E->p), kv.second * kq, (int)kq,
+                                                            out.vt->d(), vm->offset[b], (int)bn, (int)bn, (int)kq);
+      }
+      c->sync();
+    }
+  }
+  return out;
+}
+
+}  // namespace mps2

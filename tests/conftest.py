@@ -1,0 +1,23 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP path through the C ABI)")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    from mps2_amd import engine
+    n = engine.device_count()
+    if n < 1:
+        pytest.fail("GPU test collected on a host without a visible GPU")
+    c = engine.Context(0)
+    yield c
+    c.close()

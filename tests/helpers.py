@@ -1,0 +1,117 @@
+"""Random block-sparse test problems for the parity tests.
+
+The leg conventions follow the reference: MPS A=(l IN, p OUT, r OUT),
+L=(ket OUT, mpo OUT, bra IN), R=(ket IN, mpo IN, bra OUT),
+W=(b_l IN, p_in IN, p_out OUT, b_r OUT), W_0=(p_in, b_r, p_out),
+W_{N-1}=(p_in, b_l, p_out)  (two_site_update_finite_vmps_impl.h:117-129,
+mpogen_impl.h:359,377,398; test_lanczos_solver.cc:121-168).
+"""
+import numpy as np
+
+from mps2_amd.flat import BTensor, Leg, IN, OUT
+
+
+def bond(dir, dims, qns, qw=1):
+    secs = []
+    for q, d in zip(qns, dims):
+        q = tuple(q) if isinstance(q, (tuple, list)) else (q,) + (0,) * (qw - 1)
+        secs.append((q, d))
+    return Leg(dir, secs)
+
+
+def spin_phys(dir=OUT):
+    return Leg(dir, [((1,), 1), ((-1,), 1)])
+
+
+def sym_env(rng, ket: Leg, mpo: Leg, bra: Leg, scale=1.0):
+    """Environment with zero-QN MPO sectors, symmetric in (ket, bra) per MPO index."""
+    t = BTensor([ket, mpo, bra], 1)
+    for c in t.allowed_coords((0,)):
+        if c[0] != c[2]:
+            continue
+        b = rng.uniform(-1, 1, size=t.block_shape(c)) * scale
+        t.blocks[c] = 0.5 * (b + b.transpose(2, 1, 0))
+    return t
+
+
+def sym_w(rng, bl: Leg, pin: Leg, pout: Leg, br: Leg):
+    t = BTensor([bl, pin, pout, br], 1)
+    for c in t.allowed_coords((0,)):
+        if c[1] != c[2]:
+            continue
+        b = rng.uniform(-1, 1, size=t.block_shape(c))
+        t.blocks[c] = 0.5 * (b + b.transpose(0, 2, 1, 3))
+    return t
+
+
+def hermitian_cent_problem(rng, dl=(3, 4, 2), dr=(2, 5, 3), qns=(-2, 0, 2), wdims=(2, 3)):
+    """Hermitian H_eff = sum_b L_b (x) W1 (x) W2 (x) R_b with U(1) sectors on
+    the virtual bonds and zero-QN MPO channels; psi has divergence 0."""
+    l_leg = bond(IN, dl, qns)
+    r_leg = bond(OUT, dr, qns)
+    p = spin_phys(OUT)
+    m1 = bond(OUT, wdims, (0,) * len(wdims))
+    m2 = bond(OUT, wdims, (0,) * len(wdims))
+    m3 = bond(OUT, wdims, (0,) * len(wdims))
+    L = sym_env(rng, l_leg.inverse(), m1, l_leg)
+    W1 = sym_w(rng, m1.inverse(), p.inverse(), p, m2)
+    W2 = sym_w(rng, m2.inverse(), p.inverse(), p, m3)
+    R = sym_env(rng, r_leg.inverse(), m3.inverse(), r_leg)
+    psi = BTensor.random([l_leg, p, p, r_leg], (0,), rng)
+    return L, W1, W2, R, psi
+
+
+def random_cent_problem(rng, dl=(3, 4, 2), dr=(2, 5, 3), qns=(-2, 0, 2), wq=(0, 2, -2), wd=(2, 1, 1)):
+    """Non-Hermitian random H_eff with QN-changing MPO channels (S+/S- like)."""
+    l_leg = bond(IN, dl, qns)
+    r_leg = bond(OUT, dr, qns)
+    p = spin_phys(OUT)
+    m1 = bond(OUT, wd, wq)
+    m2 = bond(OUT, wd, wq)
+    m3 = bond(OUT, wd, wq)
+    L = BTensor.random([l_leg.inverse(), m1, l_leg], (0,), rng)
+    W1 = BTensor.random([m1.inverse(), p.inverse(), p, m2], (0,), rng)
+    W2 = BTensor.random([m2.inverse(), p.inverse(), p, m3], (0,), rng)
+    R = BTensor.random([r_leg.inverse(), m3.inverse(), r_leg], (0,), rng)
+    psi = BTensor.random([l_leg, p, p, r_leg], (0,), rng)
+    return L, W1, W2, R, psi
+
+
+def random_lend_problem(rng, dr=(2, 5, 3), qns=(-1, 1, 3), wq=(0, 2, -2), wd=(2, 1, 1)):
+    r_leg = bond(OUT, dr, qns)
+    p = spin_phys(OUT)
+    m1 = bond(OUT, wd, wq)
+    m2 = bond(OUT, wd, wq)
+    W1 = BTensor.random([p.inverse(), m1, p], (0,), rng)
+    W2 = BTensor.random([m1.inverse(), p.inverse(), p, m2], (0,), rng)
+    R = BTensor.random([r_leg.inverse(), m2.inverse(), r_leg], (0,), rng)
+    psi = BTensor.random([p, p, r_leg], (1,), rng)
+    return None, W1, W2, R, psi
+
+
+def random_rend_problem(rng, dl=(2, 5, 3), qns=(-1, 1, 3), wq=(0, 2, -2), wd=(2, 1, 1)):
+    l_leg = bond(IN, dl, qns)
+    p = spin_phys(OUT)
+    m1 = bond(OUT, wd, wq)
+    m2 = bond(OUT, wd, wq)
+    L = BTensor.random([l_leg.inverse(), m1, l_leg], (0,), rng)
+    W1 = BTensor.random([m1.inverse(), p.inverse(), p, m2], (0,), rng)
+    W2 = BTensor.random([p.inverse(), m2.inverse(), p], (0,), rng)
+    psi = BTensor.random([l_leg, p, p], (-1,), rng)
+    return L, W1, W2, None, psi
+
+
+def dense_heff_cent(L, W1, W2, R):
+    """Dense H_eff matrix (rows: output (l', p1', p2', r'), cols: input)."""
+    Ld, W1d, W2d, Rd = L.dense(), W1.dense(), W2.dense(), R.dense()
+    # H[(lb,t1,t2,rb),(l,p1,p2,r)] = L[l,b,lb] W1[b,p1,t1,b'] W2[b',p2,t2,b''] R[r,b'',rb]
+    H = np.einsum("abc,bdef,fghi,jik->cehkadgj", Ld, W1d, W2d, Rd)
+    n = Ld.shape[2] * W1d.shape[2] * W2d.shape[2] * Rd.shape[2]
+    m = Ld.shape[0] * W1d.shape[1] * W2d.shape[1] * Rd.shape[0]
+    return H.reshape(n, m)
+
+
+def max_rel_diff(a: BTensor, b: BTensor):
+    da, db = a.dense(), b.dense()
+    s = max(np.abs(db).max(), 1e-300)
+    return float(np.abs(da - db).max() / s)

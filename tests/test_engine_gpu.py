@@ -1,0 +1,169 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU FP64 oracle.
+
+Bar: identical block sets (bit-exact bookkeeping) and FP64 values within
+1e-12 relative (contractions reorder sums; no other difference is allowed).
+"""
+import numpy as np
+import pytest
+
+from mps2_amd import engine as E
+from mps2_amd.flat import BTensor, IN, OUT
+from oracle import oracle_py as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def same_blocks(a: BTensor, b: BTensor):
+    return sorted(a.blocks) == sorted(b.blocks) and [l.key() for l in a.legs] == [l.key() for l in b.legs]
+
+
+@pytest.mark.parametrize("case", ["trail_lead", "lead_lead", "mid_perm", "trail_trail", "a_perm", "scalar"])
+def test_contract_parity(ctx, case):
+    rng = np.random.default_rng(sum(map(ord, case)))
+    l0 = H.bond(IN, (17, 40, 9), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (33, 70, 5), (1, -1, 3))
+    A = BTensor.random([l0, p, r], (1,), rng)
+    if case == "trail_lead":
+        B = BTensor.random([r.inverse(), p, H.bond(OUT, (66, 130), (0, 2))], (0,), rng)
+        axes = [[2], [0]]
+    elif case == "lead_lead":
+        B = BTensor.random([l0.inverse(), p, H.bond(OUT, (20, 31), (0, 2))], (0,), rng)
+        axes = [[0], [0]]
+    elif case == "mid_perm":
+        B = BTensor.random([r.inverse(), H.bond(OUT, (2, 3), (1, 3)), p.inverse()], (0,), rng)
+        axes = [[1, 2], [2, 0]]
+    elif case == "trail_trail":
+        B = BTensor.random([H.bond(OUT, (7, 19), (0, 2)), p.inverse(), r.inverse()], (0,), rng)
+        axes = [[1, 2], [1, 2]]
+    elif case == "a_perm":
+        B = BTensor.random([p.inverse(), H.bond(OUT, (3, 8), (1, -1))], (0,), rng)
+        axes = [[1], [0]]
+    else:
+        B = A.dag()
+        axes = [[0, 1, 2], [0, 1, 2]]
+    ref = O.contract(A, B, axes)
+    dA, dB = ctx.put(A), ctx.put(B)
+    got = E.contract(dA, dB, axes).get()
+    assert same_blocks(got, ref)
+    assert H.max_rel_diff(got, ref) < TOL
+
+
+def test_contract_large_blocks(ctx):
+    # blocks above the 128x128 tile threshold, uneven edges
+    rng = np.random.default_rng(9)
+    l0 = H.bond(IN, (300, 157), (0, 2))
+    r = H.bond(OUT, (211, 129), (0, 2))
+    A = BTensor.random([l0, r], (0,), rng)
+    B = BTensor.random([r.inverse(), H.bond(OUT, (190, 77), (0, 2))], (0,), rng)
+    ref = O.contract(A, B, [[1], [0]])
+    got = E.contract(ctx.put(A), ctx.put(B), [[1], [0]]).get()
+    assert same_blocks(got, ref)
+    assert H.max_rel_diff(got, ref) < TOL
+
+
+@pytest.mark.parametrize("where", ["cent", "lend", "rend"])
+def test_heff_parity(ctx, where):
+    rng = np.random.default_rng(21)
+    prob = {"cent": H.random_cent_problem, "lend": H.random_lend_problem, "rend": H.random_rend_problem}[where](rng)
+    L, W1, W2, R, psi = prob
+    ref = O.heff_mul(L, W1, W2, R, psi, where)
+    d = [ctx.put(x) if x is not None else None for x in prob]
+    got = E.heff_mul(*d, where).get()
+    # the device result lives on the full block layout; compare densely
+    assert np.abs(got.dense() - ref.dense()).max() <= TOL * np.abs(ref.dense()).max()
+
+
+def test_lanczos_parity(ctx):
+    rng = np.random.default_rng(31)
+    L, W1, W2, R, psi = H.hermitian_cent_problem(rng, dl=(6, 9, 5), dr=(4, 10, 6))
+    it_o, e_o, gs_o = O.lanczos(L, W1, W2, R, psi, "cent", 1e-10, 200)
+    d = [ctx.put(x) for x in (L, W1, W2, R, psi)]
+    it_g, e_g, gs = E.lanczos(*d, "cent", 1e-10, 200, E.PREC_FP64)
+    gs_g = gs.get()
+    assert it_g == it_o
+    assert abs(e_g - e_o) < 1e-12 * max(1, abs(e_o))
+    ov = float((gs_g.dense() * gs_o.dense()).sum())
+    assert abs(abs(ov) - 1) < 1e-10
+    Hd = H.dense_heff_cent(L, W1, W2, R)
+    x = gs_g.dense().ravel()
+    assert abs(x @ Hd @ x - e_g) < 1e-8
+
+
+def test_lanczos_max_iter_mode(ctx):
+    rng = np.random.default_rng(33)
+    L, W1, W2, R, psi = H.hermitian_cent_problem(rng)
+    it_o, e_o, _ = O.lanczos(L, W1, W2, R, psi, "cent", 1e-16, 20)
+    d = [ctx.put(x) for x in (L, W1, W2, R, psi)]
+    it_g, e_g, _ = E.lanczos(*d, "cent", 1e-16, 20, E.PREC_FP64)
+    assert it_g == it_o == 19
+    assert abs(e_g - e_o) < 1e-11
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 200])
+def test_tridiag_parity(ctx, n):
+    rng = np.random.default_rng(n)
+    a, b = rng.normal(size=n), rng.normal(size=n - 1)
+    e_o, v_o = O.tridiag_lowest(a, b)
+    e_g, v_g = ctx.tridiag_lowest(a, b)
+    assert abs(e_g - e_o) < 1e-13 * max(1, abs(e_o))
+    T = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+    assert np.linalg.norm(T @ v_g - e_g * v_g) < 1e-11
+    assert abs(abs(float(v_g @ v_o)) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+@pytest.mark.parametrize("dims", [(3, 2, 4), (40, 70, 25)])
+def test_svd_parity(ctx, orth, dims):
+    rng = np.random.default_rng(sum(dims) + orth)
+    l0 = H.bond(IN, dims, (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, tuple(d + 1 for d in dims) + (2,), (0, 2, -2, 4))
+    psi = BTensor.random([l0, p, p, r], (0,), rng)
+    dpsi = ctx.put(psi)
+    for dmax in (10_000, 7, max(3, sum(dims) // 2)):
+        Uo, So, Vo, eo, Do = O.svd(psi, 2, (0,), 0.0, 1, dmax)
+        u, s, vt, eg, Dg = E.svd(dpsi, 2, (0,), 0.0, 1, dmax, orth)
+        Ug, Sg, Vg = u.get(), s.get(), vt.get()
+        assert Dg == Do
+        # bit-exact bookkeeping: identical new-bond sectors and kept counts
+        assert Ug.legs[-1].key() == Uo.legs[-1].key()
+        assert sorted(Sg.blocks) == sorted(So.blocks)
+        so, sg = np.diag(So.dense()), np.diag(Sg.dense())
+        assert np.abs(so - sg).max() < 1e-10 * so.max()
+        assert abs(eg - eo) < 1e-10
+        # the product the sweep uses is exact; the orthonormal side is orthonormal
+        if orth == E.ORTH_LEFT:
+            u2 = Ug.dense().reshape(-1, Ug.dense().shape[-1])
+            assert np.abs(u2.T @ u2 - np.eye(u2.shape[1])).max() < 1e-12
+            svt = np.einsum("kl,lcd->kcd", Sg.dense(), Vg.dense())
+            proj = np.einsum("abk,abcd->kcd", Ug.dense(), psi.dense())
+            assert np.abs(svt - proj).max() < 1e-11
+        else:
+            v2 = Vg.dense().reshape(Vg.dense().shape[0], -1)
+            assert np.abs(v2 @ v2.T - np.eye(v2.shape[0])).max() < 1e-12
+            us = np.einsum("abk,kl->abl", Ug.dense(), Sg.dense())
+            proj = np.einsum("abcd,lcd->abl", psi.dense(), Vg.dense())
+            assert np.abs(us - proj).max() < 1e-11
+        if dmax == 10_000:
+            rec = np.einsum("abk,kl,lcd->abcd", Ug.dense(), Sg.dense(), Vg.dense())
+            assert np.abs(rec - psi.dense()).max() < 1e-11
+
+
+def test_linear_combine_and_normalize(ctx):
+    rng = np.random.default_rng(41)
+    l0 = H.bond(IN, (5, 6), (0, 2))
+    p = H.spin_phys(OUT)
+    x = BTensor.random([l0, p, l0.inverse()], (1,), rng)
+    y = BTensor.random([l0, p, l0.inverse()], (1,), rng)
+    dx, dy = ctx.put(x), ctx.put(y)
+    out = dx.copy()
+    E.linear_combine([0.5, -2.0], [dx, dy], 3.0, out)
+    ref = 3.0 * x.dense() + 0.5 * x.dense() - 2.0 * y.dense()
+    assert np.abs(out.get().dense() - ref).max() < 1e-13
+    n = out.normalize()
+    assert abs(n - np.linalg.norm(ref)) < 1e-12 * n
+    assert abs(E.dot(out, out) - 1) < 1e-14

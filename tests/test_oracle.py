@@ -1,0 +1,150 @@
+"""Pins of the CPU FP64 oracle (no GPU).
+
+Each check is chosen so that a plausible slip (dropped term, wrong sign or
+index, transposed operand) fails it: dense numpy einsum on small block-sparse
+operands, closed forms, and dense eigensolvers.
+"""
+import numpy as np
+import pytest
+
+from mps2_amd.flat import BTensor, IN, OUT
+from oracle import oracle_py as O
+from tests import helpers as H
+
+
+@pytest.mark.parametrize("axes,spec", [
+    ([[2], [0]], "abc,cde->abde"),
+    ([[0], [0]], "abc,ade->bcde"),
+    ([[2, 1], [0, 1]], "abc,cbe->ae"),
+    ([[0, 2], [2, 0]], "abc,cda->bd"),
+])
+def test_contract_matches_dense_einsum(axes, spec):
+    rng = np.random.default_rng(3)
+    l0 = H.bond(IN, (3, 2, 4), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (3, 2, 1), (1, -1, 3))
+    A = BTensor.random([l0, p, r], (1,), rng)
+    if axes == [[2], [0]]:
+        B = BTensor.random([r.inverse(), p, H.bond(OUT, (2, 3), (0, 2))], (0,), rng)
+    elif axes == [[0], [0]]:
+        B = BTensor.random([l0.inverse(), p, H.bond(OUT, (2, 3), (0, 2))], (0,), rng)
+    elif axes == [[2, 1], [0, 1]]:
+        B = BTensor.random([r.inverse(), p.inverse(), H.bond(OUT, (2, 3, 2), (0, 2, 4))], (0,), rng)
+    else:
+        B = BTensor.random([r.inverse(), H.bond(OUT, (2, 3), (1, 3)), l0.inverse()], (0,), rng)
+    C = O.contract(A, B, axes)
+    ref = np.einsum(spec, A.dense(), B.dense())
+    assert np.abs(C.dense() - ref).max() < 1e-13
+    # output blocks: exactly the coordinates that receive a block-pair product
+    for c, blk in C.blocks.items():
+        assert blk.shape == C.block_shape(c)
+
+
+def test_contract_rejects_non_inverse_legs():
+    rng = np.random.default_rng(0)
+    p = H.spin_phys(OUT)
+    A = BTensor.random([p, p.inverse()], (0,), rng)
+    with pytest.raises(RuntimeError):
+        O.contract(A, A, [[0], [0]])
+
+
+def test_tridiag_toeplitz_closed_form():
+    # lambda_k = a + 2 b cos(k pi / (n + 1)); lowest is k = n for b > 0
+    n, a, b = 37, 0.3, 0.7
+    ev, v = O.tridiag_lowest(np.full(n, a), np.full(n - 1, b))
+    lam = a + 2 * b * np.cos(n * np.pi / (n + 1))
+    assert abs(ev - lam) < 1e-14
+    k = np.arange(1, n + 1)
+    ref = np.sin(k * n * np.pi / (n + 1))
+    ref /= np.linalg.norm(ref)
+    assert min(np.abs(v - ref).max(), np.abs(v + ref).max()) < 1e-12
+
+
+def test_tridiag_random_vs_eigh():
+    rng = np.random.default_rng(7)
+    for n in (1, 2, 5, 60, 200):
+        a, b = rng.normal(size=n), rng.normal(size=max(n - 1, 0))
+        ev, v = O.tridiag_lowest(a, b)
+        T = np.diag(a) + np.diag(b, 1) + np.diag(b, -1)
+        w, V = np.linalg.eigh(T)
+        assert abs(ev - w[0]) < 1e-12 * max(1, abs(w).max())
+        assert np.linalg.norm(T @ v - ev * v) < 1e-11
+
+
+def test_svd_reconstructs_and_truncates():
+    rng = np.random.default_rng(5)
+    l0 = H.bond(IN, (3, 2, 4), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (3, 2, 1, 2), (0, 2, -2, 4))
+    psi = BTensor.random([l0, p, p, r], (0,), rng)
+    U, S, Vt, err, D = O.svd(psi, 2, (0,), 0.0, 1000, 1000)
+    rec = np.einsum("abk,kl,lcd->abcd", U.dense(), S.dense(), Vt.dense())
+    assert np.abs(rec - psi.dense()).max() < 1e-13
+    u = U.dense().reshape(-1, U.dense().shape[-1])
+    assert np.abs(u.T @ u - np.eye(u.shape[1])).max() < 1e-13
+    v = Vt.dense().reshape(Vt.dense().shape[0], -1)
+    assert np.abs(v @ v.T - np.eye(v.shape[0])).max() < 1e-13
+    # singular values of the QN-blocked matrix equal those of the dense matrix
+    dense_s = np.linalg.svd(psi.dense().reshape(psi.dense().shape[0] * 2, -1), compute_uv=False)
+    ours = np.sort(np.diag(S.dense()))[::-1]
+    assert np.abs(ours - dense_s[:len(ours)]).max() < 1e-13
+    # truncation error = discarded weight
+    U5, S5, V5, err5, D5 = O.svd(psi, 2, (0,), 0.0, 5, 5)
+    assert D5 == 5
+    assert abs(err5 - (dense_s[5:] ** 2).sum() / (dense_s ** 2).sum()) < 1e-13
+    # trunc_err drives D when Dmax does not bind
+    target = (dense_s[7:] ** 2).sum() / (dense_s ** 2).sum() * 1.0000001
+    _, _, _, e7, D7 = O.svd(psi, 2, (0,), target, 1, 1000)
+    assert D7 == 7 and e7 <= target
+
+
+def test_svd_degenerate_cut_is_deterministic():
+    # two sectors with identical spectra: the cut takes the lower-QN sector first
+    q = H.bond(IN, (2, 2), (-1, 1))
+    out = H.bond(OUT, (2, 2), (-1, 1))
+    t = BTensor([q, out], 1)
+    t.blocks[(0, 0)] = np.diag([3.0, 1.0])
+    t.blocks[(1, 1)] = np.diag([3.0, 1.0])
+    U, S, Vt, err, D = O.svd(t, 1, (0,), 0.0, 1, 3)
+    assert D == 3
+    dims = [d for _, d in U.legs[-1].sectors]
+    qns = [qq for qq, _ in U.legs[-1].sectors]
+    assert qns == [(-1,), (1,)] and dims == [2, 1]
+
+
+def test_lanczos_matches_dense_eigh():
+    rng = np.random.default_rng(11)
+    L, W1, W2, R, psi = H.hermitian_cent_problem(rng)
+    Hd = H.dense_heff_cent(L, W1, W2, R)
+    assert np.abs(Hd - Hd.T).max() < 1e-12
+    # restrict to the divergence-0 subspace spanned by psi's allowed blocks
+    it, e, gs = O.lanczos(L, W1, W2, R, psi, "cent", 1e-12, 200)
+    w = np.linalg.eigvalsh(Hd)
+    x = gs.dense().ravel()
+    assert abs(np.linalg.norm(x) - 1) < 1e-10
+    rq = x @ Hd @ x
+    assert abs(rq - e) < 1e-8
+    assert e >= w[0] - 1e-8
+    # max_iter termination mode (test_lanczos_solver.cc:150-168)
+    it2, e2, _ = O.lanczos(L, W1, W2, R, psi, "cent", 1e-16, 20)
+    assert it2 == 19
+
+
+def test_heff_hermiticity_invariant():
+    rng = np.random.default_rng(2)
+    L, W1, W2, R, psi = H.hermitian_cent_problem(rng)
+    x = psi
+    y = BTensor.random(psi.legs, (0,), rng)
+    hx = O.heff_mul(L, W1, W2, R, x, "cent")
+    hy = O.heff_mul(L, W1, W2, R, y, "cent")
+    lhs = float((y.dense() * hx.dense()).sum())
+    rhs = float((hy.dense() * x.dense()).sum())
+    assert abs(lhs - rhs) < 1e-11 * max(1, abs(lhs))
+
+
+def test_heff_cent_matches_dense():
+    rng = np.random.default_rng(4)
+    L, W1, W2, R, psi = H.random_cent_problem(rng)
+    hp = O.heff_mul(L, W1, W2, R, psi, "cent")
+    ref = np.einsum("abc,bdef,fghi,jik,adgj->cehk", L.dense(), W1.dense(), W2.dense(), R.dense(), psi.dense())
+    assert np.abs(hp.dense() - ref).max() < 1e-12
