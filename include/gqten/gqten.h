@@ -1,0 +1,613 @@
+// gqten-compatible tensor API over the mps2_amd device engine.
+//
+// Mirrors the subset of GraceQ/tensor used by GraceQ/MPS2's hot path
+// (QN / QNCard / U1QNVal, QNSector, Index, GQTensor<GQTEN_Double, QNT>,
+// Contract, SVD, LinearCombine, Dag, Div, Timer; call sites listed in
+// include/mps2_amd.h) so that code written against the reference compiles
+// unchanged.  Small tensors built element by element (operators, MPO tensors,
+// initial MPS) live on the host until an operation needs them on the device;
+// every operation runs on the GPU through the C ABI.  Real (GQTEN_Double)
+// element type only.
+#ifndef MPS2_AMD_GQTEN_H
+#define MPS2_AMD_GQTEN_H
+
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../mps2_amd.h"
+
+namespace gqten {
+
+using GQTEN_Double = double;
+
+enum class GQTenIndexDirType { IN = 0, OUT = 1, NDIR = 2 };
+
+// ---------------------------------------------------------------- errors / context
+inline void CheckMps2(int rc, const char *what) {
+  if (rc != MPS2_OK) {
+    throw std::runtime_error(std::string(what) + ": " + mps2_last_error());
+  }
+}
+
+// One engine context per process (device 0 unless MPS2_DEVICE is set).
+inline mps2_ctx DefaultCtx() {
+  static std::shared_ptr<mps2_ctx_s> ctx = [] {
+    int dev = 0;
+    if (const char *e = std::getenv("MPS2_DEVICE")) dev = std::atoi(e);
+    mps2_ctx c = nullptr;
+    CheckMps2(mps2_ctx_create(dev, &c), "mps2_ctx_create");
+    return std::shared_ptr<mps2_ctx_s>(c, [](mps2_ctx p) { mps2_ctx_destroy(p); });
+  }();
+  return ctx.get();
+}
+
+// ---------------------------------------------------------------- QN
+struct U1QNVal {
+  int val;
+  explicit U1QNVal(int v) : val(v) {}
+};
+
+struct QNCard {
+  std::string name;
+  int val;
+  QNCard(const std::string &n, const U1QNVal &v) : name(n), val(v.val) {}
+};
+
+template <typename... QNVals>
+class QN {
+ public:
+  static constexpr int kWidth = (int)sizeof...(QNVals);
+  static_assert(kWidth >= 1 && kWidth <= 4, "1 to 4 U(1) components");
+  QN() { vals_.fill(0); }
+  explicit QN(const std::vector<QNCard> &cards) {
+    vals_.fill(0);
+    if ((int)cards.size() != kWidth) throw std::invalid_argument("QN: wrong number of QNCards");
+    for (int i = 0; i < kWidth; ++i) {
+      vals_[i] = cards[i].val;
+      names_[i] = cards[i].name;
+    }
+  }
+  int operator[](int i) const { return vals_[i]; }
+  int &operator[](int i) { return vals_[i]; }
+  QN operator+(const QN &o) const {
+    QN r(*this);
+    for (int i = 0; i < kWidth; ++i) r.vals_[i] += o.vals_[i];
+    return r;
+  }
+  QN operator-(const QN &o) const {
+    QN r(*this);
+    for (int i = 0; i < kWidth; ++i) r.vals_[i] -= o.vals_[i];
+    return r;
+  }
+  QN operator-() const {
+    QN r(*this);
+    for (int i = 0; i < kWidth; ++i) r.vals_[i] = -r.vals_[i];
+    return r;
+  }
+  QN &operator+=(const QN &o) { return *this = *this + o; }
+  QN &operator-=(const QN &o) { return *this = *this - o; }
+  bool operator==(const QN &o) const { return vals_ == o.vals_; }
+  bool operator!=(const QN &o) const { return !(*this == o); }
+  bool operator<(const QN &o) const { return vals_ < o.vals_; }
+
+ private:
+  std::array<int, kWidth> vals_;
+  std::array<std::string, kWidth> names_;
+};
+
+template <typename QNT>
+class QNSector {
+ public:
+  QNSector() : dim_(0) {}
+  QNSector(const QNT &qn, size_t dim) : qn_(qn), dim_(dim) {}
+  const QNT &GetQn() const { return qn_; }
+  size_t dim() const { return dim_; }
+  size_t GetDegeneracy() const { return dim_; }
+  bool operator==(const QNSector &o) const { return qn_ == o.qn_ && dim_ == o.dim_; }
+  bool operator!=(const QNSector &o) const { return !(*this == o); }
+
+ private:
+  QNT qn_;
+  size_t dim_;
+};
+
+template <typename QNT>
+using QNSectorVec = std::vector<QNSector<QNT>>;
+
+template <typename QNT>
+class Index {
+ public:
+  Index() : dir_(GQTenIndexDirType::NDIR), dim_(0) {}
+  Index(const QNSectorVec<QNT> &s, GQTenIndexDirType d) : qnscts_(s), dir_(d), dim_(0) {
+    for (auto &x : qnscts_) dim_ += x.dim();
+  }
+  size_t dim() const { return dim_; }
+  GQTenIndexDirType GetDir() const { return dir_; }
+  const QNSectorVec<QNT> &GetQNScts() const { return qnscts_; }
+  size_t GetQNSctNum() const { return qnscts_.size(); }
+  const QNSector<QNT> &GetQNSct(size_t i) const { return qnscts_[i]; }
+  // Sector holding the actual (dense) coordinate.
+  const QNSector<QNT> &GetQNSctFromActualCoor(size_t coor) const {
+    size_t s, off;
+    Locate(coor, s, off);
+    return qnscts_[s];
+  }
+  void Locate(size_t coor, size_t &sct, size_t &off) const {
+    size_t acc = 0;
+    for (size_t i = 0; i < qnscts_.size(); ++i) {
+      if (coor < acc + qnscts_[i].dim()) {
+        sct = i;
+        off = coor - acc;
+        return;
+      }
+      acc += qnscts_[i].dim();
+    }
+    throw std::out_of_range("Index: coordinate out of range");
+  }
+  void Inverse() {
+    if (dir_ == GQTenIndexDirType::IN) dir_ = GQTenIndexDirType::OUT;
+    else if (dir_ == GQTenIndexDirType::OUT) dir_ = GQTenIndexDirType::IN;
+  }
+  bool operator==(const Index &o) const { return dir_ == o.dir_ && qnscts_ == o.qnscts_; }
+  bool operator!=(const Index &o) const { return !(*this == o); }
+
+ private:
+  QNSectorVec<QNT> qnscts_;
+  GQTenIndexDirType dir_;
+  size_t dim_;
+};
+
+template <typename QNT>
+using IndexVec = std::vector<Index<QNT>>;
+
+template <typename QNT>
+Index<QNT> InverseIndex(const Index<QNT> &idx) {
+  Index<QNT> r(idx);
+  r.Inverse();
+  return r;
+}
+
+// ---------------------------------------------------------------- device handle
+struct DevTen {
+  mps2_ten h = nullptr;
+  explicit DevTen(mps2_ten t) : h(t) {}
+  ~DevTen() {
+    if (h) mps2_ten_free(h);
+  }
+  DevTen(const DevTen &) = delete;
+  DevTen &operator=(const DevTen &) = delete;
+};
+
+// ---------------------------------------------------------------- GQTensor
+template <typename ElemT, typename QNT>
+class GQTensor {
+  static_assert(std::is_same<ElemT, double>::value, "this build supports GQTEN_Double tensors");
+
+ public:
+  using BlockKey = std::vector<int32_t>;
+
+  GQTensor() : is_default_(true) {}
+  explicit GQTensor(const IndexVec<QNT> &idxs) : indexes_(idxs), is_default_(false) {}
+
+  size_t Rank() const { return indexes_.size(); }
+  const IndexVec<QNT> &GetIndexes() const { return indexes_; }
+  const Index<QNT> &GetIndex(size_t i) const { return indexes_[i]; }
+  std::vector<size_t> GetShape() const {
+    std::vector<size_t> s;
+    for (auto &i : indexes_) s.push_back(i.dim());
+    return s;
+  }
+  bool IsDefault() const { return is_default_; }
+
+  // ---- element access (host side)
+  double &operator()(const std::vector<size_t> &coors) {
+    ToHost();
+    dev_.reset();
+    return ElemRef(coors);
+  }
+  template <typename... Ints>
+  double &operator()(size_t c0, Ints... cs) {
+    return (*this)(std::vector<size_t>{c0, (size_t)cs...});
+  }
+  double GetElem(const std::vector<size_t> &coors) const {
+    ToHost();
+    BlockKey k;
+    size_t off;
+    Addr(coors, k, off);
+    auto it = hblk_.find(k);
+    return it == hblk_.end() ? 0.0 : it->second[off];
+  }
+  // rank-0 scalar
+  double operator()() const {
+    ToHost();
+    auto it = hblk_.find(BlockKey());
+    return it == hblk_.end() ? 0.0 : it->second[0];
+  }
+
+  double Normalize() {
+    ToDevice();
+    double n = 0;
+    CheckMps2(mps2_normalize(dev_->h, &n), "Normalize");
+    hblk_.clear();
+    host_valid_ = false;
+    return n;
+  }
+
+  void Random(const QNT &div);
+  void Transpose(const std::vector<size_t> &perm);
+
+  bool operator==(const GQTensor &o) const {
+    if (is_default_ || o.is_default_) return is_default_ == o.is_default_;
+    if (indexes_ != o.indexes_) return false;
+    ToHost();
+    o.ToHost();
+    return hblk_ == o.hblk_;
+  }
+  bool operator!=(const GQTensor &o) const { return !(*this == o); }
+
+  // ---- engine interop
+  mps2_ten Handle() const {
+    ToDevice();
+    return dev_->h;
+  }
+  // Adopt a device tensor produced by the engine; indexes are rebuilt from it.
+  void AdoptDevice(mps2_ten h) {
+    dev_ = std::make_shared<DevTen>(h);
+    is_default_ = false;
+    hblk_.clear();
+    host_valid_ = false;
+    int64_t nm = 0;
+    CheckMps2(mps2_ten_export_sizes(h, &nm, nullptr), "export_sizes");
+    std::vector<int32_t> meta(nm);
+    CheckMps2(mps2_ten_export(h, meta.data(), nullptr), "export meta");
+    ParseLegs(meta);
+  }
+  bool OnDevice() const { return (bool)dev_; }
+  void ReleaseHost() const {
+    if (dev_) {
+      hblk_.clear();
+      host_valid_ = false;
+    }
+  }
+  // Flat exchange format (include/mps2_amd.h).
+  void ToFlat(std::vector<int32_t> &meta, std::vector<double> &data) const;
+  void FromFlat(const std::vector<int32_t> &meta, const std::vector<double> &data);
+
+  void ToDevice() const {
+    if (dev_) return;
+    if (is_default_) throw std::runtime_error("operation on a default-constructed GQTensor");
+    std::vector<int32_t> meta;
+    std::vector<double> data;
+    ToFlat(meta, data);
+    if (data.empty()) data.push_back(0.0);
+    mps2_ten h = nullptr;
+    CheckMps2(mps2_ten_import(DefaultCtx(), meta.data(), data.data(), &h), "upload");
+    dev_ = std::make_shared<DevTen>(h);
+  }
+  void ToHost() const {
+    if (host_valid_ || !dev_) return;
+    int64_t nm = 0, nd = 0;
+    CheckMps2(mps2_ten_export_sizes(dev_->h, &nm, &nd), "export_sizes");
+    std::vector<int32_t> meta(nm);
+    std::vector<double> data(std::max<int64_t>(nd, 1));
+    CheckMps2(mps2_ten_export(dev_->h, meta.data(), data.data()), "download");
+    LoadBlocks(meta, data);
+    host_valid_ = true;
+  }
+
+ private:
+  IndexVec<QNT> indexes_;
+  bool is_default_ = true;
+  mutable std::map<BlockKey, std::vector<double>> hblk_;
+  mutable bool host_valid_ = true;
+  mutable std::shared_ptr<DevTen> dev_;
+
+  size_t BlockSize(const BlockKey &k) const {
+    size_t s = 1;
+    for (size_t i = 0; i < k.size(); ++i) s *= indexes_[i].GetQNSct(k[i]).dim();
+    return s;
+  }
+  void Addr(const std::vector<size_t> &coors, BlockKey &k, size_t &off) const {
+    if (coors.size() != indexes_.size()) throw std::invalid_argument("GQTensor: wrong number of coordinates");
+    k.resize(coors.size());
+    off = 0;
+    for (size_t i = 0; i < coors.size(); ++i) {
+      size_t s, o;
+      indexes_[i].Locate(coors[i], s, o);
+      k[i] = (int32_t)s;
+      off = off * indexes_[i].GetQNSct(s).dim() + o;
+    }
+  }
+  double &ElemRef(const std::vector<size_t> &coors) {
+    BlockKey k;
+    size_t off;
+    Addr(coors, k, off);
+    auto &b = hblk_[k];
+    if (b.empty()) b.assign(BlockSize(k), 0.0);
+    return b[off];
+  }
+  void ParseLegs(const std::vector<int32_t> &m) {
+    int p = 0;
+    int rank = m[p++];
+    int qw = m[p++];
+    p++;
+    if (qw != QNT::kWidth) throw std::runtime_error("QN width mismatch");
+    indexes_.clear();
+    for (int l = 0; l < rank; ++l) {
+      auto dir = m[p++] == 1 ? GQTenIndexDirType::OUT : GQTenIndexDirType::IN;
+      int ns = m[p++];
+      QNSectorVec<QNT> secs;
+      for (int s = 0; s < ns; ++s) {
+        QNT q;
+        for (int k = 0; k < qw; ++k) q[k] = m[p++];
+        secs.emplace_back(q, (size_t)m[p++]);
+      }
+      indexes_.emplace_back(secs, dir);
+    }
+  }
+  void LoadBlocks(const std::vector<int32_t> &m, const std::vector<double> &data) const {
+    int p = 0;
+    int rank = m[p++];
+    int qw = m[p++];
+    int nblk = m[p++];
+    for (int l = 0; l < rank; ++l) {
+      p++;
+      int ns = m[p++];
+      p += ns * (qw + 1);
+    }
+    hblk_.clear();
+    size_t off = 0;
+    for (int b = 0; b < nblk; ++b) {
+      BlockKey k(m.begin() + p, m.begin() + p + rank);
+      p += rank;
+      size_t sz = BlockSize(k);
+      hblk_[k] = std::vector<double>(data.begin() + off, data.begin() + off + sz);
+      off += sz;
+    }
+  }
+};
+
+// Seeded generator behind GQTensor::Random (uniform in [0, 1)).
+inline std::mt19937_64 &RandomEngine() {
+  static std::mt19937_64 g(0x5EEDull);
+  return g;
+}
+inline void SetRandomSeed(uint64_t s) { RandomEngine().seed(s); }
+
+template <typename QNT>
+QNT BlockDiv(const IndexVec<QNT> &idx, const std::vector<int32_t> &k) {
+  QNT d;
+  for (size_t i = 0; i < idx.size(); ++i) {
+    const QNT &q = idx[i].GetQNSct(k[i]).GetQn();
+    if (idx[i].GetDir() == GQTenIndexDirType::OUT) d += q;
+    else d -= q;
+  }
+  return d;
+}
+
+template <typename ElemT, typename QNT>
+void GQTensor<ElemT, QNT>::Random(const QNT &div) {
+  ToHost();
+  dev_.reset();
+  hblk_.clear();
+  host_valid_ = true;
+  size_t r = indexes_.size();
+  BlockKey k(r, 0);
+  std::uniform_real_distribution<double> u(0.0, 1.0);
+  if (r == 0) return;
+  while (true) {
+    if (BlockDiv(indexes_, k) == div) {
+      auto &b = hblk_[k];
+      b.resize(BlockSize(k));
+      for (auto &x : b) x = u(RandomEngine());
+    }
+    int i = (int)r - 1;
+    for (; i >= 0; --i) {
+      if (++k[i] < (int32_t)indexes_[i].GetQNSctNum()) break;
+      k[i] = 0;
+    }
+    if (i < 0) break;
+  }
+}
+
+template <typename ElemT, typename QNT>
+void GQTensor<ElemT, QNT>::Transpose(const std::vector<size_t> &perm) {
+  ToHost();
+  dev_.reset();
+  IndexVec<QNT> ni;
+  for (auto p : perm) ni.push_back(indexes_[p]);
+  std::map<BlockKey, std::vector<double>> nb;
+  size_t r = perm.size();
+  for (auto &kv : hblk_) {
+    BlockKey k(r);
+    std::vector<size_t> od(r), nd(r);
+    for (size_t i = 0; i < r; ++i) od[i] = indexes_[i].GetQNSct(kv.first[i]).dim();
+    for (size_t i = 0; i < r; ++i) {
+      k[i] = kv.first[perm[i]];
+      nd[i] = od[perm[i]];
+    }
+    std::vector<double> out(kv.second.size());
+    std::vector<size_t> ostr(r, 1);
+    for (int i = (int)r - 2; i >= 0; --i) ostr[i] = ostr[i + 1] * od[i + 1];
+    std::vector<size_t> idx(r, 0);
+    for (size_t e = 0; e < out.size(); ++e) {
+      size_t so = 0;
+      for (size_t i = 0; i < r; ++i) so += idx[i] * ostr[perm[i]];
+      out[e] = kv.second[so];
+      for (int i = (int)r - 1; i >= 0; --i) {
+        if (++idx[i] < nd[i]) break;
+        idx[i] = 0;
+      }
+    }
+    nb[k] = std::move(out);
+  }
+  indexes_ = ni;
+  hblk_ = std::move(nb);
+}
+
+template <typename ElemT, typename QNT>
+void GQTensor<ElemT, QNT>::ToFlat(std::vector<int32_t> &meta, std::vector<double> &data) const {
+  ToHost();
+  meta.clear();
+  data.clear();
+  meta.push_back((int32_t)indexes_.size());
+  meta.push_back(QNT::kWidth);
+  meta.push_back((int32_t)hblk_.size());
+  for (auto &idx : indexes_) {
+    meta.push_back(idx.GetDir() == GQTenIndexDirType::OUT ? 1 : 0);
+    meta.push_back((int32_t)idx.GetQNSctNum());
+    for (auto &s : idx.GetQNScts()) {
+      for (int k = 0; k < QNT::kWidth; ++k) meta.push_back(s.GetQn()[k]);
+      meta.push_back((int32_t)s.dim());
+    }
+  }
+  for (auto &kv : hblk_)
+    for (auto c : kv.first) meta.push_back(c);
+  for (auto &kv : hblk_) data.insert(data.end(), kv.second.begin(), kv.second.end());
+}
+
+template <typename ElemT, typename QNT>
+void GQTensor<ElemT, QNT>::FromFlat(const std::vector<int32_t> &meta, const std::vector<double> &data) {
+  is_default_ = false;
+  dev_.reset();
+  ParseLegs(meta);
+  LoadBlocks(meta, data);
+  host_valid_ = true;
+}
+
+// ---------------------------------------------------------------- free functions
+template <typename T>
+void Contract(const T *a, const T *b, const std::vector<std::vector<size_t>> &axes, T *out) {
+  std::vector<int32_t> xa(axes[0].begin(), axes[0].end()), xb(axes[1].begin(), axes[1].end());
+  mps2_ten h = nullptr;
+  CheckMps2(mps2_contract(a->Handle(), b->Handle(), (int)xa.size(), xa.data(), xb.data(), &h), "Contract");
+  out->AdoptDevice(h);
+}
+
+template <typename ElemT, typename QNT>
+GQTensor<ElemT, QNT> Dag(const GQTensor<ElemT, QNT> &t) {
+  mps2_ten h = nullptr;
+  CheckMps2(mps2_ten_dag(t.Handle(), &h), "Dag");
+  GQTensor<ElemT, QNT> r;
+  r.AdoptDevice(h);
+  return r;
+}
+
+template <typename ElemT, typename QNT>
+QNT Div(const GQTensor<ElemT, QNT> &t) {
+  if (!t.OnDevice()) {
+    std::vector<int32_t> meta;
+    std::vector<double> data;
+    t.ToFlat(meta, data);
+    int r = meta[0], qw = meta[1], nb = meta[2];
+    QNT d;
+    if (nb == 0) return d;
+    int p = 3;
+    for (int l = 0; l < r; ++l) {
+      p++;
+      int ns = meta[p++];
+      p += ns * (qw + 1);
+    }
+    std::vector<int32_t> k(meta.begin() + p, meta.begin() + p + r);
+    return BlockDiv(t.GetIndexes(), k);
+  }
+  int32_t q[4] = {0, 0, 0, 0};
+  CheckMps2(mps2_ten_div(t.Handle(), q), "Div");
+  QNT d;
+  for (int i = 0; i < QNT::kWidth; ++i) d[i] = q[i];
+  return d;
+}
+
+// out = beta*out + sum_i coefs[i]*tens[i]
+template <typename ElemT, typename QNT>
+void LinearCombine(const std::vector<ElemT> &coefs, const std::vector<GQTensor<ElemT, QNT> *> &tens,
+                   const ElemT beta, GQTensor<ElemT, QNT> *out) {
+  std::vector<mps2_ten> hs;
+  for (auto *t : tens) hs.push_back(t->Handle());
+  if (out->IsDefault() || out->GetIndexes().empty()) {
+    // start from a zero tensor with the indexes of the first operand
+    mps2_ten z = nullptr;
+    CheckMps2(mps2_ten_copy(hs[0], &z), "LinearCombine");
+    GQTensor<ElemT, QNT> tmp;
+    tmp.AdoptDevice(z);
+    std::vector<double> zc{0.0};
+    mps2_ten h0 = tmp.Handle();
+    CheckMps2(mps2_linear_combine(0, zc.data(), nullptr, 0.0, h0), "LinearCombine");
+    *out = tmp;
+  }
+  std::vector<double> c(coefs.begin(), coefs.end());
+  CheckMps2(mps2_linear_combine((int)hs.size(), c.data(), hs.data(), beta, out->Handle()), "LinearCombine");
+  out->ReleaseHost();
+}
+
+template <typename ElemT, typename QNT>
+void LinearCombine(const size_t n, const ElemT *coefs, const std::vector<GQTensor<ElemT, QNT> *> &tens,
+                   const ElemT beta, GQTensor<ElemT, QNT> *out) {
+  std::vector<ElemT> c(coefs, coefs + n);
+  std::vector<GQTensor<ElemT, QNT> *> t(tens.begin(), tens.begin() + n);
+  LinearCombine(c, t, beta, out);
+}
+
+enum class SvdOrth { Left = MPS2_ORTH_LEFT, Right = MPS2_ORTH_RIGHT };
+
+// Truncated SVD with an explicit choice of the exactly-orthonormal factor.
+template <typename ElemT, typename QNT>
+void SVD(const GQTensor<ElemT, QNT> *t, const size_t ldims, const QNT &lqndiv, const double trunc_err,
+         const size_t Dmin, const size_t Dmax, GQTensor<ElemT, QNT> *u, GQTensor<double, QNT> *s,
+         GQTensor<ElemT, QNT> *vt, double *actual_trunc_err, size_t *D, SvdOrth orth) {
+  int32_t q[4] = {0, 0, 0, 0};
+  for (int i = 0; i < QNT::kWidth; ++i) q[i] = lqndiv[i];
+  mps2_ten hu = nullptr, hs = nullptr, hv = nullptr;
+  double err = 0;
+  int64_t d = 0;
+  CheckMps2(mps2_svd(t->Handle(), (int)ldims, q, trunc_err, (int64_t)Dmin, (int64_t)Dmax, (int)orth, &hu, &hs, &hv,
+                     &err, &d),
+            "SVD");
+  u->AdoptDevice(hu);
+  s->AdoptDevice(hs);
+  vt->AdoptDevice(hv);
+  if (actual_trunc_err) *actual_trunc_err = err;
+  if (D) *D = (size_t)d;
+}
+
+// gqten signature (U exactly orthonormal).
+template <typename ElemT, typename QNT>
+void SVD(const GQTensor<ElemT, QNT> *t, const size_t ldims, const QNT &lqndiv, const double trunc_err,
+         const size_t Dmin, const size_t Dmax, GQTensor<ElemT, QNT> *u, GQTensor<double, QNT> *s,
+         GQTensor<ElemT, QNT> *vt, double *actual_trunc_err, size_t *D) {
+  SVD(t, ldims, lqndiv, trunc_err, Dmin, Dmax, u, s, vt, actual_trunc_err, D, SvdOrth::Left);
+}
+
+// ---------------------------------------------------------------- Timer
+class Timer {
+ public:
+  explicit Timer(const std::string &notes) : notes_(notes), start_(Clock::now()) {}
+  void Restart() { start_ = Clock::now(); }
+  double Elapsed() const { return std::chrono::duration<double>(Clock::now() - start_).count(); }
+  double PrintElapsed(size_t precision = 5) const {
+    double e = Elapsed();
+    std::cout << "[timing] " << notes_ << "\t" << std::setw(precision + 3) << std::setprecision(precision)
+              << std::fixed << e << std::endl;
+    return e;
+  }
+
+ private:
+  using Clock = std::chrono::steady_clock;
+  std::string notes_;
+  Clock::time_point start_;
+};
+
+}  // namespace gqten
+
+#endif  // MPS2_AMD_GQTEN_H

@@ -274,9 +274,21 @@ double dot(const Ten &a, const Ten &b) {
 
 double norm(const Ten &a) { return std::sqrt(dot(a, a)); }
 
+// In-place writers call this first: a buffer shared with a Dag view or a
+// copied handle is duplicated before it is modified.
+void make_unique_buf(Ten &a) {
+  if (a.buf.use_count() > 1) {
+    auto r = ten_copy(a);
+    a.buf = r->buf;
+  }
+}
+
 double normalize(Ten &a) {
   double n = norm(a);
-  if (n > 0) k_scale(a.ctx, a.n(), 1.0 / n, a.d());
+  if (n > 0) {
+    make_unique_buf(a);
+    k_scale(a.ctx, a.n(), 1.0 / n, a.d());
+  }
   return n;
 }
 
@@ -300,6 +312,8 @@ void lincomb(const std::vector<double> &coefs, const std::vector<TenP> &ts, doub
     auto grown = embed(out, m);
     out->meta = grown->meta;
     out->buf = grown->buf;
+  } else {
+    make_unique_buf(*out);
   }
   if (beta != 1.0) k_scale(c, out->n(), beta, out->d());
   for (size_t i = 0; i < ts.size(); ++i) {

@@ -1,0 +1,413 @@
+// extern "C" driver over the gqmps2-compatible host layer
+// (declared in include/gqmps2_capi.h).
+#include "gqmps2_capi.h"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "gqmps2/gqmps2.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F f) {
+  try {
+    f();
+    return MPS2_OK;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return MPS2_ERR_ARG;
+  }
+}
+
+struct ModelBase {
+  virtual ~ModelBase() = default;
+  virtual int32_t add_op(int site, const double *dense) = 0;
+  virtual void add_term(double coef, int nops, const int32_t *ops, const int64_t *sites, int ninst,
+                        const int32_t *inst) = 0;
+  virtual std::vector<int64_t> gen_mpo() = 0;
+  virtual void export_mpo(int site, std::vector<int32_t> &meta, std::vector<double> &data) = 0;
+  virtual void mps_init(int nsets, const int64_t *labs, int64_t enlarged, uint64_t seed, int centralize) = 0;
+  virtual void export_mps(int site, std::vector<int32_t> &meta, std::vector<double> &data) = 0;
+  virtual void import_mps(int site, const int32_t *meta, const double *data) = 0;
+  virtual void checkpoint() = 0;
+  virtual void restore() = 0;
+  virtual double dmrg(int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *sched, double trunc_err, double lerr,
+                      int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te,
+                      double *ut, double *st) = 0;
+};
+
+template <typename QNT>
+struct Model : ModelBase {
+  using TenT = gqten::GQTensor<double, QNT>;
+  size_t N;
+  std::unique_ptr<gqmps2::SiteVec<double, QNT>> sites;
+  std::unique_ptr<gqmps2::MPOGenerator<double, QNT>> gen;
+  std::vector<TenT> ops;
+  std::vector<int> op_site;
+  gqmps2::MPO<TenT> mpo;
+  std::unique_ptr<gqmps2::FiniteMPS<double, QNT>> mps;
+  std::vector<TenT> saved;
+
+  Model(int nsites, const int32_t *nsect, const int32_t *qns, const int32_t *dims) : N(nsites) {
+    gqten::IndexVec<QNT> idx;
+    int p = 0;
+    for (int i = 0; i < nsites; ++i) {
+      gqten::QNSectorVec<QNT> secs;
+      for (int s = 0; s < nsect[i]; ++s) {
+        QNT q;
+        for (int k = 0; k < QNT::kWidth; ++k) q[k] = qns[p * QNT::kWidth + k];
+        secs.emplace_back(q, (size_t)dims[p]);
+        ++p;
+      }
+      idx.emplace_back(secs, gqten::GQTenIndexDirType::OUT);
+    }
+    sites = std::make_unique<gqmps2::SiteVec<double, QNT>>(idx);
+    gen = std::make_unique<gqmps2::MPOGenerator<double, QNT>>(*sites, QNT());
+    mps = std::make_unique<gqmps2::FiniteMPS<double, QNT>>(*sites);
+  }
+
+  int32_t add_op(int site, const double *dense) override {
+    if (site < 0 || (size_t)site >= N) throw std::out_of_range("site out of range");
+    auto pout = sites->sites[site];
+    auto pin = gqten::InverseIndex(pout);
+    TenT op({pin, pout});
+    size_t d = pout.dim();
+    for (size_t i = 0; i < d; ++i)
+      for (size_t j = 0; j < d; ++j)
+        if (dense[i * d + j] != 0.0) op({i, j}) = dense[i * d + j];
+    ops.push_back(op);
+    op_site.push_back(site);
+    return (int32_t)ops.size() - 1;
+  }
+
+  void add_term(double coef, int nops, const int32_t *oid, const int64_t *st, int ninst, const int32_t *inst) override {
+    std::vector<TenT> o;
+    std::vector<size_t> s;
+    for (int k = 0; k < nops; ++k) {
+      o.push_back(ops.at(oid[k]));
+      s.push_back((size_t)st[k]);
+    }
+    if (ninst == 0) {
+      gen->AddTerm(coef, o, s);
+    } else {
+      std::vector<TenT> io;
+      for (int k = 0; k < ninst; ++k) io.push_back(ops.at(inst[k]));
+      gen->AddTerm(coef, o, s, io);
+    }
+  }
+
+  std::vector<int64_t> gen_mpo() override {
+    mpo = gen->Gen();
+    std::vector<int64_t> bd;
+    for (size_t i = 0; i + 1 < N; ++i) {
+      size_t leg = i == 0 ? 1 : 3;
+      bd.push_back((int64_t)mpo[i].GetIndexes()[leg].dim());
+    }
+    return bd;
+  }
+
+  void export_mpo(int site, std::vector<int32_t> &meta, std::vector<double> &data) override {
+    mpo[site].ToFlat(meta, data);
+  }
+
+  void mps_init(int nsets, const int64_t *labs, int64_t enlarged, uint64_t seed, int centralize) override {
+    gqten::SetRandomSeed(seed);
+    std::vector<std::vector<size_t>> sets(nsets, std::vector<size_t>(N));
+    for (int k = 0; k < nsets; ++k)
+      for (size_t i = 0; i < N; ++i) sets[k][i] = (size_t)labs[k * N + i];
+    mps = std::make_unique<gqmps2::FiniteMPS<double, QNT>>(*sites);
+    if (centralize) {
+      if (enlarged == 0) gqmps2::DirectStateInitMps(*mps, sets[0], QNT());
+      else gqmps2::ExtendDirectRandomInitMps(*mps, sets, QNT(), (size_t)enlarged);
+    } else {
+      // same tensors, no Centralize(0): build through a site-local copy
+      InitRaw(sets, enlarged);
+    }
+  }
+
+  void InitRaw(const std::vector<std::vector<size_t>> &sets, int64_t enlarged) {
+    using IndexT = gqten::Index<QNT>;
+    using QNSctT = gqten::QNSector<QNT>;
+    auto pb = sites->sites;
+    size_t nf = sets.size();
+    size_t ed = enlarged == 0 ? 1 : (size_t)enlarged;
+    auto div = pb[0].GetQNSctFromActualCoor(sets[0][0]).GetQn();
+    for (size_t i = 1; i < N; ++i) div += pb[i].GetQNSctFromActualCoor(sets[0][i]).GetQn();
+    std::vector<QNSctT> secs;
+    for (size_t j = 0; j < nf; ++j) secs.push_back(QNSctT(div - pb[0].GetQNSctFromActualCoor(sets[j][0]).GetQn(), ed));
+    IndexT rvb(secs, gqten::GQTenIndexDirType::OUT), lvb;
+    secs.clear();
+    (*mps)(0) = new TenT({pb[0], rvb});
+    if (enlarged == 0) (*mps)[0]({sets[0][0], 0}) = 1;
+    else (*mps)[0].Random(div);
+    for (size_t i = 1; i + 1 < N; ++i) {
+      lvb = gqten::InverseIndex(rvb);
+      for (size_t j = 0; j < nf; ++j)
+        secs.push_back(QNSctT(QNT() - pb[i].GetQNSctFromActualCoor(sets[j][i]).GetQn() +
+                                  lvb.GetQNSctFromActualCoor(j * ed).GetQn(),
+                              ed));
+      rvb = IndexT(secs, gqten::GQTenIndexDirType::OUT);
+      secs.clear();
+      (*mps)(i) = new TenT({lvb, pb[i], rvb});
+      if (enlarged == 0) (*mps)[i]({0, sets[0][i], 0}) = 1;
+      else (*mps)[i].Random(QNT());
+    }
+    lvb = gqten::InverseIndex(rvb);
+    (*mps)(N - 1) = new TenT({lvb, pb[N - 1]});
+    if (enlarged == 0) (*mps)[N - 1]({0, sets[0][N - 1]}) = 1;
+    else (*mps)[N - 1].Random(QNT());
+  }
+
+  void export_mps(int site, std::vector<int32_t> &meta, std::vector<double> &data) override {
+    if (!(*mps)(site)) throw std::runtime_error("MPS tensor not resident");
+    (*mps)[site].ToFlat(meta, data);
+  }
+
+  void import_mps(int site, const int32_t *meta, const double *data) override {
+    std::vector<int32_t> m(meta, meta + 3);
+    // size the meta: rank, qw, nblk then legs then coords
+    int rank = meta[0], qw = meta[1], nb = meta[2];
+    int p = 3;
+    for (int l = 0; l < rank; ++l) {
+      int ns = meta[p + 1];
+      p += 2 + ns * (qw + 1);
+    }
+    p += nb * rank;
+    std::vector<int32_t> mm(meta, meta + p);
+    TenT tmp;
+    tmp.FromFlat(mm, {});
+    int64_t total = 0;
+    {
+      std::vector<int32_t> mt;
+      std::vector<double> dt;
+      // count elements from the block list
+      int q = 3;
+      std::vector<std::vector<int64_t>> dims(rank);
+      for (int l = 0; l < rank; ++l) {
+        int ns = meta[q + 1];
+        q += 2;
+        for (int s = 0; s < ns; ++s) {
+          q += qw;
+          dims[l].push_back(meta[q++]);
+        }
+      }
+      for (int b = 0; b < nb; ++b) {
+        int64_t sz = 1;
+        for (int l = 0; l < rank; ++l) sz *= dims[l][meta[q++]];
+        total += sz;
+      }
+    }
+    std::vector<double> d(data, data + total);
+    TenT t;
+    t.FromFlat(mm, d);
+    if ((*mps)(site)) mps->dealloc(site);
+    (*mps)(site) = new TenT(t);
+  }
+
+  void checkpoint() override {
+    saved.clear();
+    for (size_t i = 0; i < N; ++i) {
+      TenT c;
+      mps2_ten h = nullptr;
+      gqten::CheckMps2(mps2_ten_copy((*mps)[i].Handle(), &h), "checkpoint");
+      c.AdoptDevice(h);
+      saved.push_back(std::move(c));
+    }
+  }
+
+  void restore() override {
+    if (saved.size() != N) throw std::runtime_error("no checkpoint");
+    for (size_t i = 0; i < N; ++i) {
+      mps2_ten h = nullptr;
+      gqten::CheckMps2(mps2_ten_copy(saved[i].Handle(), &h), "restore");
+      TenT c;
+      c.AdoptDevice(h);
+      (*mps)[i] = std::move(c);
+    }
+  }
+
+  double dmrg(int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *sched, double trunc_err, double lerr,
+              int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te, double *ut,
+              double *st) override {
+    for (size_t i = 0; i < N; ++i)
+      if (!(*mps)(i)) throw std::runtime_error("MPS not initialised");
+    gqmps2::LanczosParams lp(lerr, (size_t)lmax,
+                             prec == MPS2_PREC_MIXED ? gqmps2::LanczosPrecision::Mixed : gqmps2::LanczosPrecision::FP64);
+    double e0 = 0;
+    int64_t k = 0;
+    auto record = [&](const gqmps2::UpdateRecord &r) {
+      if (en) en[k] = r.energy;
+      if (dims) dims[k] = (int64_t)r.D;
+      if (its) its[k] = (int64_t)r.iters;
+      if (te) te[k] = r.trunc_err;
+      if (ut) ut[k] = r.update_time;
+      ++k;
+    };
+    std::unique_ptr<gqmps2::TwoSiteVMPSEngine<double, QNT>> eng;
+    for (int64_t s = 0; s < sweeps; ++s) {
+      int64_t dm = sched ? sched[s] : dmax;
+      gqmps2::SweepParams sp((size_t)1, (size_t)dmin, (size_t)dm, trunc_err, lp);
+      auto *old = eng.release();
+      eng = std::make_unique<gqmps2::TwoSiteVMPSEngine<double, QNT>>(*mps, mpo, sp);
+      if (old) {
+        eng->renvs() = std::move(old->renvs());
+        eng->lenvs() = std::move(old->lenvs());
+        delete old;
+      } else {
+        eng->InitEnvs();
+      }
+      eng->quiet = !verbose;
+      eng->on_update = record;
+      gqten::Timer t("sweep");
+      e0 = eng->Sweep((size_t)s + 1);
+      gqten::CheckMps2(mps2_ctx_sync(gqten::DefaultCtx()), "sync");
+      if (st) st[s] = t.Elapsed();
+      if (verbose) t.PrintElapsed();
+    }
+    return e0;
+  }
+};
+
+}  // namespace
+
+struct gqmps2_model_s {
+  std::unique_ptr<ModelBase> m;
+};
+
+extern "C" {
+
+int gqmps2_model_create(int qn_width, int nsites, const int32_t *nsect, const int32_t *qns, const int32_t *dims,
+                        gqmps2_model *out) {
+  return guard([&] {
+    if (!out || !nsect || !qns || !dims) throw std::invalid_argument("NULL argument");
+    if (nsites < 2) throw std::invalid_argument("at least two sites are required");
+    auto h = new gqmps2_model_s;
+    if (qn_width == 1) h->m = std::make_unique<Model<gqten::QN<gqten::U1QNVal>>>(nsites, nsect, qns, dims);
+    else if (qn_width == 2)
+      h->m = std::make_unique<Model<gqten::QN<gqten::U1QNVal, gqten::U1QNVal>>>(nsites, nsect, qns, dims);
+    else {
+      delete h;
+      throw std::invalid_argument("qn_width must be 1 or 2");
+    }
+    *out = h;
+  });
+}
+
+int gqmps2_model_destroy(gqmps2_model m) {
+  return guard([&] { delete m; });
+}
+
+int gqmps2_model_add_op(gqmps2_model m, int site, const double *dense, int32_t *op_id) {
+  return guard([&] {
+    if (!m || !dense || !op_id) throw std::invalid_argument("NULL argument");
+    *op_id = m->m->add_op(site, dense);
+  });
+}
+
+int gqmps2_model_add_term(gqmps2_model m, double coef, int nops, const int32_t *op_ids, const int64_t *sites,
+                          int ninst, const int32_t *inst_op_ids) {
+  return guard([&] {
+    if (!m || !op_ids || !sites) throw std::invalid_argument("NULL argument");
+    m->m->add_term(coef, nops, op_ids, sites, ninst, inst_op_ids);
+  });
+}
+
+int gqmps2_model_gen_mpo(gqmps2_model m, int64_t *bond_dims) {
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    auto bd = m->m->gen_mpo();
+    if (bond_dims)
+      for (size_t i = 0; i < bd.size(); ++i) bond_dims[i] = bd[i];
+  });
+}
+
+int gqmps2_model_mpo_export_sizes(gqmps2_model m, int site, int64_t *nmeta, int64_t *ndata) {
+  return guard([&] {
+    std::vector<int32_t> meta;
+    std::vector<double> data;
+    m->m->export_mpo(site, meta, data);
+    if (nmeta) *nmeta = (int64_t)meta.size();
+    if (ndata) *ndata = (int64_t)data.size();
+  });
+}
+
+int gqmps2_model_mpo_export(gqmps2_model m, int site, int32_t *meta, double *data) {
+  return guard([&] {
+    std::vector<int32_t> mt;
+    std::vector<double> dt;
+    m->m->export_mpo(site, mt, dt);
+    if (meta) std::copy(mt.begin(), mt.end(), meta);
+    if (data) std::copy(dt.begin(), dt.end(), data);
+  });
+}
+
+int gqmps2_mps_init(gqmps2_model m, int nsets, const int64_t *stat_labs, int64_t enlarged_dim, uint64_t seed,
+                    int centralize) {
+  return guard([&] {
+    if (!m || !stat_labs || nsets < 1) throw std::invalid_argument("bad arguments");
+    m->m->mps_init(nsets, stat_labs, enlarged_dim, seed, centralize);
+  });
+}
+
+int gqmps2_mps_export_sizes(gqmps2_model m, int site, int64_t *nmeta, int64_t *ndata) {
+  return guard([&] {
+    std::vector<int32_t> meta;
+    std::vector<double> data;
+    m->m->export_mps(site, meta, data);
+    if (nmeta) *nmeta = (int64_t)meta.size();
+    if (ndata) *ndata = (int64_t)data.size();
+  });
+}
+
+int gqmps2_mps_export(gqmps2_model m, int site, int32_t *meta, double *data) {
+  return guard([&] {
+    std::vector<int32_t> mt;
+    std::vector<double> dt;
+    m->m->export_mps(site, mt, dt);
+    if (meta) std::copy(mt.begin(), mt.end(), meta);
+    if (data) std::copy(dt.begin(), dt.end(), data);
+  });
+}
+
+int gqmps2_mps_import(gqmps2_model m, int site, const int32_t *meta, const double *data) {
+  return guard([&] {
+    if (!m || !meta) throw std::invalid_argument("NULL argument");
+    m->m->import_mps(site, meta, data);
+  });
+}
+
+int gqmps2_mps_checkpoint(gqmps2_model m) {
+  return guard([&] { m->m->checkpoint(); });
+}
+
+int gqmps2_mps_restore(gqmps2_model m) {
+  return guard([&] { m->m->restore(); });
+}
+
+int gqmps2_dmrg(gqmps2_model m, int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *dmax_schedule,
+                double trunc_err, double lanczos_error, int64_t lanczos_max_iter, int precision, int verbose,
+                double *energies, int64_t *dims, int64_t *iters, double *trunc_errs, double *update_times,
+                double *sweep_times, double *final_energy) {
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    double e = m->m->dmrg(sweeps, dmin, dmax, dmax_schedule, trunc_err, lanczos_error, lanczos_max_iter, precision,
+                          verbose, energies, dims, iters, trunc_errs, update_times, sweep_times);
+    if (final_energy) *final_energy = e;
+  });
+}
+
+mps2_ctx gqmps2_engine_ctx(void) {
+  try {
+    return gqten::DefaultCtx();
+  } catch (...) {
+    return nullptr;
+  }
+}
+
+const char *gqmps2_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

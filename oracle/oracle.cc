@@ -755,6 +755,8 @@ LanczosOut lanczos(const Ten *L, const Ten *W1, const Ten *W2, const Ten *R, con
       std::vector<double> z;
       if (tridiag_lowest((int)m, a.data(), b.data(), &ev, &z)) throw std::runtime_error("?stev error.");
       Ten gs;
+      gs.qw = init.qw;
+      gs.legs = bases[0].legs;
       std::vector<const Ten *> ps;
       for (int64_t i = 0; i < m; ++i) ps.push_back(&bases[i]);
       lincomb(z, ps, 0.0, gs);

@@ -1,0 +1,145 @@
+"""Host-side pieces without a GPU: MPOGenerator, MPS initialisation and the
+oracle's two-site sweep against the reference's golden energies.
+
+Pins: (1) every generated MPO contracts to the same dense Hamiltonian as an
+independent numpy Kronecker sum of the same terms; (2) exact diagonalisation
+of that dense Hamiltonian reproduces the reference's golden energies; (3) the
+oracle sweep, started from the product MPS initialisation, reproduces them.
+"""
+import numpy as np
+import pytest
+
+from mps2_amd import hamiltonians as HM
+from mps2_amd.flat import BTensor
+from oracle import oracle_py as O
+
+
+def mpo_dense(model):
+    """H[out..., in...] from the MPO tensors (W_0=(p_in,b,p_out), W_i=(b_l,p_in,p_out,b_r), W_N-1=(p_in,b_l,p_out))."""
+    N = model.N
+    Ws = [model.mpo(i).dense() for i in range(N)]
+    # running tensor R[b, out..., in...]
+    W0 = Ws[0]                       # (i, b, j)
+    R = np.transpose(W0, (1, 2, 0))  # (b, j, i)
+    d = [model.site_dims[0]]
+    for k in range(1, N):
+        W = Ws[k]
+        if k == N - 1:
+            W = np.transpose(W, (1, 0, 2))[..., None]   # (b_l, p_in, p_out, 1)
+        # R (b, J, I) ; W (b, i, j, b')
+        R = np.einsum("bJI,bijc->cJjIi", R.reshape(R.shape[0], int(np.prod(d)), int(np.prod(d))), W)
+        d.append(model.site_dims[k])
+        R = R.reshape(R.shape[0], int(np.prod(d)), int(np.prod(d)))
+    return R[0]
+
+
+def kron_terms_dense(N, dims, terms):
+    """terms: list of (coef, [(site, op_in_out_matrix)]) with identity elsewhere."""
+    D = int(np.prod(dims))
+    H = np.zeros((D, D))
+    for coef, ops in terms:
+        mats = [np.eye(d) for d in dims]
+        for s, op in ops:
+            mats[s] = mats[s] @ np.asarray(op).T   # M[out, in] = op[in, out]
+        full = mats[0]
+        for m in mats[1:]:
+            full = np.kron(full, m)
+        H += coef * full
+    return H
+
+
+def heis_terms(pairs, J=1.0):
+    t = []
+    for (i, j) in pairs:
+        t += [(J, [(i, HM.SZ), (j, HM.SZ)]), (0.5 * J, [(i, HM.SP), (j, HM.SM)]), (0.5 * J, [(i, HM.SM), (j, HM.SP)])]
+    return t
+
+
+def test_heisenberg_mpo_matches_kron_sum():
+    N = 6
+    m = HM.heisenberg_chain(N)
+    assert list(m.bond_dims) == [5] * (N - 1)
+    H = kron_terms_dense(N, [2] * N, heis_terms([(i, i + 1) for i in range(N - 1)]))
+    assert np.abs(mpo_dense(m) - H).max() < 1e-13
+
+
+def test_j1j2_cylinder_mpo_matches_kron_sum():
+    Lx, W = 3, 3
+    m = HM.j1j2_cylinder(Lx, W, 1.0, 0.5)
+    j1, j2 = HM.j1j2_cylinder_bonds(Lx, W)
+    H = kron_terms_dense(Lx * W, [2] * (Lx * W), heis_terms(j1, 1.0) + heis_terms(j2, 0.5))
+    assert np.abs(mpo_dense(m) - H).max() < 1e-12
+
+
+def fermion_terms_with_strings(pairs_ops, N, f):
+    """Expand (coef, opA, i, opB, j) with the insertion operator f on the sites strictly between."""
+    t = []
+    for coef, A, i, B, j in pairs_ops:
+        ops = [(i, A)] + [(k, f) for k in range(i + 1, j)] + [(j, B)]
+        t.append((coef, ops))
+    return t
+
+
+def test_tj_mpo_matches_kron_sum():
+    o = HM.TJ
+    pairs = [(0, 1), (0, 2), (2, 3), (1, 3)]
+    m = HM.tj_model(4, pairs)
+    spec = []
+    for (i, j) in pairs:
+        spec += [(-3.0, o["cdagup"], i, o["cup"], j), (-3.0, o["cdagdn"], i, o["cdn"], j),
+                 (-3.0, o["cup"], i, o["cdagup"], j), (-3.0, o["cdn"], i, o["cdagdn"], j)]
+    terms = fermion_terms_with_strings(spec, 4, o["f"])
+    for (i, j) in pairs:
+        terms += [(1.0, [(i, o["sz"]), (j, o["sz"])]), (0.5, [(i, o["sp"]), (j, o["sm"])]),
+                  (0.5, [(i, o["sm"]), (j, o["sp"])])]
+    H = kron_terms_dense(4, [3] * 4, terms)
+    assert np.abs(mpo_dense(m) - H).max() < 1e-13
+
+
+def sector_ground_energy(H, site_qns, total):
+    """Lowest eigenvalue of H restricted to basis states with total QN `total`."""
+    import itertools
+    states = list(itertools.product(*[range(len(q)) for q in site_qns]))
+    keep = [k for k, s in enumerate(states)
+            if tuple(np.sum([site_qns[i][s[i]] for i in range(len(s))], axis=0)) == tuple(total)]
+    Hs = H[np.ix_(keep, keep)]
+    return np.linalg.eigvalsh(Hs)[0]
+
+
+@pytest.mark.parametrize("name", ["heisenberg_1d_N6", "heisenberg_2d_6", "ising_1d_N6", "tj_1d_N4", "tj_2d_N4",
+                                  "hubbard_2x2", "kondo_8", "kitaev_simple_8"])
+def test_ed_reproduces_reference_golden_energy(name):
+    case = HM.golden_cases()[name]
+    m = case["build"]()
+    H = mpo_dense(m)
+    assert np.abs(H - H.T).max() < 1e-12
+    sites = {"tj": HM.TJ_SITE, "hubbard": HM.HUB_SITE}
+    if name.startswith("kondo"):
+        e = np.linalg.eigvalsh(H)[0]
+    else:
+        key = "tj" if name.startswith("tj") else ("hubbard" if name.startswith("hubbard") else None)
+        site = sites[key] if key else HM.SPIN_SITE
+        qns = [[q for q, _ in site]] * m.N
+        labs = case["init"]["labs"][0]
+        total = tuple(np.sum([qns[i][labs[i]] for i in range(m.N)], axis=0))
+        e = sector_ground_energy(H, qns, total)
+    assert abs(e - case["e0"]) < max(case["tol"], 1e-11), (e, case["e0"])
+
+
+@pytest.mark.parametrize("name", ["heisenberg_1d_N6", "heisenberg_2d_6", "ising_1d_N6", "tj_1d_N4", "hubbard_2x2",
+                                  "kitaev_simple_8", "kondo_8"])
+def test_oracle_sweep_reproduces_golden_energy(name):
+    case = HM.golden_cases()[name]
+    m = case["build"]()
+    init = case["init"]
+    m.init_mps(init["labs"], enlarged_dim=init.get("enlarged", 0), seed=7, centralize=False)
+    raw = [m.mps(i) for i in range(m.N)]
+    mps = O.centralize0(raw)
+    mpo = [m.mpo(i) for i in range(m.N)]
+    if init.get("enlarged", 0):
+        # ExtendDirectRandomInitMps normalises the head tensor after Centralize(0)
+        h = mps[0]
+        n = h.norm()
+        mps[0] = BTensor(h.legs, h.qw, {c: b / n for c, b in h.blocks.items()})
+    r = O.dmrg(mps, mpo, case["sweeps"], case["dmin"], case["dmax"], case["trunc"], case["lerr"], case["lmax"])
+    assert abs(r["energy"] - case["e0"]) < case["tol"], (r["energy"], case["e0"])
