@@ -111,11 +111,14 @@ class TwoSiteVMPSEngine {
       : mps_(mps), mpo_(mpo), sp_(sp), N_(mps.size()), lenvs_(N_), renvs_(N_) {}
 
   std::function<void(const UpdateRecord &)> on_update;
+  // Called with the effective Hamiltonian and initial state right before each
+  // Lanczos solve (measurement hook; extension).
+  std::function<void(const std::vector<TenT *> &, const TenT &, const std::string &, char, size_t)> before_lanczos;
   bool quiet = false;
 
   // two_site_update_finite_vmps_impl.h:104-135
-  void InitEnvs() {
-    for (size_t i = 1; i + 2 <= N_; ++i) {
+  void InitEnvs(size_t max_len = (size_t)-1) {
+    for (size_t i = 1; i + 2 <= N_ && i <= max_len; ++i) {
       const TenT &A = mps_[N_ - i];
       if (i == 1) {
         TenT temp;
@@ -196,6 +199,7 @@ class TwoSiteVMPSEngine {
     eff_ham[3] = where == "rend" ? nullptr : &renvs_[rlen];
     auto init_state = new TenT;
     Contract(&mps_[ls], &mps_[rs], init_axes, init_state);
+    if (before_lanczos) before_lanczos(eff_ham, *init_state, where, dir, site);
     Timer lancz_timer("Lancz");
     auto lr = LanczosSolver(eff_ham, init_state, sp_.lancz_params, where);
     double lancz_t = lancz_timer.Elapsed();

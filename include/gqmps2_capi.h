@@ -66,6 +66,22 @@ int gqmps2_dmrg(gqmps2_model m, int64_t sweeps, int64_t dmin, int64_t dmax, cons
                 double *energies, int64_t *dims, int64_t *iters, double *trunc_errs, double *update_times,
                 double *sweep_times, double *final_energy);
 
+/* Measurement hook (no reference counterpart): during later gqmps2_dmrg
+ * calls, right before the Lanczos solve of the right-moving update at `site`
+ * of the first sweep, run mps2_heff_bench on that bond's effective
+ * Hamiltonian for FP64 and mixed precision and keep copies of L, W1, W2, R
+ * and the initial state.  site < 0 disables it. */
+int gqmps2_set_bench(gqmps2_model m, int64_t site, int warmup, int reps);
+/* Results: out[0..10] FP64, out[11..21] mixed (layout of mps2_heff_bench). */
+int gqmps2_get_bench(gqmps2_model m, double *out);
+/* New handles on the kept tensors (L, W1, W2, R, psi); NULL where absent. */
+int gqmps2_bench_tensors(gqmps2_model m, mps2_ten *out5);
+/* Move the MPS centre to `site` (FiniteMPS::Centralize), build the left and
+ * right environments of the bond (site, site+1) and benchmark its matvec as
+ * gqmps2_set_bench does; results through gqmps2_get_bench and
+ * gqmps2_bench_tensors.  Leaves the MPS centred at `site`. */
+int gqmps2_bench_bond(gqmps2_model m, int64_t site, int warmup, int reps);
+
 /* Engine context used by this library (for mps2_ctx_stats). */
 mps2_ctx gqmps2_engine_ctx(void);
 /* Message of this thread's last gqmps2_* failure. */

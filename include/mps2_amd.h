@@ -90,6 +90,11 @@ int mps2_ten_div(mps2_ten t, int32_t *qn);
  * Reference call sites: lanczos_solver_impl.h:136,198,231-234,242-244,252-254;
  * two_site_update_finite_vmps_impl.h:118,120,124,127,129,243,272,276,290-317. */
 int mps2_contract(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, mps2_ten *out);
+/* Contract with the arithmetic precision selectable (MPS2_PREC_MIXED: operands
+ * rounded to FP32, FP32 MFMA with FP32 accumulation, result widened to FP64).
+ * Same semantics as mps2_contract otherwise; used by the mixed matvec. */
+int mps2_contract_ex(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, int precision,
+                     mps2_ten *out);
 
 /* gqten LinearCombine: out = beta*out + sum_i coefs[i]*ts[i] (block sets
  * are united).  Reference call sites: lanczos_solver_impl.h:151,153,173,215. */
@@ -124,6 +129,18 @@ int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi
  * Returns iterations, ground energy and a new ground-state tensor. */
 int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init, const char *where, double error,
                  int64_t max_iter, int precision, int64_t *iters, double *energy, mps2_ten *gs);
+
+/* Benchmark of one effective-Hamiltonian matvec (no reference counterpart:
+ * measurement only).  Runs `warmup` untimed then `reps` timed matvecs on the
+ * context stream, bracketed by HIP events, and times every grouped-GEMM
+ * launch with its own events.  out[0] ms per matvec, out[1] algorithmic
+ * flops per matvec (sum over QN-matched block pairs of 2*M*N*K), out[2] ms
+ * per matvec inside grouped GEMMs, out[3] index of the stage with the longest
+ * GEMM time, out[4] that stage's flops per launch, out[5] its ms per launch,
+ * out[6] FP64 bytes of L, W1, W2, R and two states (the fused-ideal traffic),
+ * out[7..10] ms per launch of stages 1..4. */
+int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                    int precision, int warmup, int reps, double *out);
 
 /* TridiagGsSolver (lanczos_solver_impl.h:259-312): lowest eigenpair of the
  * symmetric tridiagonal (a[0..n-1], b[0..n-2]), solved on the device.  vec may

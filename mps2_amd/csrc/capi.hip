@@ -189,6 +189,34 @@ int mps2_contract(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const 
   });
 }
 
+int mps2_contract_ex(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, int precision,
+                     mps2_ten *out) {
+  return guard([&] {
+    need(a, "a");
+    need(b, "b");
+    need(out, "out");
+    if (nax > 0) {
+      need(axes_a, "axes_a");
+      need(axes_b, "axes_b");
+    }
+    std::vector<int> xa(axes_a, axes_a + nax), xb(axes_b, axes_b + nax);
+    if (precision != MPS2_PREC_MIXED) {
+      *out = wrap(mps2::contract(a->t, b->t, xa, xb));
+      return;
+    }
+    mps2::Ctx *c = a->t->ctx;
+    auto to32 = [&](const mps2::TenP &t) {
+      auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
+      mps2::k_cast_d2f(c, t->n(), t->d(), r->f());
+      return r;
+    };
+    auto r32 = mps2::contract(to32(a->t), to32(b->t), xa, xb);
+    auto r64 = mps2::ten_alloc(c, r32->meta, mps2::kF64);
+    mps2::k_cast_f2d(c, r32->n(), r32->f(), r64->d());
+    *out = wrap(r64);
+  });
+}
+
 int mps2_linear_combine(int n, const double *coefs, const mps2_ten *ts, double beta, mps2_ten out) {
   return guard([&] {
     need(out, "out");
@@ -273,6 +301,88 @@ int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init
     if (iters) *iters = r.iters;
     if (energy) *energy = r.energy;
     *gs = wrap(r.gs);
+  });
+}
+
+int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                    int precision, int warmup, int reps, double *out) {
+  return guard([&] {
+    need(W1, "W1");
+    need(W2, "W2");
+    need(psi, "psi");
+    need(where, "where");
+    need(out, "out");
+    if (reps < 1) throw mps2::Error(mps2::kErrArg, "reps must be >= 1");
+    std::string w(where);
+    if (w == "cent" || w == "rend") need(L, "L");
+    if (w == "cent" || w == "lend") need(R, "R");
+    mps2::Ctx *c = psi->t->ctx;
+    auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
+    mps2::Prec p = precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64;
+    auto cast = [&](const mps2::TenP &t) -> mps2::TenP {
+      if (!t || p == mps2::kF64) return t;
+      auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
+      mps2::k_cast_d2f(c, t->n(), t->d(), r->f());
+      return r;
+    };
+    auto Lp = cast(opt(L)), W1p = cast(W1->t), W2p = cast(W2->t), Rp = cast(opt(R));
+    auto x = cast(mps2::embed(psi->t, full));
+    auto y = mps2::ten_alloc(c, full, p);
+    for (int i = 0; i < warmup; ++i) mps2::heff_mul(Lp, W1p, W2p, Rp, x, w, y, p);
+    c->sync();
+    for (auto &g : c->gemm_timings) {
+      (void)hipEventDestroy(g.start);
+      (void)hipEventDestroy(g.stop);
+    }
+    c->gemm_timings.clear();
+    c->time_gemms = true;
+    hipEvent_t e0, e1;
+    MPS2_HIP(hipEventCreate(&e0));
+    MPS2_HIP(hipEventCreate(&e1));
+    MPS2_HIP(hipEventRecord(e0, c->stream));
+    try {
+      for (int i = 0; i < reps; ++i) mps2::heff_mul(Lp, W1p, W2p, Rp, x, w, y, p);
+    } catch (...) {
+      c->time_gemms = false;
+      throw;
+    }
+    MPS2_HIP(hipEventRecord(e1, c->stream));
+    c->time_gemms = false;
+    MPS2_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MPS2_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    size_t per = c->gemm_timings.size() / reps;
+    std::vector<double> st_ms(per, 0.0), st_fl(per, 0.0);
+    double gemm_ms = 0, flops = 0;
+    for (size_t k = 0; k < c->gemm_timings.size(); ++k) {
+      float t = 0;
+      MPS2_HIP(hipEventElapsedTime(&t, c->gemm_timings[k].start, c->gemm_timings[k].stop));
+      gemm_ms += t;
+      st_ms[k % per] += t;
+      if (k < per) st_fl[k] = c->gemm_timings[k].flops;
+      (void)hipEventDestroy(c->gemm_timings[k].start);
+      (void)hipEventDestroy(c->gemm_timings[k].stop);
+    }
+    c->gemm_timings.clear();
+    for (size_t k = 0; k < per; ++k) flops += st_fl[k];
+    size_t dom = 0;
+    for (size_t k = 1; k < per; ++k)
+      if (st_ms[k] > st_ms[dom]) dom = k;
+    double bytes = 0;
+    for (auto &t : {opt(L), W1->t, W2->t, opt(R)})
+      if (t) bytes += 8.0 * t->n();
+    bytes += 2 * 8.0 * full->total;
+    for (int i = 0; i < 11; ++i) out[i] = 0;
+    out[0] = ms / reps;
+    out[1] = flops;
+    out[2] = gemm_ms / reps;
+    out[3] = (double)dom;
+    out[4] = per ? st_fl[dom] : 0;
+    out[5] = per ? st_ms[dom] / reps : 0;
+    out[6] = bytes;
+    for (size_t k = 0; k < per && k < 4; ++k) out[7 + k] = st_ms[k] / reps;
   });
 }
 

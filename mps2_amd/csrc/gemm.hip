@@ -20,12 +20,17 @@ namespace {
 template <typename T>
 struct Mfma;
 
+// Operand layout (both types): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16].  Accumulator element r of lane l is D[row(l, r)][l % 16],
+// where the row mapping differs between the FP64 and FP32 instructions
+// (measured on gfx950): FP64 rows l/16 + 4r, FP32 rows 4(l/16) + r.
 template <>
 struct Mfma<double> {
   typedef double v4 __attribute__((ext_vector_type(4)));
   static __device__ __forceinline__ v4 op(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 
 template <>
@@ -34,6 +39,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ v4 op(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 constexpr int kThreads = 256;
@@ -139,7 +145,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
       __syncthreads();
     }
   }
-  // ---- epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 fragment
+  // ---- epilogue: element r of each 16x16 fragment is row Mfma<T>::row(lane, r), column lane%16
   T *Cb = C + blk.c_off;
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -148,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
       const int gn = n0 + wn * WN + j * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        const int gm = m0 + wm * WM + i * 16 + Mfma<T>::row(lane, r);
         if (gm < M && gn < N) Cb[(int64_t)gm * N + gn] = acc[i][j][r];
       }
     }

@@ -103,10 +103,179 @@ LanczosOut lanczos_plain(const TenP &L, const TenP &W1, const TenP &W2, const Te
   }
 }
 
+namespace {
+
+TenP cast_to(const TenP &t, Prec p) {
+  if (!t) return nullptr;
+  if (t->prec == p) return t;
+  auto r = ten_alloc(t->ctx, t->meta, p);
+  if (p == kF32) k_cast_d2f(t->ctx, t->n(), t->d(), r->f());
+  else k_cast_f2d(t->ctx, t->n(), t->f(), r->d());
+  return r;
+}
+
+// Lowest eigenpair of a small dense symmetric matrix (row-major n x n) by
+// cyclic Jacobi; returns the eigenvalue and fills vec.
+double small_sym_lowest(int n, std::vector<double> A, std::vector<double> &vec) {
+  std::vector<double> V(n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    double off = 0, tot = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        tot += A[i * n + j] * A[i * n + j];
+        if (i != j) off += A[i * n + j] * A[i * n + j];
+      }
+    if (off <= 1e-32 * tot) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        double zeta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
+        double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
+        double c = 1 / std::sqrt(1 + t * t), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double x = A[p * n + k], y = A[q * n + k];
+          A[p * n + k] = c * x - s * y;
+          A[q * n + k] = s * x + c * y;
+        }
+        for (int k = 0; k < n; ++k) {
+          double x = A[k * n + p], y = A[k * n + q];
+          A[k * n + p] = c * x - s * y;
+          A[k * n + q] = s * x + c * y;
+          double u = V[k * n + p], v = V[k * n + q];
+          V[k * n + p] = c * u - s * v;
+          V[k * n + q] = s * u + c * v;
+        }
+      }
+  }
+  int im = 0;
+  for (int i = 1; i < n; ++i)
+    if (A[i * n + i] < A[im * n + im]) im = i;
+  vec.resize(n);
+  for (int k = 0; k < n; ++k) vec[k] = V[k * n + im];
+  return A[im * n + im];
+}
+
+}  // namespace
+
+// Mixed-precision form: restarted Lanczos with FP64 residual correction.
+// Each restart k computes, in FP64, h = H v_k, E_k = <v_k|h> and the residual
+// r = h - E_k v_k.  The Krylov space {v_k, r, H~ r, ...} is then extended with
+// the FP32-MFMA operator H~ (FP32 operands, FP32 accumulation), every new
+// vector orthogonalised twice in FP64 against the whole basis.  The projected
+// matrix uses the exact FP64 row of v_k (E_k, |r|, 0, ...) and FP64 dot
+// products of the FP32 images, so the error left by H~ enters the Ritz vector
+// only at second order and each restart contracts it further: the result
+// converges to the FP64 answer.  Stop when the FP64 Rayleigh quotient
+// decreases by less than `error` between restarts (the reference's energy
+// criterion) or the residual vanishes; the energy returned is the FP64
+// Rayleigh quotient.
+LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
+                         const std::string &where, double error, int64_t max_iter) {
+  Ctx *c = init->ctx;
+  MPS2_REQUIRE(max_iter >= 2, "Lanczos: max_iterations must be >= 2");
+  int64_t eff_dim = eff_dim_of(L, W1, W2, R, where);
+  MetaP full = full_meta(init->meta->qw, init->meta->legs, init->meta->div());
+  TenP L32 = cast_to(L, kF32), W132 = cast_to(W1, kF32), W232 = cast_to(W2, kF32), R32 = cast_to(R, kF32);
+  const int64_t n = full->total;
+  const int kInnerMax = (int)std::min<int64_t>(std::max<int64_t>(max_iter, 2), 40);
+  TenP v = embed(init, full);
+  normalize(*v);
+  TenP h = ten_alloc(c, full);
+  TenP x32 = ten_alloc(c, full, kF32), y32 = ten_alloc(c, full, kF32);
+  LanczosOut res;
+  double E_prev = 0;
+  int64_t total_inner = 0;
+  for (int restart = 0; restart < 64; ++restart) {
+    heff_mul(L, W1, W2, R, v, where, h, kF64);
+    res.matvecs_hi++;
+    double E = dot(*h, *v);
+    // r = h - E v  (in h)
+    k_axpby(c, n, -E, v->d(), 1.0, h->d());
+    double beta = norm(*h);
+    bool done = (restart > 0 && (E_prev - E) < error) || beta == 0.0 || total_inner >= max_iter - 1 ||
+                (eff_dim <= 1);
+    if (done) {
+      res.iters = std::max<int64_t>(total_inner, 1);
+      res.energy = E;
+      res.gs = v;
+      return res;
+    }
+    E_prev = E;
+    std::vector<TenP> Q{v};
+    auto q1 = ten_alloc(c, full);
+    MPS2_HIP(hipMemcpyAsync(q1->d(), h->d(), n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    k_scale(c, n, 1.0 / beta, q1->d());
+    Q.push_back(q1);
+    // projected matrix, grown column by column (kept as a full symmetric matrix)
+    std::vector<std::vector<double>> T(2, std::vector<double>(2, 0.0));
+    T[0][0] = E;
+    T[0][1] = T[1][0] = beta;
+    std::vector<double> y;
+    double theta_prev = E;
+    int m_in = 0;
+    for (int j = 1; j <= kInnerMax; ++j) {
+      // w = H~ q_j
+      k_cast_d2f(c, n, Q[j]->d(), x32->f());
+      heff_mul(L32, W132, W232, R32, x32, where, y32, kF32);
+      res.matvecs_lo++;
+      auto w = ten_alloc(c, full);
+      k_cast_f2d(c, n, y32->f(), w->d());
+      ++total_inner;
+      int k = (int)Q.size();
+      std::vector<double> col(k, 0.0);
+      for (int i = 1; i < k; ++i) col[i] = dot(*Q[i], *w);
+      col[0] = (j == 1) ? beta : 0.0;
+      for (int i = 1; i <= j; ++i) {
+        T[i][j] = col[i];
+        T[j][i] = col[i];
+      }
+      // full reorthogonalisation (two passes, FP64)
+      for (int pass = 0; pass < 2; ++pass)
+        for (int i = 0; i < k; ++i) {
+          double cf = pass == 0 && i > 0 ? col[i] : dot(*Q[i], *w);
+          k_axpby(c, n, -cf, Q[i]->d(), 1.0, w->d());
+        }
+      int kk = j + 1;
+      std::vector<double> A(kk * kk);
+      for (int a = 0; a < kk; ++a)
+        for (int b = 0; b < kk; ++b) A[a * kk + b] = T[a][b];
+      double theta = small_sym_lowest(kk, A, y);
+      m_in = j;
+      double wn = norm(*w);
+      bool inner_done = std::fabs(theta_prev - theta) < 0.1 * error || wn <= 1e-14 * std::fabs(theta) ||
+                        (int64_t)Q.size() >= eff_dim || j == kInnerMax || total_inner >= max_iter - 1;
+      theta_prev = theta;
+      if (inner_done) break;
+      k_scale(c, n, 1.0 / wn, w->d());
+      Q.push_back(w);
+      for (auto &row : T) row.push_back(0.0);
+      T.push_back(std::vector<double>(T[0].size(), 0.0));
+    }
+    // The inner solve predicts no decrease beyond the tolerance: v is converged.
+    if (restart > 0 && (E - theta_prev) < error) {
+      res.iters = std::max<int64_t>(total_inner, 1);
+      res.energy = E;
+      res.gs = v;
+      return res;
+    }
+    // v <- sum_i y_i Q_i  (y has m_in + 1 entries: Q[0..m_in])
+    int nv = m_in + 1;
+    std::vector<double> z(y.begin(), y.begin() + nv);
+    v = final_combination(c, full, Q, nv, z);
+    normalize(*v);
+  }
+  res.iters = total_inner;
+  res.energy = E_prev;
+  res.gs = v;
+  return res;
+}
+
 LanczosOut lanczos(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
                    const std::string &where, double error, int64_t max_iter, Prec prec) {
   if (prec == kF64) return lanczos_plain(L, W1, W2, R, init, where, error, max_iter);
-  throw Error(kErrArg, "Lanczos: precision mode not available");
+  return lanczos_mixed(L, W1, W2, R, init, where, error, max_iter);
 }
 
 }  // namespace mps2

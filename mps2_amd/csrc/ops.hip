@@ -168,9 +168,20 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
                  static_cast<const int64_t *>(dp.perm_b_start->p), dp.nperm_b, b->n());
     B = static_cast<const T *>(pb->p);
   }
+  Ctx::GemmTiming tm{};
+  if (c->time_gemms) {
+    MPS2_HIP(hipEventCreate(&tm.start));
+    MPS2_HIP(hipEventCreate(&tm.stop));
+    MPS2_HIP(hipEventRecord(tm.start, c->stream));
+  }
   launch_grouped_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, static_cast<const TileD *>(dp.tiles->p), dp.ntiles,
                          static_cast<const BlkD *>(dp.blks->p), static_cast<const PairD *>(dp.pairs->p), dp.tile_m,
                          dp.tile_n);
+  if (c->time_gemms) {
+    MPS2_HIP(hipEventRecord(tm.stop, c->stream));
+    tm.flops = dp.hp->flops;
+    c->gemm_timings.push_back(tm);
+  }
   c->gemm_flops += dp.hp->flops;
 }
 

@@ -74,6 +74,13 @@ struct Ctx {
   double gemm_flops = 0;
   int64_t gemm_launches = 0;
   std::string last_kernel;
+  // per-launch timing of grouped GEMMs (benchmarking only)
+  bool time_gemms = false;
+  struct GemmTiming {
+    hipEvent_t start, stop;
+    double flops;
+  };
+  std::vector<GemmTiming> gemm_timings;
 
   explicit Ctx(int dev);
   ~Ctx();

@@ -24,7 +24,7 @@
 //  1. all singular values, sorted descending, ties by (sector, index);
 //  2. D_err = smallest D >= 1 with sum_{i>=D} s_i^2 <= trunc_err * sum s_i^2;
 //  3. D = min(max(D_err, Dmin), Dmax, count);
-//  4. values within relative 1e-8 of the last kept value form the cut cluster;
+//  4. values within relative 1e-6 of the last kept value form the cut cluster;
 //     the cut keeps as many of them as step 3 does, chosen by (sector, index).
 // New-bond sectors are ordered by ascending QN; states in a sector by
 // descending singular value.
@@ -43,7 +43,7 @@ namespace {
 
 constexpr int kB = 32;        // Jacobi block width
 constexpr int kP = 2 * kB;    // subproblem size
-constexpr double kDegTol = 1e-8;
+constexpr double kDegTol = 1e-6;
 
 BufP upload(Ctx *c, const void *h, size_t bytes) {
   auto b = c->alloc(std::max<size_t>(bytes, 16));
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void pair_eig_kernel(const double *G, const Se
   for (int e = tid; e < kP * kP; e += 256) {
     int i = e / kP, j = e % kP;
     int gi = (i < kB ? w.I * kB + i : w.J * kB + (i - kB));
-    int gj = (j < kB ? w.J * kB * 0 + w.I * kB + j : w.J * kB + (j - kB));
+    int gj = (j < kB ? w.I * kB + j : w.J * kB + (j - kB));
     double v1 = g[(int64_t)gi * np + gj], v2 = g[(int64_t)gj * np + gi];
     S[i][j] = 0.5 * (v1 + v2);
     Q[i][j] = (i == j) ? 1.0 : 0.0;

@@ -51,6 +51,7 @@ SIGNATURES = {
     "mps2_lanczos": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_double, C.c_int64, C.c_int, _i64p, _dp,
                                C.POINTER(_vp)]),
     "mps2_tridiag_lowest": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "mps2_heff_bench": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]),
 }
 
 
@@ -215,3 +216,23 @@ def lanczos(L, W1, W2, R, init, where, error, max_iter=200, precision=PREC_FP64)
     check(lib().mps2_lanczos(_h(L), W1.h, W2.h, _h(R), init.h, where.encode(), error, max_iter, precision,
                              C.byref(it), C.byref(e), C.byref(gs)))
     return it.value, e.value, DTen(gs, init.ctx)
+
+
+def heff_bench(L, W1, W2, R, psi, where, precision=PREC_FP64, warmup=2, reps=10):
+    out = np.zeros(11)
+    check(lib().mps2_heff_bench(_h(L), W1.h, W2.h, _h(R), psi.h, where.encode(), precision, warmup, reps,
+                                out.ctypes.data_as(_dp)))
+    return dict(ms=out[0], flops=out[1], gemm_ms=out[2], dom_stage=int(out[3]), dom_flops=out[4],
+                dom_ms=out[5], bytes_min=out[6], stage_ms=list(out[7:11]))
+
+
+SIGNATURES["mps2_contract_ex"] = (C.c_int, [_vp, _vp, C.c_int, _i32p, _i32p, C.c_int, C.POINTER(_vp)])
+
+
+def contract_ex(a: DTen, b: DTen, axes, precision=PREC_FP64) -> DTen:
+    xa = np.asarray(axes[0], dtype=np.int32)
+    xb = np.asarray(axes[1], dtype=np.int32)
+    h = _vp()
+    check(lib().mps2_contract_ex(a.h, b.h, len(xa), xa.ctypes.data_as(_i32p), xb.ctypes.data_as(_i32p), precision,
+                                 C.byref(h)))
+    return DTen(h, a.ctx)

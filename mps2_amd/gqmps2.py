@@ -38,6 +38,10 @@ SIGNATURES = {
     "gqmps2_mps_restore": (C.c_int, [_vp]),
     "gqmps2_dmrg": (C.c_int, [_vp, C.c_int64, C.c_int64, C.c_int64, _i64p, C.c_double, C.c_double, C.c_int64,
                               C.c_int, C.c_int, _dp, _i64p, _i64p, _dp, _dp, _dp, _dp]),
+    "gqmps2_set_bench": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int]),
+    "gqmps2_get_bench": (C.c_int, [_vp, _dp]),
+    "gqmps2_bench_tensors": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "gqmps2_bench_bond": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int]),
     "gqmps2_engine_ctx": (_vp, []),
     "gqmps2_last_error": (C.c_char_p, []),
 }
@@ -150,6 +154,28 @@ class Model:
 
     def restore(self):
         check(lib().gqmps2_mps_restore(self.h))
+
+    def set_bench(self, site, warmup=2, reps=10):
+        check(lib().gqmps2_set_bench(self.h, int(site), int(warmup), int(reps)))
+
+    def bench_bond(self, site, warmup=2, reps=10):
+        check(lib().gqmps2_bench_bond(self.h, int(site), int(warmup), int(reps)))
+        return self.get_bench()
+
+    def get_bench(self):
+        out = np.zeros(22)
+        check(lib().gqmps2_get_bench(self.h, out.ctypes.data_as(_dp)))
+
+        def unpack(o):
+            return dict(ms=o[0], flops=o[1], gemm_ms=o[2], dom_stage=int(o[3]), dom_flops=o[4], dom_ms=o[5],
+                        bytes_min=o[6], stage_ms=list(o[7:11]))
+        return dict(fp64=unpack(out[:11]), mixed=unpack(out[11:]))
+
+    def bench_tensors(self):
+        """(L, W1, W2, R, psi) of the benchmarked bond as engine DTen handles."""
+        arr = (_vp * 5)()
+        check(lib().gqmps2_bench_tensors(self.h, arr))
+        return [engine.DTen(_vp(arr[k]), None) if arr[k] else None for k in range(5)]
 
     def dmrg(self, sweeps, dmin, dmax, trunc_err, lanczos_error, lanczos_max_iter=200,
              precision=engine.PREC_FP64, dmax_schedule=None, verbose=False):

@@ -35,9 +35,21 @@ struct ModelBase {
   virtual void import_mps(int site, const int32_t *meta, const double *data) = 0;
   virtual void checkpoint() = 0;
   virtual void restore() = 0;
+  int64_t bench_site = -1;
+  int bench_warmup = 2, bench_reps = 10;
+  bool bench_done = false;
+  double bench_out[22] = {0};
+  std::vector<mps2_ten> bench_tens = std::vector<mps2_ten>(5, nullptr);
+  void clear_bench_tens() {
+    for (auto &h : bench_tens) {
+      if (h) mps2_ten_free(h);
+      h = nullptr;
+    }
+  }
   virtual double dmrg(int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *sched, double trunc_err, double lerr,
                       int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te,
                       double *ut, double *st) = 0;
+  virtual void bench_bond(int64_t site, int warmup, int reps) = 0;
 };
 
 template <typename QNT>
@@ -178,12 +190,8 @@ struct Model : ModelBase {
     }
     p += nb * rank;
     std::vector<int32_t> mm(meta, meta + p);
-    TenT tmp;
-    tmp.FromFlat(mm, {});
     int64_t total = 0;
     {
-      std::vector<int32_t> mt;
-      std::vector<double> dt;
       // count elements from the block list
       int q = 3;
       std::vector<std::vector<int64_t>> dims(rank);
@@ -230,6 +238,47 @@ struct Model : ModelBase {
     }
   }
 
+  void bench_bond(int64_t site, int warmup, int reps) override {
+    if (site < 1 || (size_t)site + 2 >= N) throw std::invalid_argument("bench_bond: site must be an interior bond");
+    for (size_t i = 0; i < N; ++i)
+      if (!(*mps)(i)) throw std::runtime_error("MPS not initialised");
+    mps->Centralize((int)site);
+    gqmps2::SweepParams sp(1, 1, 1, 0.0, gqmps2::LanczosParams());
+    gqmps2::TwoSiteVMPSEngine<double, QNT> eng(*mps, mpo, sp);
+    auto &L = eng.lenvs();
+    auto &R = eng.renvs();
+    // left environments of sites 0 .. site-1 (two_site_update_finite_vmps_impl.h:288-299)
+    for (int64_t s = 0; s < site; ++s) {
+      if (s == 0) {
+        TenT temp;
+        Contract(&(*mps)[0], &mpo[0], {{0}, {0}}, &temp);
+        auto ad = gqten::Dag((*mps)[0]);
+        Contract(&temp, &ad, {{2}, {0}}, &L[1]);
+      } else {
+        TenT t1, t2;
+        Contract(&L[s], &(*mps)[s], {{0}, {0}}, &t1);
+        Contract(&t1, &mpo[s], {{0, 2}, {0, 1}}, &t2);
+        auto ad = gqten::Dag((*mps)[s]);
+        Contract(&t2, &ad, {{0, 2}, {0, 1}}, &L[s + 1]);
+      }
+    }
+    // right environments up to length N - site - 2
+    size_t rlen = N - (size_t)site - 2;
+    eng.InitEnvs(rlen);
+    std::vector<TenT *> H{&L[site], &mpo[site], &mpo[site + 1], &R[rlen]};
+    TenT init;
+    Contract(&(*mps)[site], &(*mps)[site + 1], {{2}, {0}}, &init);
+    for (int p = 0; p < 2; ++p)
+      gqten::CheckMps2(mps2_heff_bench(H[0]->Handle(), H[1]->Handle(), H[2]->Handle(), H[3]->Handle(), init.Handle(),
+                                       "cent", p == 0 ? MPS2_PREC_FP64 : MPS2_PREC_MIXED, warmup, reps,
+                                       bench_out + 11 * p),
+                       "heff_bench");
+    clear_bench_tens();
+    mps2_ten src[5] = {H[0]->Handle(), H[1]->Handle(), H[2]->Handle(), H[3]->Handle(), init.Handle()};
+    for (int k = 0; k < 5; ++k) gqten::CheckMps2(mps2_ten_copy(src[k], &bench_tens[k]), "bench copy");
+    bench_done = true;
+  }
+
   double dmrg(int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *sched, double trunc_err, double lerr,
               int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te, double *ut,
               double *st) override {
@@ -262,6 +311,23 @@ struct Model : ModelBase {
       }
       eng->quiet = !verbose;
       eng->on_update = record;
+      if (bench_site >= 0 && !bench_done) {
+        eng->before_lanczos = [this](const std::vector<TenT *> &H, const TenT &init, const std::string &where,
+                                     char dir, size_t site) {
+          if (bench_done || dir != 'r' || (int64_t)site != bench_site) return;
+          auto h = [](TenT *t) -> mps2_ten { return t ? t->Handle() : nullptr; };
+          for (int p = 0; p < 2; ++p)
+            gqten::CheckMps2(mps2_heff_bench(h(H[0]), H[1]->Handle(), H[2]->Handle(), h(H[3]), init.Handle(),
+                                             where.c_str(), p == 0 ? MPS2_PREC_FP64 : MPS2_PREC_MIXED, bench_warmup,
+                                             bench_reps, bench_out + 11 * p),
+                             "heff_bench");
+          clear_bench_tens();
+          mps2_ten src[5] = {h(H[0]), H[1]->Handle(), H[2]->Handle(), h(H[3]), init.Handle()};
+          for (int k = 0; k < 5; ++k)
+            if (src[k]) gqten::CheckMps2(mps2_ten_copy(src[k], &bench_tens[k]), "bench copy");
+          bench_done = true;
+        };
+      }
       gqten::Timer t("sweep");
       e0 = eng->Sweep((size_t)s + 1);
       gqten::CheckMps2(mps2_ctx_sync(gqten::DefaultCtx()), "sync");
@@ -298,7 +364,10 @@ int gqmps2_model_create(int qn_width, int nsites, const int32_t *nsect, const in
 }
 
 int gqmps2_model_destroy(gqmps2_model m) {
-  return guard([&] { delete m; });
+  return guard([&] {
+    if (m) m->m->clear_bench_tens();
+    delete m;
+  });
 }
 
 int gqmps2_model_add_op(gqmps2_model m, int site, const double *dense, int32_t *op_id) {
@@ -397,6 +466,41 @@ int gqmps2_dmrg(gqmps2_model m, int64_t sweeps, int64_t dmin, int64_t dmax, cons
     double e = m->m->dmrg(sweeps, dmin, dmax, dmax_schedule, trunc_err, lanczos_error, lanczos_max_iter, precision,
                           verbose, energies, dims, iters, trunc_errs, update_times, sweep_times);
     if (final_energy) *final_energy = e;
+  });
+}
+
+int gqmps2_set_bench(gqmps2_model m, int64_t site, int warmup, int reps) {
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    m->m->bench_site = site;
+    m->m->bench_warmup = warmup;
+    m->m->bench_reps = reps;
+    m->m->bench_done = false;
+  });
+}
+
+int gqmps2_bench_bond(gqmps2_model m, int64_t site, int warmup, int reps) {
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    m->m->bench_bond(site, warmup, reps);
+  });
+}
+
+int gqmps2_get_bench(gqmps2_model m, double *out) {
+  return guard([&] {
+    if (!m || !out) throw std::invalid_argument("NULL argument");
+    if (!m->m->bench_done) throw std::runtime_error("no benchmark recorded");
+    for (int i = 0; i < 22; ++i) out[i] = m->m->bench_out[i];
+  });
+}
+
+int gqmps2_bench_tensors(gqmps2_model m, mps2_ten *out5) {
+  return guard([&] {
+    if (!m || !out5) throw std::invalid_argument("NULL argument");
+    for (int k = 0; k < 5; ++k) {
+      out5[k] = nullptr;
+      if (m->m->bench_tens[k]) gqten::CheckMps2(mps2_ten_copy(m->m->bench_tens[k], &out5[k]), "bench copy");
+    }
   });
 }
 

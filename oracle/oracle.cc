@@ -499,7 +499,7 @@ void svd_jacobi(int64_t m, int64_t n, const double *A, std::vector<double> &U, s
 //     cluster; the cut keeps as many of them as step 3 keeps, chosen by
 //     (sector position, index) ascending, so near-ties do not depend on
 //     rounding.
-const double kDegTol = 1e-8;
+const double kDegTol = 1e-6;
 
 struct SvInfo {
   double v;

@@ -167,3 +167,49 @@ def test_linear_combine_and_normalize(ctx):
     n = out.normalize()
     assert abs(n - np.linalg.norm(ref)) < 1e-12 * n
     assert abs(E.dot(out, out) - 1) < 1e-14
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED])
+def test_grouped_gemm_shapes_vs_numpy(ctx, prec):
+    """Single-block contractions over edge shapes and all four operand layouts
+    against numpy (FP64: 1e-12 relative; FP32 MFMA: 1e-5 relative to the
+    magnitude bound sum_k |a||b|)."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(5)
+
+    def lg(d, dr):
+        return Leg(dr, [((0,), d)])
+
+    for (M, K, N) in [(1, 1, 1), (2, 3, 1), (3, 2, 5), (16, 4, 16), (17, 5, 3), (8, 2, 64), (64, 16, 64),
+                      (65, 17, 65), (130, 40, 97), (300, 211, 190)]:
+        for case in ("NN", "TN", "NT", "TT"):
+            a = rng.uniform(-1, 1, (M, K))
+            b = rng.uniform(-1, 1, (K, N))
+            if case[0] == "N":
+                A, ax = BTensor([lg(M, IN), lg(K, OUT)], 1, {(0, 0): a}), [1]
+            else:
+                A, ax = BTensor([lg(K, OUT), lg(M, IN)], 1, {(0, 0): a.T.copy()}), [0]
+            if case[1] == "N":
+                B, bx = BTensor([lg(K, IN), lg(N, OUT)], 1, {(0, 0): b}), [0]
+            else:
+                B, bx = BTensor([lg(N, OUT), lg(K, IN)], 1, {(0, 0): b.T.copy()}), [1]
+            got = E.contract_ex(ctx.put(A), ctx.put(B), [ax, bx], prec).get().dense()
+            ref = a @ b
+            if prec == E.PREC_FP64:
+                assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (M, K, N, case)
+            else:
+                bound = np.abs(a) @ np.abs(b)
+                assert (np.abs(got - ref) <= 1e-5 * bound + 1e-30).all(), (M, K, N, case)
+
+
+def test_lanczos_mixed_matches_fp64(ctx):
+    """FP32-MFMA matvec + FP64 residual restarts reach the FP64 ground state:
+    |dE|/|E| <= 1e-10 and state overlap >= 1 - 1e-9."""
+    rng = np.random.default_rng(77)
+    L, W1, W2, R, psi = H.hermitian_cent_problem(rng, dl=(9, 14, 7), dr=(8, 15, 6))
+    d = [ctx.put(x) for x in (L, W1, W2, R, psi)]
+    _, e64, g64 = E.lanczos(*d, "cent", 1e-12, 200, E.PREC_FP64)
+    _, emx, gmx = E.lanczos(*d, "cent", 1e-12, 200, E.PREC_MIXED)
+    assert abs(emx - e64) <= 1e-10 * abs(e64)
+    ov = abs(float((g64.get().dense() * gmx.get().dense()).sum()))
+    assert ov >= 1 - 1e-9

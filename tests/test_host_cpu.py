@@ -143,3 +143,22 @@ def test_oracle_sweep_reproduces_golden_energy(name):
         mps[0] = BTensor(h.legs, h.qw, {c: b / n for c, b in h.blocks.items()})
     r = O.dmrg(mps, mpo, case["sweeps"], case["dmin"], case["dmax"], case["trunc"], case["lerr"], case["lmax"])
     assert abs(r["energy"] - case["e0"]) < case["tol"], (r["energy"], case["e0"])
+
+
+def test_oracle_config1_heisenberg_l16_matches_ed():
+    """Heisenberg OBC L=16, U(1) Sz=0, Dmax=64, 4 sweeps, Lanczos tol 1e-10:
+    the oracle's final energy against exact diagonalisation."""
+    import json
+    import os
+    ed = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "ed_energies.json")))
+    m = HM.heisenberg_chain(16)
+    m.init_mps([[i % 2 for i in range(16)], [(i + 1) % 2 for i in range(16)]], enlarged_dim=4, seed=0x5EED,
+               centralize=False)
+    raw = [m.mps(i) for i in range(m.N)]
+    mps = O.centralize0(raw)
+    h = mps[0]
+    n = h.norm()
+    mps[0] = BTensor(h.legs, h.qw, {c: b / n for c, b in h.blocks.items()})
+    r = O.dmrg(mps, [m.mpo(i) for i in range(m.N)], 4, 1, 64, 0.0, 1e-10, 200)
+    assert abs(r["energy"] - ed["heisenberg_obc_L16_sz0"]) < 1e-8
+    assert r["dims"].max() == 64
