@@ -119,7 +119,12 @@ def main():
         r = model.dmrg(args.steps, 1, args.dmax, 0.0, le, lm, precision=prec)
         wall = time.perf_counter() - t0
         st = G.engine_stats()
-        res[name] = dict(r=r, wall=wall, stats=st)
+        cnt = engine.ctx_counters(G.lib().gqmps2_engine_ctx())
+        log(f"{name}: counters {cnt}")
+        res[name] = dict(r=r, wall=wall, stats=st, counters=cnt)
+        ph = r["phase_times"].sum(axis=0) / args.steps
+        res[name]["phase"] = dict(lanczos=round(float(ph[0]), 3), svd=round(float(ph[1]), 3), env=round(float(ph[2]), 3))
+        log(f"{name}: phase seconds per sweep {res[name]['phase']}")
         log(f"{name}: {args.steps} sweep(s) {wall:.2f}s, E={r['energy']:.14f}, maxD={r['dims'].max()}, "
             f"mean Lanczos iters {r['iters'].mean():.1f}, gemm flops {st['gemm_flops']:.3e}, "
             f"peak HBM {st['peak_bytes'] / 2**30:.1f} GiB")
@@ -179,6 +184,7 @@ def main():
                    "lanczos_max_iter": lm, "step": "one full sweep (2(N-1) two-site updates)"},
         "sweep_s": {"fp64": round(res["fp64"]["wall"] / args.steps, 3),
                     "mixed": round(res["mixed"]["wall"] / args.steps, 3)},
+        "sweep_phase_s": {"fp64": res["fp64"]["phase"], "mixed": res["mixed"]["phase"]},
         "sweep_speedup_mixed_vs_fp64": round(res["fp64"]["wall"] / res["mixed"]["wall"], 3),
         "matvec_tflops": {"fp64": round(tflops(mb["fp64"]), 4), "mixed": round(tflops(mb["mixed"]), 4)},
         "matvec_ms": {"fp64": round(mb["fp64"]["ms"], 4), "mixed": round(mb["mixed"]["ms"], 4)},

@@ -90,6 +90,7 @@ struct UpdateRecord {
   char dir;
   double energy, trunc_err, ee, lancz_time, update_time;
   size_t D, iters;
+  double svd_time, env_time;
 };
 
 // two_site_update_finite_vmps_impl.h:43-52
@@ -204,6 +205,7 @@ class TwoSiteVMPSEngine {
     auto lr = LanczosSolver(eff_ham, init_state, sp_.lancz_params, where);
     double lancz_t = lancz_timer.Elapsed();
 
+    Timer svd_timer("SVD");
     TenT u, vt;
     GQTensor<double, QNT> s;
     double err = 0;
@@ -212,6 +214,8 @@ class TwoSiteVMPSEngine {
         dir == 'r' ? SvdOrth::Left : SvdOrth::Right);
     delete lr.gs_vec;
     double ee = MeasureEE(s, D);
+    double svd_t = svd_timer.Elapsed();
+    Timer env_timer("env");
 
     if (dir == 'r') {
       mps_[ls] = std::move(u);
@@ -260,6 +264,8 @@ class TwoSiteVMPSEngine {
         }
       }
     }
+    CheckMps2(mps2_ctx_sync(DefaultCtx()), "sync");
+    double env_t = env_timer.Elapsed();
     double upd_t = update_timer.Elapsed();
     if (!quiet) {
       std::cout << "Site " << std::setw(4) << site << " E0 = " << std::setw(20)
@@ -270,7 +276,8 @@ class TwoSiteVMPSEngine {
                 << " S = " << std::setw(10) << std::setprecision(7) << ee;
       std::cout << std::scientific << std::endl;
     }
-    if (on_update) on_update(UpdateRecord{sweep_no, site, dir, lr.gs_eng, err, ee, lancz_t, upd_t, D, lr.iters});
+    if (on_update)
+      on_update(UpdateRecord{sweep_no, site, dir, lr.gs_eng, err, ee, lancz_t, upd_t, D, lr.iters, svd_t, env_t});
     return lr.gs_eng;
   }
 

@@ -33,6 +33,13 @@ const int kLanczEnergyOutputPrecision = 16;
 const std::vector<size_t> kNullUintVec;
 const std::vector<std::vector<size_t>> kNullUintVecVec;
 
+// Console output of the library (MPO bond dimensions, per-update lines) as in
+// the reference; FFI drivers that print machine-readable results switch it off.
+inline bool &Verbose() {
+  static bool v = true;
+  return v;
+}
+
 inline bool IsPathExist(const std::string &path) {
   struct stat buffer;
   return stat(path.c_str(), &buffer) == 0;

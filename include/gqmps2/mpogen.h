@@ -193,7 +193,7 @@ class MPOGenerator {
         for (auto &key : groups[q]) coor[b][key] = c++;
       }
       bonds[b] = IndexT(secs, GQTenIndexDirType::OUT);
-      std::cout << std::setw(3) << c << std::endl;   // bond dimension, as mpogen_impl.h:254-256
+      if (Verbose()) std::cout << std::setw(3) << c << std::endl;   // bond dimension, as mpogen_impl.h:254-256
     }
     // ---- realise the tensors
     MPO<GQTensorT> mpo(N_);

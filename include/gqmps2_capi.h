@@ -82,6 +82,12 @@ int gqmps2_bench_tensors(gqmps2_model m, mps2_ten *out5);
  * gqmps2_bench_tensors.  Leaves the MPS centred at `site`. */
 int gqmps2_bench_bond(gqmps2_model m, int64_t site, int warmup, int reps);
 
+/* Host wall-clock seconds (Lanczos, SVD, environment growth) of each update
+ * of the last gqmps2_dmrg call, 3 values per update, n values at most. */
+int gqmps2_phase_times(gqmps2_model m, double *out, int64_t n);
+/* Console output of the host layer on (1, the reference's behaviour) or off. */
+int gqmps2_set_verbose(int verbose);
+
 /* Engine context used by this library (for mps2_ctx_stats). */
 mps2_ctx gqmps2_engine_ctx(void);
 /* Message of this thread's last gqmps2_* failure. */

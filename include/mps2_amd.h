@@ -69,6 +69,10 @@ int mps2_ctx_sync(mps2_ctx ctx);
 int mps2_ctx_stats(mps2_ctx ctx, double *gemm_flops, int64_t *gemm_launches, int64_t *live_bytes,
                    int64_t *peak_bytes);
 int mps2_ctx_reset_stats(mps2_ctx ctx);
+/* Named event counters of the context ("svd_calls", "svd_sweeps",
+ * "svd_rounds", "syncs", ...); a name never counted reads 0.  Reset with
+ * mps2_ctx_reset_stats. */
+int mps2_ctx_counter(mps2_ctx ctx, const char *name, double *value);
 
 /* Tensors.  Replace GQTensor construction, element fill and the stream I/O
  * used by TenVec::LoadTen/DumpTen (one_dim_tn/framework/ten_vec.h:50-85). */

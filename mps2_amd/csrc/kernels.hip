@@ -98,7 +98,10 @@ void Ctx::trim() {
   cached_bytes = 0;
 }
 
-void Ctx::sync() { MPS2_HIP(hipStreamSynchronize(stream)); }
+void Ctx::sync() {
+  counters["syncs"] += 1;
+  MPS2_HIP(hipStreamSynchronize(stream));
+}
 
 TenP ten_alloc(Ctx *c, MetaP m, Prec p) {
   auto t = std::make_shared<Ten>();

@@ -274,8 +274,13 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
 
 LanczosOut lanczos(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
                    const std::string &where, double error, int64_t max_iter, Prec prec) {
-  if (prec == kF64) return lanczos_plain(L, W1, W2, R, init, where, error, max_iter);
-  return lanczos_mixed(L, W1, W2, R, init, where, error, max_iter);
+  Ctx *c = init->ctx;
+  LanczosOut r = prec == kF64 ? lanczos_plain(L, W1, W2, R, init, where, error, max_iter)
+                              : lanczos_mixed(L, W1, W2, R, init, where, error, max_iter);
+  c->count("lanczos_calls");
+  c->count("matvec_fp64", (double)r.matvecs_hi);
+  c->count("matvec_fp32", (double)r.matvecs_lo);
+  return r;
 }
 
 }  // namespace mps2

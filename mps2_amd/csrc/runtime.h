@@ -81,6 +81,9 @@ struct Ctx {
     double flops;
   };
   std::vector<GemmTiming> gemm_timings;
+  // named event counters (svd calls, Jacobi sweeps, host syncs, ...)
+  std::map<std::string, double> counters;
+  void count(const char *name, double v = 1.0) { counters[name] += v; }
 
   explicit Ctx(int dev);
   ~Ctx();

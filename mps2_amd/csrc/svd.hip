@@ -32,7 +32,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <map>
 
 #include "runtime.h"
@@ -44,6 +46,26 @@ namespace {
 constexpr int kB = 32;        // Jacobi block width
 constexpr int kP = 2 * kB;    // subproblem size
 constexpr double kDegTol = 1e-6;
+
+// Jacobi sweeps per 2b x 2b subproblem and per outer round (MPS2_JACOBI_INNER,
+// default 1), and the rotation threshold relative to trace(G_q)
+// (MPS2_JACOBI_TOL, default 1e-15).
+int jacobi_inner_sweeps() {
+  static int v = [] {
+    const char *e = std::getenv("MPS2_JACOBI_INNER");
+    int x = e ? std::atoi(e) : 1;
+    return x < 1 ? 1 : (x > 15 ? 15 : x);
+  }();
+  return v;
+}
+double jacobi_rel_tol() {
+  static double v = [] {
+    const char *e = std::getenv("MPS2_JACOBI_TOL");
+    double x = e ? std::atof(e) : 1e-15;
+    return x > 0 && x < 1e-6 ? x : 1e-15;
+  }();
+  return v;
+}
 
 BufP upload(Ctx *c, const void *h, size_t bytes) {
   auto b = c->alloc(std::max<size_t>(bytes, 16));
@@ -81,11 +103,6 @@ struct PairWork {    // one (sector, I, J) subproblem
   int32_t sec, I, J, slot;   // slot: Q storage index
 };
 
-struct ApplyWork {   // one (pair, other block) tile of the row / column update
-  int32_t pair;      // index into the round's PairWork list (absolute)
-  int32_t blk;
-};
-
 // Tournament (circle method) pairing of nb players in round r (nb even).
 void tournament(int nb, int r, std::vector<std::pair<int, int>> &out) {
   out.clear();
@@ -112,19 +129,27 @@ __device__ inline void jacobi_rot(double app, double aqq, double apq, double &c,
 
 // Diagonalise the 64x64 subproblem [[G_II, G_IJ], [G_IJ^T, G_JJ]] by cyclic
 // Jacobi in LDS; write the accumulated rotation Q (64x64, row-major).
-__global__ __launch_bounds__(256) void pair_eig_kernel(const double *G, const SecDesc *secs, const PairWork *work,
-                                                       double *Qout, int *rotated) {
+// Each of the 63 rounds of a sweep rotates 32 disjoint index pairs (p_k, q_k).
+// The 32 x 32 blocks {p_k, q_k} x {p_k', q_k'} of S are disjoint, so thread
+// (k, k') applies both rotations to its own 2x2 block in one step
+// (S <- G S G^T), and 64 x 32 thread-pairs rotate the columns of Q.
+constexpr int kEigThreads = 1024;
+
+__global__ __launch_bounds__(kEigThreads) void pair_eig_kernel(const double *G, const SecDesc *secs,
+                                                               const PairWork *work, double *Qout, int *rotated,
+                                                               const int *active, int max_inner) {
   __shared__ double S[kP][kP + 1];
   __shared__ double Q[kP][kP + 1];
   __shared__ double cs[kP / 2], sn[kP / 2];
   __shared__ int pp[kP / 2], qq[kP / 2];
   __shared__ int any_rot, any_sweep;
   const PairWork w = work[blockIdx.x];
+  if (!active[w.sec]) return;
   const SecDesc sd = secs[w.sec];
   const double *g = G + sd.g_off;
   const int np = sd.np;
   const int tid = threadIdx.x;
-  for (int e = tid; e < kP * kP; e += 256) {
+  for (int e = tid; e < kP * kP; e += kEigThreads) {
     int i = e / kP, j = e % kP;
     int gi = (i < kB ? w.I * kB + i : w.J * kB + (i - kB));
     int gj = (j < kB ? w.I * kB + j : w.J * kB + (j - kB));
@@ -135,7 +160,8 @@ __global__ __launch_bounds__(256) void pair_eig_kernel(const double *G, const Se
   if (tid == 0) any_rot = 0;
   __syncthreads();
   const double tau = sd.tau;
-  for (int sweep = 0; sweep < 15; ++sweep) {
+  const int bk = tid >> 5, bk2 = tid & 31;   // this thread's 2x2 block of S
+  for (int sweep = 0; sweep < max_inner; ++sweep) {
     if (tid == 0) any_sweep = 0;
     __syncthreads();
     for (int r = 0; r < kP - 1; ++r) {
@@ -168,29 +194,35 @@ __global__ __launch_bounds__(256) void pair_eig_kernel(const double *G, const Se
         }
       }
       __syncthreads();
-      // rows p, q of S
-      for (int e = tid; e < (kP / 2) * kP; e += 256) {
-        int k = e / kP, j = e % kP;
-        double c = cs[k], s = sn[k];
-        if (s == 0.0) continue;
-        int p = pp[k], q = qq[k];
-        double x = S[p][j], y = S[q][j];
-        S[p][j] = c * x - s * y;
-        S[q][j] = s * x + c * y;
+      {
+        const double c1 = cs[bk], s1 = sn[bk], c2 = cs[bk2], s2 = sn[bk2];
+        if (s1 != 0.0 || s2 != 0.0) {
+          const int p = pp[bk], q = qq[bk], p2 = pp[bk2], q2 = qq[bk2];
+          const double a = S[p][p2], b = S[p][q2], cc = S[q][p2], d = S[q][q2];
+          const double r0 = c1 * a - s1 * cc, r1 = c1 * b - s1 * d;   // rotated rows
+          const double r2 = s1 * a + c1 * cc, r3 = s1 * b + c1 * d;
+          double npp = c2 * r0 - s2 * r1, npq = s2 * r0 + c2 * r1;   // rotated columns
+          double nqp = c2 * r2 - s2 * r3, nqq = s2 * r2 + c2 * r3;
+          if (bk == bk2) {   // the pivot block: its off-diagonal is annihilated
+            npq = 0.0;
+            nqp = 0.0;
+          }
+          S[p][p2] = npp;
+          S[p][q2] = npq;
+          S[q][p2] = nqp;
+          S[q][q2] = nqq;
+        }
       }
-      __syncthreads();
-      // columns p, q of S and Q
-      for (int e = tid; e < (kP / 2) * kP; e += 256) {
-        int k = e / kP, i = e % kP;
-        double c = cs[k], s = sn[k];
-        if (s == 0.0) continue;
-        int p = pp[k], q = qq[k];
-        double x = S[i][p], y = S[i][q];
-        S[i][p] = c * x - s * y;
-        S[i][q] = s * x + c * y;
-        double u = Q[i][p], v = Q[i][q];
-        Q[i][p] = c * u - s * v;
-        Q[i][q] = s * u + c * v;
+#pragma unroll
+      for (int u = 0; u < (kP / 2) * kP / kEigThreads; ++u) {
+        const int e = tid + u * kEigThreads, k = e >> 6, i = e & 63;
+        const double c = cs[k], s = sn[k];
+        if (s != 0.0) {
+          const int p = pp[k], q = qq[k];
+          const double x = Q[i][p], y = Q[i][q];
+          Q[i][p] = c * x - s * y;
+          Q[i][q] = s * x + c * y;
+        }
       }
       __syncthreads();
     }
@@ -200,65 +232,88 @@ __global__ __launch_bounds__(256) void pair_eig_kernel(const double *G, const Se
     if (done) break;
   }
   double *qo = Qout + (int64_t)w.slot * kP * kP;
-  for (int e = tid; e < kP * kP; e += 256) qo[e] = Q[e / kP][e % kP];
+  for (int e = tid; e < kP * kP; e += kEigThreads) qo[e] = Q[e / kP][e % kP];
   if (tid == 0 && any_rot) atomicOr(rotated + w.sec, 1);
 }
 
-// Rows: [G_I*; G_J*] <- Q^T [G_I*; G_J*] for one column block.
-__global__ __launch_bounds__(256) void apply_rows_kernel(double *G, const SecDesc *secs, const PairWork *pw,
-                                                         const ApplyWork *aw, const double *Qall) {
-  __shared__ double Q[kP][kP + 1];
-  __shared__ double X[kP][kB + 1];
-  const ApplyWork a = aw[blockIdx.x];
-  const PairWork w = pw[a.pair];
-  const SecDesc sd = secs[w.sec];
-  double *g = G + sd.g_off;
+// One 64x64 tile of the round's update.  The round's pairs partition the
+// column blocks, so tiles (pair P rows, pair P2 columns) partition the matrix
+// and every tile is read and written by its own workgroup only:
+//   mat 0:  G[P, P2] <- Q_P^T G[P, P2] Q_P2
+//   mat 1:  V[P, P2] <- V[P, P2] Q_P2
+struct TileWork {
+  int32_t P, P2;   // absolute indices into the PairWork list (same sector)
+  int32_t mat, pad;
+};
+
+__device__ inline int pair_row(const PairWork &w, int i) { return i < kB ? w.I * kB + i : w.J * kB + (i - kB); }
+
+__global__ __launch_bounds__(256) void apply_tiles_kernel(double *G, double *V, const SecDesc *secs,
+                                                          const PairWork *pw, const TileWork *tw, const double *Qall,
+                                                          const int *active) {
+  __shared__ double X[kP][kP + 1];
+  __shared__ double Q1[kP][kP + 1];
+  __shared__ double Q2[kP][kP + 1];
+  const TileWork t = tw[blockIdx.x];
+  const PairWork w1 = pw[t.P], w2 = pw[t.P2];
+  if (!active[w1.sec]) return;
+  const SecDesc sd = secs[w1.sec];
   const int np = sd.np, tid = threadIdx.x;
-  const double *qs = Qall + (int64_t)w.slot * kP * kP;
-  for (int e = tid; e < kP * kP; e += 256) Q[e / kP][e % kP] = qs[e];
-  for (int e = tid; e < kP * kB; e += 256) {
-    int i = e / kB, j = e % kB;
-    int gi = i < kB ? w.I * kB + i : w.J * kB + (i - kB);
-    X[i][j] = g[(int64_t)gi * np + a.blk * kB + j];
+  double *g = (t.mat == 0 ? G : V) + sd.g_off;
+  const double *q1 = Qall + (int64_t)w1.slot * kP * kP;
+  const double *q2 = Qall + (int64_t)w2.slot * kP * kP;
+  for (int e = tid; e < kP * kP; e += 256) {
+    int i = e / kP, j = e % kP;
+    X[i][j] = g[(int64_t)pair_row(w1, i) * np + pair_row(w2, j)];
+    Q2[i][j] = q2[e];
+    if (t.mat == 0) Q1[i][j] = q1[e];
   }
   __syncthreads();
-  for (int e = tid; e < kP * kB; e += 256) {
-    int i = e / kB, j = e % kB;
-    double s = 0;
-    for (int k = 0; k < kP; ++k) s += Q[k][i] * X[k][j];
-    int gi = i < kB ? w.I * kB + i : w.J * kB + (i - kB);
-    g[(int64_t)gi * np + a.blk * kB + j] = s;
-  }
-}
-
-// Columns: [G_*I, G_*J] <- [G_*I, G_*J] Q and the same for V, one row block.
-__global__ __launch_bounds__(256) void apply_cols_kernel(double *G, double *V, const SecDesc *secs,
-                                                         const PairWork *pw, const ApplyWork *aw, const double *Qall) {
-  __shared__ double Q[kP][kP + 1];
-  __shared__ double X[kB][kP + 1];
-  const ApplyWork a = aw[blockIdx.x];
-  const PairWork w = pw[a.pair];
-  const SecDesc sd = secs[w.sec];
-  const int np = sd.np, tid = threadIdx.x;
-  const double *qs = Qall + (int64_t)w.slot * kP * kP;
-  for (int e = tid; e < kP * kP; e += 256) Q[e / kP][e % kP] = qs[e];
-  for (int mat = 0; mat < 2; ++mat) {
-    double *g = (mat == 0 ? G : V) + sd.g_off;
-    __syncthreads();
-    for (int e = tid; e < kB * kP; e += 256) {
-      int i = e / kP, j = e % kP;
-      int gj = j < kB ? w.I * kB + j : w.J * kB + (j - kB);
-      X[i][j] = g[(int64_t)(a.blk * kB + i) * np + gj];
+  // 64x64x64 products on FP64 MFMA (16x16x4): wave w owns output rows
+  // 16*w .. 16*w+15 (four 16x16 fragments along the columns).  Lane l feeds
+  // A[l%16][l/16] and B[l/16][l%16]; accumulator element r of lane l is row
+  // l/16 + 4r, column l%16 of the fragment.
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  const int lane = tid & 63, wv = tid >> 6;
+  const int i0 = 16 * wv;
+  d4 acc[4];
+  if (t.mat == 0) {
+    // T = Q1^T X
+#pragma unroll
+    for (int f = 0; f < 4; ++f) acc[f] = d4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < kP; k0 += 4) {
+      const double a = Q1[k0 + (lane >> 4)][i0 + (lane & 15)];
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const double b = X[k0 + (lane >> 4)][16 * f + (lane & 15)];
+        acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[f], 0, 0, 0);
+      }
     }
     __syncthreads();
-    for (int e = tid; e < kB * kP; e += 256) {
-      int i = e / kP, j = e % kP;
-      double s = 0;
-      for (int k = 0; k < kP; ++k) s += X[i][k] * Q[k][j];
-      int gj = j < kB ? w.I * kB + j : w.J * kB + (j - kB);
-      g[(int64_t)(a.blk * kB + i) * np + gj] = s;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) X[i0 + (lane >> 4) + 4 * r][16 * f + (lane & 15)] = acc[f][r];
+    __syncthreads();
+  }
+  // Y = X Q2
+#pragma unroll
+  for (int f = 0; f < 4; ++f) acc[f] = d4{0, 0, 0, 0};
+  for (int k0 = 0; k0 < kP; k0 += 4) {
+    const double a = X[i0 + (lane & 15)][k0 + (lane >> 4)];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const double b = Q2[k0 + (lane >> 4)][16 * f + (lane & 15)];
+      acc[f] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[f], 0, 0, 0);
     }
   }
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = i0 + (lane >> 4) + 4 * r, j = 16 * f + (lane & 15);
+      g[(int64_t)pair_row(w1, i) * np + pair_row(w2, j)] = acc[f][r];
+    }
 }
 
 // Copy the n x n Gram into the padded np x np layout; set V = I.
@@ -332,7 +387,7 @@ EigOut batched_eigh(Ctx *c, const double *Gsrc, const std::vector<int64_t> &goff
     eo.secs[q].g_off = tot;
     eo.secs[q].nb = nb;
     eo.secs[q].np = nb * kB;
-    eo.secs[q].tau = std::max(1e-15 * traces[q], 1e-300);
+    eo.secs[q].tau = std::max(jacobi_rel_tol() * traces[q], 1e-300);
     tot += (int64_t)nb * kB * nb * kB;
     maxnb = std::max(maxnb, nb);
   }
@@ -343,55 +398,67 @@ EigOut batched_eigh(Ctx *c, const double *Gsrc, const std::vector<int64_t> &goff
     pad_init_kernel<<<gridn((int64_t)eo.secs[q].np * eo.secs[q].np), 256, 0, c->stream>>>(
         Gsrc, goffs[q], ns[q], G, V, eo.secs[q].g_off, eo.secs[q].np);
   auto dsecs = upload(c, eo.secs.data(), nsec * sizeof(SecDesc));
-  // Work lists for every round of one sweep.
+  // Work lists for every round of one sweep: pair subproblems, then the
+  // (pair rows x pair columns) tiles of G and V.
   std::vector<PairWork> pw;
-  std::vector<ApplyWork> aw;
-  std::vector<int> round_p0(maxnb, 0), round_p1(maxnb, 0), round_a0(maxnb, 0), round_a1(maxnb, 0);
+  std::vector<TileWork> tw;
+  std::vector<int> round_p0(maxnb, 0), round_p1(maxnb, 0), round_t0(maxnb, 0), round_t1(maxnb, 0);
   std::vector<std::pair<int, int>> tp;
   int slots = 0;
   for (int r = 0; r < maxnb - 1; ++r) {
     round_p0[r] = (int)pw.size();
-    round_a0[r] = (int)aw.size();
+    round_t0[r] = (int)tw.size();
     int slot = 0;
     for (int q = 0; q < nsec; ++q) {
       int nb = eo.secs[q].nb;
       if (r >= nb - 1) continue;
       tournament(nb, r, tp);
-      for (auto &pq : tp) {
-        int pidx = (int)pw.size();
-        pw.push_back(PairWork{q, pq.first, pq.second, slot++});
-        for (int bk = 0; bk < nb; ++bk) aw.push_back(ApplyWork{pidx, bk});
-      }
+      int first = (int)pw.size();
+      for (auto &pq : tp) pw.push_back(PairWork{q, pq.first, pq.second, slot++});
+      int last = (int)pw.size();
+      for (int a = first; a < last; ++a)
+        for (int b = first; b < last; ++b) {
+          tw.push_back(TileWork{a, b, 0, 0});
+          tw.push_back(TileWork{a, b, 1, 0});
+        }
     }
     slots = std::max(slots, slot);
     round_p1[r] = (int)pw.size();
-    round_a1[r] = (int)aw.size();
+    round_t1[r] = (int)tw.size();
   }
   auto dpw = upload(c, pw.data(), pw.size() * sizeof(PairWork));
-  auto daw = upload(c, aw.data(), aw.size() * sizeof(ApplyWork));
+  auto dtw = upload(c, tw.data(), tw.size() * sizeof(TileWork));
   auto dq = c->alloc((int64_t)std::max(slots, 1) * kP * kP * sizeof(double));
   auto drot = c->alloc(std::max(nsec, 1) * sizeof(int));
+  auto dact = c->alloc(std::max(nsec, 1) * sizeof(int));
   const PairWork *PW = static_cast<const PairWork *>(dpw->p);
-  const ApplyWork *AW = static_cast<const ApplyWork *>(daw->p);
+  const TileWork *TW = static_cast<const TileWork *>(dtw->p);
   const SecDesc *SD = static_cast<const SecDesc *>(dsecs->p);
   double *Q = static_cast<double *>(dq->p);
   int *ROT = static_cast<int *>(drot->p);
-  std::vector<int> hrot(nsec);
-  for (int sweep = 0; sweep < 30; ++sweep) {
+  int *ACT = static_cast<int *>(dact->p);
+  std::vector<int> hrot(nsec), hact(nsec, 1);
+  MPS2_HIP(hipMemcpyAsync(ACT, hact.data(), nsec * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  for (int sweep = 0; sweep < 40; ++sweep) {
     MPS2_HIP(hipMemsetAsync(ROT, 0, nsec * sizeof(int), c->stream));
     for (int r = 0; r < maxnb - 1; ++r) {
-      int np_ = round_p1[r] - round_p0[r], na = round_a1[r] - round_a0[r];
+      int np_ = round_p1[r] - round_p0[r], nt = round_t1[r] - round_t0[r];
       if (np_ == 0) continue;
-      pair_eig_kernel<<<np_, 256, 0, c->stream>>>(G, SD, PW + round_p0[r], Q, ROT);
-      // ApplyWork.pair is absolute; Q slots are per round -> kernels index PW absolutely
-      apply_rows_kernel<<<na, 256, 0, c->stream>>>(G, SD, PW, AW + round_a0[r], Q);
-      apply_cols_kernel<<<na, 256, 0, c->stream>>>(G, V, SD, PW, AW + round_a0[r], Q);
+      pair_eig_kernel<<<np_, kEigThreads, 0, c->stream>>>(G, SD, PW + round_p0[r], Q, ROT, ACT,
+                                                          jacobi_inner_sweeps());
+      apply_tiles_kernel<<<nt, 256, 0, c->stream>>>(G, V, SD, PW, TW + round_t0[r], Q, ACT);
     }
     MPS2_HIP(hipMemcpyAsync(hrot.data(), ROT, nsec * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     c->sync();
     bool any = false;
-    for (int q = 0; q < nsec; ++q) any |= hrot[q] != 0;
+    for (int q = 0; q < nsec; ++q) {
+      hact[q] = hact[q] && hrot[q] != 0;
+      any |= hact[q] != 0;
+    }
+    c->count("svd_sweeps");
+    c->count("svd_rounds", maxnb - 1);
     if (!any) break;
+    MPS2_HIP(hipMemcpyAsync(ACT, hact.data(), nsec * sizeof(int), hipMemcpyHostToDevice, c->stream));
   }
   // eigenvalues
   eo.evals.resize(nsec);
@@ -470,7 +537,19 @@ void truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err,
 
 }  // namespace
 
+SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax,
+                OrthSide side);
+
 SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax, OrthSide side) {
+  auto t0 = std::chrono::steady_clock::now();
+  SvdOut o = svd_impl(t, ldims, ldiv, trunc_err, dmin, dmax, side);
+  t->ctx->count("svd_calls");
+  t->ctx->count("svd_ms", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+  return o;
+}
+
+SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax,
+                OrthSide side) {
   MPS2_REQUIRE(t->prec == kF64, "SVD: FP64 tensors only");
   Ctx *c = t->ctx;
   const Meta &m = *t->meta;
@@ -592,7 +671,15 @@ SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t d
     for (int q = 0; q < ns; ++q)
       for (double x : dg[q]) traces[q] += x;
   }
+  auto te0 = std::chrono::steady_clock::now();
   EigOut eo = batched_eigh(c, Gd, goff, gn, traces);
+  c->count("svd_eigh_ms", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te0).count());
+  {
+    int maxn = 0;
+    for (int x : gn) maxn = std::max(maxn, x);
+    c->count("svd_sectors", ns);
+    c->count("svd_max_sector_sum", maxn);
+  }
   // 5. singular values: top min(m, n) eigenvalues per sector, descending
   std::vector<std::vector<double>> sv(ns);
   std::vector<std::vector<int32_t>> order(ns);

@@ -236,3 +236,18 @@ def contract_ex(a: DTen, b: DTen, axes, precision=PREC_FP64) -> DTen:
     check(lib().mps2_contract_ex(a.h, b.h, len(xa), xa.ctypes.data_as(_i32p), xb.ctypes.data_as(_i32p), precision,
                                  C.byref(h)))
     return DTen(h, a.ctx)
+
+
+SIGNATURES["mps2_ctx_counter"] = (C.c_int, [_vp, C.c_char_p, _dp])
+COUNTERS = ("svd_calls", "svd_ms", "svd_eigh_ms", "svd_sweeps", "svd_rounds", "svd_sectors", "svd_max_sector_sum",
+            "syncs", "lanczos_calls", "matvec_fp64", "matvec_fp32")
+
+
+def ctx_counters(ctx_handle, names=COUNTERS):
+    """Read the named counters of a context handle (c_void_p)."""
+    out = {}
+    v = C.c_double()
+    for n in names:
+        check(lib().mps2_ctx_counter(ctx_handle, n.encode(), C.byref(v)))
+        out[n] = v.value
+    return out

@@ -42,6 +42,8 @@ SIGNATURES = {
     "gqmps2_get_bench": (C.c_int, [_vp, _dp]),
     "gqmps2_bench_tensors": (C.c_int, [_vp, C.POINTER(_vp)]),
     "gqmps2_bench_bond": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int]),
+    "gqmps2_phase_times": (C.c_int, [_vp, _dp, C.c_int64]),
+    "gqmps2_set_verbose": (C.c_int, [C.c_int]),
     "gqmps2_engine_ctx": (_vp, []),
     "gqmps2_last_error": (C.c_char_p, []),
 }
@@ -59,7 +61,13 @@ def lib():
             f.restype = res
             f.argtypes = args
         _lib = L
+        # stdout stays machine-readable for FFI callers; set_verbose(True) restores the log lines
+        L.gqmps2_set_verbose(0)
     return _lib
+
+
+def set_verbose(on: bool):
+    check(lib().gqmps2_set_verbose(1 if on else 0))
 
 
 def check(rc):
@@ -196,7 +204,10 @@ class Model:
                                 en.ctypes.data_as(_dp), dims.ctypes.data_as(_i64p), its.ctypes.data_as(_i64p),
                                 te.ctypes.data_as(_dp), ut.ctypes.data_as(_dp), st.ctypes.data_as(_dp),
                                 C.byref(e0)))
+        ph = np.zeros(3 * nup)
+        check(lib().gqmps2_phase_times(self.h, ph.ctypes.data_as(_dp), 3 * nup))
         return dict(energy=e0.value, energies=en, dims=dims, iters=its, trunc_errs=te, update_times=ut,
+                    phase_times=ph.reshape(nup, 3),
                     sweep_times=st)
 
 

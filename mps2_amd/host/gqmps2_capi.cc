@@ -39,6 +39,7 @@ struct ModelBase {
   int bench_warmup = 2, bench_reps = 10;
   bool bench_done = false;
   double bench_out[22] = {0};
+  std::vector<double> phase_times;   // (lanczos, svd, env) seconds per update of the last dmrg call
   std::vector<mps2_ten> bench_tens = std::vector<mps2_ten>(5, nullptr);
   void clear_bench_tens() {
     for (auto &h : bench_tens) {
@@ -288,12 +289,16 @@ struct Model : ModelBase {
                              prec == MPS2_PREC_MIXED ? gqmps2::LanczosPrecision::Mixed : gqmps2::LanczosPrecision::FP64);
     double e0 = 0;
     int64_t k = 0;
+    phase_times.clear();
     auto record = [&](const gqmps2::UpdateRecord &r) {
       if (en) en[k] = r.energy;
       if (dims) dims[k] = (int64_t)r.D;
       if (its) its[k] = (int64_t)r.iters;
       if (te) te[k] = r.trunc_err;
       if (ut) ut[k] = r.update_time;
+      phase_times.push_back(r.lancz_time);
+      phase_times.push_back(r.svd_time);
+      phase_times.push_back(r.env_time);
       ++k;
     };
     std::unique_ptr<gqmps2::TwoSiteVMPSEngine<double, QNT>> eng;
@@ -477,6 +482,18 @@ int gqmps2_set_bench(gqmps2_model m, int64_t site, int warmup, int reps) {
     m->m->bench_reps = reps;
     m->m->bench_done = false;
   });
+}
+
+int gqmps2_phase_times(gqmps2_model m, double *out, int64_t n) {
+  return guard([&] {
+    if (!m || !out) throw std::invalid_argument("NULL argument");
+    auto &v = m->m->phase_times;
+    for (int64_t i = 0; i < n && i < (int64_t)v.size(); ++i) out[i] = v[i];
+  });
+}
+
+int gqmps2_set_verbose(int verbose) {
+  return guard([&] { gqmps2::Verbose() = verbose != 0; });
 }
 
 int gqmps2_bench_bond(gqmps2_model m, int64_t site, int warmup, int reps) {
