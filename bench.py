@@ -98,11 +98,10 @@ def main():
     ramp = [int(x) for x in args.ramp.split(",") if x]
     ramp = [d for d in ramp if d < args.dmax]
     le, lm = args.lanczos_error, args.lanczos_max_iter
-    if ramp:
+    for d in ramp:
         t0 = time.perf_counter()
-        r = model.dmrg(len(ramp), 1, args.dmax, 0.0, le, lm, dmax_schedule=ramp)
-        log(f"ramp {ramp}: E={r['energy']:.12f} in {time.perf_counter() - t0:.1f}s "
-            f"(sweeps {[round(x, 1) for x in r['sweep_times']]})")
+        r = model.dmrg(1, 1, d, 0.0, le, lm)
+        log(f"ramp sweep Dmax={d}: E={r['energy']:.12f} maxD={r['dims'].max()} in {time.perf_counter() - t0:.1f}s")
     for k in range(args.warmup):
         t0 = time.perf_counter()
         r = model.dmrg(1, 1, args.dmax, 0.0, le, lm)
