@@ -45,17 +45,96 @@ struct Mfma<float> {
 constexpr int kThreads = 256;
 constexpr int kBK = 16;
 
+// Per-thread share of one BK-deep slab of A (BM x BK) and B (BK x BN),
+// loaded from global memory into registers, then stored to LDS as
+// As[k][m], Bs[k][n].  Out-of-range elements read as zero.
+template <typename T, int BM, int BN, bool TA, bool TB>
+struct Slab {
+  static constexpr int NA = BM * kBK / kThreads;
+  static constexpr int NB = BN * kBK / kThreads;
+  static constexpr int PA = BM + 1, PB = BN + 1;
+  T ra[NA], rb[NB];
+
+  __device__ __forceinline__ void load(const T *__restrict__ Ab, const T *__restrict__ Bb, int M, int N, int K,
+                                       int m0, int n0, int k0, int tid) {
+    if (!TA) {
+      constexpr int RSTEP = kThreads / kBK;
+      const int kk = tid % kBK, gk = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int gm = m0 + tid / kBK + i * RSTEP;
+        ra[i] = (gm < M && gk < K) ? Ab[(int64_t)gm * K + gk] : T(0);
+      }
+    } else {
+      constexpr int KSTEP = kThreads / BM;
+      const int gm = m0 + tid % BM;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int gk = k0 + tid / BM + i * KSTEP;
+        ra[i] = (gm < M && gk < K) ? Ab[(int64_t)gk * M + gm] : T(0);
+      }
+    }
+    if (!TB) {
+      constexpr int KSTEP = kThreads / BN;
+      const int gn = n0 + tid % BN;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int gk = k0 + tid / BN + i * KSTEP;
+        rb[i] = (gn < N && gk < K) ? Bb[(int64_t)gk * N + gn] : T(0);
+      }
+    } else {
+      constexpr int RSTEP = kThreads / kBK;
+      const int kk = tid % kBK, gk = k0 + kk;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int gn = n0 + tid / kBK + i * RSTEP;
+        rb[i] = (gn < N && gk < K) ? Bb[(int64_t)gn * K + gk] : T(0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(T *As, T *Bs, int tid) const {
+    if (!TA) {
+      constexpr int RSTEP = kThreads / kBK;
+      const int kk = tid % kBK;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) As[kk * PA + tid / kBK + i * RSTEP] = ra[i];
+    } else {
+      constexpr int KSTEP = kThreads / BM;
+      const int mm = tid % BM;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) As[(tid / BM + i * KSTEP) * PA + mm] = ra[i];
+    }
+    if (!TB) {
+      constexpr int KSTEP = kThreads / BN;
+      const int nn = tid % BN;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) Bs[(tid / BN + i * KSTEP) * PB + nn] = rb[i];
+    } else {
+      constexpr int RSTEP = kThreads / kBK;
+      const int kk = tid % kBK;
+#pragma unroll
+      for (int i = 0; i < NB; ++i) Bs[kk * PB + tid / kBK + i * RSTEP] = rb[i];
+    }
+  }
+};
+
+// The (block pair, K slab) sequence of a tile is walked as one flat loop so
+// the global loads of the next slab (possibly of the next pair) overlap the
+// matrix-core work on the current one: registers -> LDS double buffer, one
+// barrier per slab.
 template <typename T, int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
                                                                  T *__restrict__ C, const TileD *__restrict__ tiles,
                                                                  const BlkD *__restrict__ blks,
                                                                  const PairD *__restrict__ pairs, int ntiles) {
   using V4 = typename Mfma<T>::v4;
+  using SlabT = Slab<T, BM, BN, TA, TB>;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int PA = BM + 1, PB = BN + 1;
-  __shared__ T As[kBK * PA];
-  __shared__ T Bs[kBK * PB];
+  constexpr int PA = SlabT::PA, PB = SlabT::PB;
+  __shared__ T As[2][kBK * PA];
+  __shared__ T Bs[2][kBK * PB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -74,76 +153,45 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
 
-  for (int p = blk.pb; p < blk.pe; ++p) {
-    const PairD pr = pairs[p];
-    const int K = pr.K;
-    const T *Ab = A + pr.a_off;
-    const T *Bb = B + pr.b_off;
-    for (int k0 = 0; k0 < K; k0 += kBK) {
-      // ---- stage A slab (BM x BK) into As[k][m]
-      if (!TA) {
-        constexpr int RSTEP = kThreads / kBK;
-        const int kk = tid % kBK;
-#pragma unroll
-        for (int i = 0; i < BM / RSTEP; ++i) {
-          const int mm = tid / kBK + i * RSTEP;
-          const int gm = m0 + mm, gk = k0 + kk;
-          T v = 0;
-          if (gm < M && gk < K) v = Ab[(int64_t)gm * K + gk];
-          As[kk * PA + mm] = v;
-        }
-      } else {
-        constexpr int KSTEP = kThreads / BM;
-        const int mm = tid % BM;
-#pragma unroll
-        for (int i = 0; i < kBK / KSTEP; ++i) {
-          const int kk = tid / BM + i * KSTEP;
-          const int gm = m0 + mm, gk = k0 + kk;
-          T v = 0;
-          if (gm < M && gk < K) v = Ab[(int64_t)gk * M + gm];
-          As[kk * PA + mm] = v;
-        }
-      }
-      // ---- stage B slab (BK x BN) into Bs[k][n]
-      if (!TB) {
-        constexpr int KSTEP = kThreads / BN;
-        const int nn = tid % BN;
-#pragma unroll
-        for (int i = 0; i < kBK / KSTEP; ++i) {
-          const int kk = tid / BN + i * KSTEP;
-          const int gn = n0 + nn, gk = k0 + kk;
-          T v = 0;
-          if (gn < N && gk < K) v = Bb[(int64_t)gk * N + gn];
-          Bs[kk * PB + nn] = v;
-        }
-      } else {
-        constexpr int RSTEP = kThreads / kBK;
-        const int kk = tid % kBK;
-#pragma unroll
-        for (int i = 0; i < BN / RSTEP; ++i) {
-          const int nn = tid / kBK + i * RSTEP;
-          const int gn = n0 + nn, gk = k0 + kk;
-          T v = 0;
-          if (gn < N && gk < K) v = Bb[(int64_t)gn * K + gk];
-          Bs[kk * PB + nn] = v;
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int ks = 0; ks < kBK; ks += 4) {
-        const int kr = ks + (lane >> 4);
-        T af[FM], bf[FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) af[i] = As[kr * PA + wm * WM + i * 16 + (lane & 15)];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) bf[j] = Bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
-      }
-      __syncthreads();
+  SlabT slab;
+  int p = blk.pb, k0 = 0;
+  PairD pr = pairs[p];
+  slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, 0, tid);
+  slab.store(As[0], Bs[0], tid);
+  __syncthreads();
+  int cur = 0;
+  while (true) {
+    int np = p, nk0 = k0 + kBK;
+    PairD npr = pr;
+    if (nk0 >= pr.K) {
+      np = p + 1;
+      nk0 = 0;
+      if (np < blk.pe) npr = pairs[np];
     }
+    const bool has_next = np < blk.pe;
+    if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
+    const T *as = As[cur];
+    const T *bs = Bs[cur];
+#pragma unroll
+    for (int ks = 0; ks < kBK; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = as[kr * PA + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+    }
+    if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
+    __syncthreads();
+    if (!has_next) break;
+    p = np;
+    k0 = nk0;
+    pr = npr;
+    cur ^= 1;
   }
   // ---- epilogue: element r of each 16x16 fragment is row Mfma<T>::row(lane, r), column lane%16
   T *Cb = C + blk.c_off;

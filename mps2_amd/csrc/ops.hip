@@ -64,14 +64,21 @@ void perm_descs(const Meta &m, const std::vector<int> &perm, std::vector<PermDes
 std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPlan> &hp, const Meta *target) {
   auto dp = std::make_shared<DevPlan>();
   dp->hp = hp;
-  // Tile shape: large tiles when most flops sit in blocks >= 96 on both sides.
-  double big = 0;
+  // Tile shape: 128x128 tiles move half the operand bytes per flop of 64x64
+  // tiles; use them unless edge padding inflates the work by more than 30%
+  // or there are too few of them to occupy every compute unit.
+  double w64 = 0, w128 = 0;
+  int64_t t128 = 0;
   for (auto &b : hp->blks) {
-    double f = 0;
-    for (int p = b.pair_begin; p < b.pair_end; ++p) f += 2.0 * b.M * b.N * hp->pairs[p].K;
-    if (b.M >= 96 && b.N >= 96) big += f;
+    double kc = 0;
+    for (int p = b.pair_begin; p < b.pair_end; ++p) kc += hp->pairs[p].K;
+    int64_t n64 = (int64_t)((b.M + 63) / 64) * ((b.N + 63) / 64);
+    int64_t n128 = (int64_t)((b.M + 127) / 128) * ((b.N + 127) / 128);
+    w64 += (double)n64 * 4096.0 * kc;
+    w128 += (double)n128 * 16384.0 * kc;
+    t128 += n128;
   }
-  int ts = (hp->flops > 0 && big > 0.5 * hp->flops) ? 128 : 64;
+  int ts = (w128 <= 1.3 * w64 && t128 >= c->num_cu) ? 128 : 64;
   dp->tile_m = dp->tile_n = ts;
   std::vector<BlkD> blks(hp->blks.size());
   std::vector<PairD> pairs(hp->pairs.size());
