@@ -127,7 +127,7 @@ template <typename T, int BM, int BN, bool TA, bool TB>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
                                                                  T *__restrict__ C, const TileD *__restrict__ tiles,
                                                                  const BlkD *__restrict__ blks,
-                                                                 const PairD *__restrict__ pairs, int ntiles) {
+                                                                 const PairD *__restrict__ pairs, int ntiles, int sub) {
   using V4 = typename Mfma<T>::v4;
   using SlabT = Slab<T, BM, BN, TA, TB>;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -146,6 +146,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   const BlkD blk = blks[tile.blk];
   const int M = blk.M, N = blk.N;
   const int m0 = tile.m0, n0 = tile.n0;
+  // Tiles listed for an upper bound of a block whose size is set on the device.
+  if (m0 >= M || n0 >= N || blk.pb >= blk.pe) return;
 
   V4 acc[FM][FN];
 #pragma unroll
@@ -203,39 +205,42 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int gm = m0 + wm * WM + i * 16 + Mfma<T>::row(lane, r);
-        if (gm < M && gn < N) Cb[(int64_t)gm * N + gn] = acc[i][j][r];
+        if (gm < M && gn < N) {
+          if (sub) Cb[(int64_t)gm * N + gn] -= acc[i][j][r];
+          else Cb[(int64_t)gm * N + gn] = acc[i][j][r];
+        }
       }
     }
 }
 
 template <typename T, int BM, int BN>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                  const BlkD *blks, const PairD *pairs) {
+                  const BlkD *blks, const PairD *pairs, int sub) {
   dim3 grid(ntiles), block(kThreads);
   if (la == 0 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, false, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+    grouped_gemm_kernel<T, BM, BN, false, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
   else if (la == 0 && lb == 1)
-    grouped_gemm_kernel<T, BM, BN, false, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+    grouped_gemm_kernel<T, BM, BN, false, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
   else if (la == 1 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, true, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+    grouped_gemm_kernel<T, BM, BN, true, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
   else
-    grouped_gemm_kernel<T, BM, BN, true, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles);
+    grouped_gemm_kernel<T, BM, BN, true, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
 }
 
 }  // namespace
 
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n) {
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub) {
   if (ntiles <= 0) return;
-  if (tile_m == 128 && tile_n == 128) launch_tiles<T, 128, 128>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs);
-  else launch_tiles<T, 64, 64>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs);
+  if (tile_m == 128 && tile_n == 128) launch_tiles<T, 128, 128>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  else launch_tiles<T, 64, 64>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
   c->gemm_launches++;
 }
 
 template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *,
-                                          int, const BlkD *, const PairD *, int, int);
+                                          int, const BlkD *, const PairD *, int, int, int);
 template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
-                                         const BlkD *, const PairD *, int, int);
+                                         const BlkD *, const PairD *, int, int, int);
 
 }  // namespace mps2

@@ -163,7 +163,7 @@ struct PairD {
 };
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n);
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0);
 
 // ------------------------------------------------------------------ ops (ops.hip)
 // Device-side contraction plan (descriptors uploaded once, cached by layout).

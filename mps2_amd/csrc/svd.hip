@@ -37,6 +37,7 @@
 #include <chrono>
 #include <map>
 
+#include "eigh.h"
 #include "runtime.h"
 
 namespace mps2 {
@@ -57,6 +58,12 @@ int jacobi_inner_sweeps() {
     return x < 1 ? 1 : (x > 15 ? 15 : x);
   }();
   return v;
+}
+// Eigensolver of the Gram matrices: tridiagonalisation + divide and conquer
+// (default), or the block Jacobi method (MPS2_EIGH=jacobi).
+bool use_jacobi() {
+  const char *e = std::getenv("MPS2_EIGH");
+  return e && std::string(e) == "jacobi";
 }
 double jacobi_rel_tol() {
   static double v = [] {
@@ -343,22 +350,31 @@ __global__ void diag_kernel(const double *G, int64_t g_off, int np, int n, doubl
     out[i] = G[g_off + (int64_t)i * np + i];
 }
 
-// Scale rows (mode 0) or columns (mode 1) of a rows x cols matrix.
-__global__ void scale_rc_kernel(double *X, int rows, int cols, const double *sc, int mode) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)rows * cols;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int i = (int)(e / cols), j = (int)(e % cols);
-    X[e] *= mode == 0 ? sc[i] : sc[j];
+
+
+// Row (mode 0) or column (mode 1) scaling of one matrix per blockIdx.y.
+struct ScaleD {
+  int64_t off, sc_off;
+  int32_t rows, cols;
+  int32_t mode, pad;
+};
+__global__ void scale_batch_kernel(double *X, const double *sc, const ScaleD *desc) {
+  const ScaleD d = desc[blockIdx.y];
+  const int64_t n = (int64_t)d.rows * d.cols;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / d.cols, j = e - i * d.cols;
+    X[d.off + e] *= sc[d.sc_off + (d.mode == 0 ? i : j)];
   }
 }
 
-// Transposed block copy: dst[c * ldd + r] = src[r * lds + c] for r < rows, c < cols.
-__global__ void copy_t_kernel(const double *src, int64_t src_off, int lds, double *dst, int64_t dst_off, int ldd,
-                              int rows, int cols) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)rows * cols;
-       e += (int64_t)gridDim.x * blockDim.x) {
-    int r = (int)(e / cols), cc = (int)(e % cols);
-    dst[dst_off + (int64_t)cc * ldd + r] = src[src_off + (int64_t)r * lds + cc];
+// Transposed block copies, one descriptor per blockIdx.x:
+// dst[dst_off + c * ldd + r] = src[src_off + r * lds + c], r < rows, c < cols.
+__global__ void copy2d_t_kernel(const double *src, double *dst, const Copy2D *desc) {
+  const Copy2D d = desc[blockIdx.x];
+  const int64_t n = (int64_t)d.rows * d.cols;
+  for (int64_t e = blockIdx.y * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.y * blockDim.x) {
+    const int64_t r = e / d.cols, cc = e - r * d.cols;
+    dst[d.dst_off + cc * d.ldd + r] = d.scale * src[d.src_off + r * d.lds + cc];
   }
 }
 
@@ -672,7 +688,12 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       for (double x : dg[q]) traces[q] += x;
   }
   auto te0 = std::chrono::steady_clock::now();
-  EigOut eo = batched_eigh(c, Gd, goff, gn, traces);
+  const bool jac = use_jacobi();
+  EigOut eo;
+  EighResult er;
+  if (jac) eo = batched_eigh(c, Gd, goff, gn, traces);
+  else er = eigh_batched(c, Gd, goff, gn);
+  const auto &evals = jac ? eo.evals : er.evals;
   c->count("svd_eigh_ms", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te0).count());
   {
     int maxn = 0;
@@ -686,7 +707,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   for (int q = 0; q < ns; ++q) {
     std::vector<int32_t> idx(gn[q]);
     for (int i = 0; i < gn[q]; ++i) idx[i] = i;
-    const auto &ev = eo.evals[q];
+    const auto &ev = evals[q];
     std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return ev[x] > ev[y]; });
     int64_t rq = std::min(secs[q].m, secs[q].n);
     idx.resize(rq);
@@ -743,80 +764,110 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     }
     out.s = ten_upload(c, sm, sh.data());
   }
-  // 7. factors per sector: E (orthonormal side, n x kq) and F (other side)
+  // 7. factors, batched over the kept sectors: E (orthonormal side, n x kq
+  //    row-major) and F (the other side, projected and divided by s)
+  std::vector<std::vector<int32_t>> kcols(ns);
+  std::vector<int64_t> eoff(ns, 0), foff(ns, 0), ioff(ns, 0);
+  int64_t etot = 0, ftot = 0, itot = 0;
+  std::vector<double> inv;
+  for (int q = 0; q < ns; ++q) {
+    if (secpos[q] < 0) continue;
+    const int64_t kq = out.kept[q];
+    kcols[q].assign(order[q].begin(), order[q].begin() + kq);
+    eoff[q] = etot;
+    etot += gn[q] * kq;
+    foff[q] = ftot;
+    ftot += kq * (side == kOrthLeft ? secs[q].n : secs[q].m);
+    ioff[q] = itot;
+    itot += kq;
+    for (int64_t k = 0; k < kq; ++k) inv.push_back(sv[q][k] > 0 ? 1.0 / sv[q][k] : 0.0);
+  }
+  auto Eb = c->alloc(std::max<int64_t>(etot, 1) * sizeof(double));
+  auto Fb = c->alloc(std::max<int64_t>(ftot, 1) * sizeof(double));
+  double *Ed = static_cast<double *>(Eb->p), *Fd = static_cast<double *>(Fb->p);
+  std::vector<BufP> keep;
+  if (jac) {
+    for (int q = 0; q < ns; ++q) {
+      if (secpos[q] < 0) continue;
+      const int64_t kq = out.kept[q];
+      keep.push_back(upload(c, kcols[q].data(), kq * sizeof(int32_t)));
+      gather_cols_kernel<<<gridn(gn[q] * kq), 256, 0, c->stream>>>(
+          static_cast<double *>(eo.V->p), eo.secs[q].g_off, eo.secs[q].np, gn[q],
+          static_cast<int32_t *>(keep.back()->p), (int)kq, Ed + eoff[q]);
+    }
+  } else {
+    eigh_vectors_all(c, er, kcols, Ed, eoff);
+  }
+  {
+    std::vector<BlkD> fb;
+    std::vector<PairD> fp;
+    std::vector<ScaleD> sd;
+    for (int q = 0; q < ns; ++q) {
+      if (secpos[q] < 0) continue;
+      const int32_t kq = (int32_t)out.kept[q], idx = (int32_t)fb.size();
+      if (side == kOrthLeft) {
+        // F = E^T M (kq x n), rows scaled by 1/s
+        fb.push_back(BlkD{foff[q], kq, (int32_t)secs[q].n, idx, idx + 1});
+        fp.push_back(PairD{eoff[q], moff[q], (int32_t)secs[q].m, 0});
+        sd.push_back(ScaleD{foff[q], ioff[q], kq, (int32_t)secs[q].n, 0, 0});
+      } else {
+        // F = M E (m x kq), columns scaled by 1/s
+        fb.push_back(BlkD{foff[q], (int32_t)secs[q].m, kq, idx, idx + 1});
+        fp.push_back(PairD{moff[q], eoff[q], (int32_t)secs[q].n, 0});
+        sd.push_back(ScaleD{foff[q], ioff[q], (int32_t)secs[q].m, kq, 1, 0});
+      }
+    }
+    if (!fb.empty()) {
+      if (side == kOrthLeft) raw_gemm(c, kTrans, kNormal, Ed, Md, Fd, fb, fp);
+      else raw_gemm(c, kNormal, kNormal, Md, Ed, Fd, fb, fp);
+      keep.push_back(upload(c, inv.data(), inv.size() * sizeof(double)));
+      const double *dinv = static_cast<const double *>(keep.back()->p);
+      keep.push_back(upload(c, sd.data(), sd.size() * sizeof(ScaleD)));
+      scale_batch_kernel<<<dim3(256, (unsigned)sd.size()), 256, 0, c->stream>>>(
+          Fd, dinv, static_cast<const ScaleD *>(keep.back()->p));
+    }
+  }
+  // scatter into the U and Vt blocks
+  std::vector<Copy2D> cpu, cpv, cpvt;
   for (int q = 0; q < ns; ++q) {
     if (secpos[q] < 0) continue;
     auto &s = secs[q];
-    int64_t kq = out.kept[q];
-    int n_ = gn[q];
-    std::vector<int32_t> cols(order[q].begin(), order[q].begin() + kq);
-    auto dcols = upload(c, cols.data(), kq * sizeof(int32_t));
-    auto E = c->alloc(std::max<int64_t>(n_ * kq, 1) * sizeof(double));
-    gather_cols_kernel<<<gridn(n_ * kq), 256, 0, c->stream>>>(static_cast<double *>(eo.V->p), eo.secs[q].g_off,
-                                                             eo.secs[q].np, n_, static_cast<int32_t *>(dcols->p),
-                                                             (int)kq, static_cast<double *>(E->p));
-    std::vector<double> inv(kq);
-    for (int64_t k = 0; k < kq; ++k) inv[k] = sv[q][k] > 0 ? 1.0 / sv[q][k] : 0.0;
-    auto dinv = upload(c, inv.data(), kq * sizeof(double));
-    if (side == kOrthLeft) {
-      // U rows = E (m x kq); Vt = S^-1 E^T M (kq x n)
-      auto F = c->alloc(std::max<int64_t>(kq * s.n, 1) * sizeof(double));
-      std::vector<BlkD> fb{BlkD{0, (int32_t)kq, (int32_t)s.n, 0, 1}};
-      std::vector<PairD> fp{PairD{0, moff[q], (int32_t)s.m, 0}};
-      raw_gemm(c, kTrans, kNormal, static_cast<double *>(E->p), Md, static_cast<double *>(F->p), fb, fp);
-      scale_rc_kernel<<<gridn(kq * s.n), 256, 0, c->stream>>>(static_cast<double *>(F->p), (int)kq, (int)s.n,
-                                                             static_cast<double *>(dinv->p), 0);
-      std::vector<Copy2D> cp;
-      for (auto &kv : s.rows) {
-        Coord cc = kv.first;
-        cc.push_back(secpos[q]);
-        int32_t b = um->find(cc);
-        int64_t bm = um->bsize[b] / kq;
-        cp.push_back(Copy2D{kv.second * kq, um->offset[b], (int32_t)bm, (int32_t)kq, (int32_t)kq, (int32_t)kq, 1.0});
-      }
-      auto d1 = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
-      k_copy2d(c, static_cast<double *>(E->p), out.u->d(), static_cast<const Copy2D *>(d1->p), (int)cp.size());
-      cp.clear();
-      for (auto &kv : s.cols) {
-        Coord cc{secpos[q]};
-        cc.insert(cc.end(), kv.first.begin(), kv.first.end());
-        int32_t b = vm->find(cc);
-        int64_t bn = vm->bsize[b] / kq;
-        cp.push_back(Copy2D{kv.second, vm->offset[b], (int32_t)kq, (int32_t)bn, (int32_t)s.n, (int32_t)bn, 1.0});
-      }
-      auto d2 = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
-      k_copy2d(c, static_cast<double *>(F->p), out.vt->d(), static_cast<const Copy2D *>(d2->p), (int)cp.size());
-      c->sync();
-    } else {
-      // Vt = E^T (kq x n); U = M E S^-1 (m x kq)
-      auto F = c->alloc(std::max<int64_t>(s.m * kq, 1) * sizeof(double));
-      std::vector<BlkD> fb{BlkD{0, (int32_t)s.m, (int32_t)kq, 0, 1}};
-      std::vector<PairD> fp{PairD{moff[q], 0, (int32_t)s.n, 0}};
-      raw_gemm(c, kNormal, kNormal, Md, static_cast<double *>(E->p), static_cast<double *>(F->p), fb, fp);
-      scale_rc_kernel<<<gridn(s.m * kq), 256, 0, c->stream>>>(static_cast<double *>(F->p), (int)s.m, (int)kq,
-                                                             static_cast<double *>(dinv->p), 1);
-      std::vector<Copy2D> cp;
-      for (auto &kv : s.rows) {
-        Coord cc = kv.first;
-        cc.push_back(secpos[q]);
-        int32_t b = um->find(cc);
-        int64_t bm = um->bsize[b] / kq;
-        cp.push_back(Copy2D{kv.second * kq, um->offset[b], (int32_t)bm, (int32_t)kq, (int32_t)kq, (int32_t)kq, 1.0});
-      }
-      auto d1 = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
-      k_copy2d(c, static_cast<double *>(F->p), out.u->d(), static_cast<const Copy2D *>(d1->p), (int)cp.size());
-      for (auto &kv : s.cols) {
-        Coord cc{secpos[q]};
-        cc.insert(cc.end(), kv.first.begin(), kv.first.end());
-        int32_t b = vm->find(cc);
-        int64_t bn = vm->bsize[b] / kq;
-        // Vt block (kq x bn) = E[kv.second : kv.second + bn, :]^T
-        copy_t_kernel<<<gridn(bn * kq), 256, 0, c->stream>>>(static_cast<double *>(E->p), kv.second * kq, (int)kq,
-                                                            out.vt->d(), vm->offset[b], (int)bn, (int)bn, (int)kq);
-      }
-      c->sync();
+    const int64_t kq = out.kept[q];
+    for (auto &kv : s.rows) {
+      Coord cc = kv.first;
+      cc.push_back(secpos[q]);
+      int32_t b = um->find(cc);
+      int64_t bm = um->bsize[b] / kq;
+      const int64_t src = (side == kOrthLeft ? eoff[q] : foff[q]) + kv.second * kq;
+      cpu.push_back(Copy2D{src, um->offset[b], (int32_t)bm, (int32_t)kq, (int32_t)kq, (int32_t)kq, 1.0});
+    }
+    for (auto &kv : s.cols) {
+      Coord cc{secpos[q]};
+      cc.insert(cc.end(), kv.first.begin(), kv.first.end());
+      int32_t b = vm->find(cc);
+      int64_t bn = vm->bsize[b] / kq;
+      if (side == kOrthLeft)
+        cpv.push_back(Copy2D{foff[q] + kv.second, vm->offset[b], (int32_t)kq, (int32_t)bn, (int32_t)s.n, (int32_t)bn,
+                             1.0});
+      else  // Vt block (kq x bn) = E[rows kv.second .. +bn, :]^T
+        cpvt.push_back(Copy2D{eoff[q] + kv.second * kq, vm->offset[b], (int32_t)bn, (int32_t)kq, (int32_t)kq,
+                              (int32_t)bn, 1.0});
     }
   }
+  if (!cpu.empty()) {
+    keep.push_back(upload(c, cpu.data(), cpu.size() * sizeof(Copy2D)));
+    k_copy2d(c, side == kOrthLeft ? Ed : Fd, out.u->d(), static_cast<const Copy2D *>(keep.back()->p), (int)cpu.size());
+  }
+  if (!cpv.empty()) {
+    keep.push_back(upload(c, cpv.data(), cpv.size() * sizeof(Copy2D)));
+    k_copy2d(c, Fd, out.vt->d(), static_cast<const Copy2D *>(keep.back()->p), (int)cpv.size());
+  }
+  if (!cpvt.empty()) {
+    keep.push_back(upload(c, cpvt.data(), cpvt.size() * sizeof(Copy2D)));
+    copy2d_t_kernel<<<dim3((unsigned)cpvt.size(), 64), 256, 0, c->stream>>>(
+        Ed, out.vt->d(), static_cast<const Copy2D *>(keep.back()->p));
+  }
+  c->sync();
   return out;
 }
 
