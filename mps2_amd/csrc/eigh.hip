@@ -41,7 +41,8 @@ namespace {
 
 constexpr int kNB = 32;   // panel width
 constexpr int kT = 256;   // threads of the per-sector kernels
-constexpr int kSymvRows = 8;
+constexpr int kRowsPerWave = 2;
+constexpr int kSymvRows = 4 * kRowsPerWave;  // 4 wavefronts per workgroup
 constexpr double kEps = 2.220446049250313e-16;
 
 struct ESec {
@@ -59,7 +60,8 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Sum over the workgroup (kT threads), fixed order; red holds kT/64 doubles.
+// Sum over the workgroup (up to 1024 threads), fixed order; red holds
+// blockDim.x / 64 doubles.
 __device__ double block_sum(double v, double *red) {
   v = wave_sum(v);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -67,8 +69,7 @@ __device__ double block_sum(double v, double *red) {
   if (lane == 0) red[w] = v;
   __syncthreads();
   double s = 0;
-#pragma unroll
-  for (int i = 0; i < kT / 64; ++i) s += red[i];
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
   return s;
 }
 
@@ -80,24 +81,18 @@ __device__ double block_max(double v, double *red) {
   if (lane == 0) red[w] = v;
   __syncthreads();
   double s = 0;
-#pragma unroll
-  for (int i = 0; i < kT / 64; ++i) s = fmax(s, red[i]);
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s = fmax(s, red[i]);
   return s;
 }
 
-// Column step c (one workgroup per sector): finish w_{c-1} from
+// Column step c of one sector, run by one workgroup: finish w_{c-1} from
 // y = A' v_{c-1} (w = tau y - tau^2/2 (y.v) v), form row c of the matrix
 // updated by (v_{c-1}, w_{c-1}), take d[c], and build the reflector v_c of
 // row c (alpha = a[c+1], x = a[c+2..]) into VH column c, e[c], tau[c].
-__global__ __launch_bounds__(kT) void trd_row_kernel(const ESec *__restrict__ secs, int c,
-                                                     const double *__restrict__ A, double *VH, double *wbuf,
-                                                     const double *__restrict__ ybuf, double *abuf, double *dv,
-                                                     double *ev, double *tauv) {
-  __shared__ double red[kT / 64];
-  __shared__ double sh[2];
-  const ESec S = secs[blockIdx.x];
-  const int n = S.n, tid = threadIdx.x;
-  if (c >= n) return;
+__device__ void row_step(const ESec &S, int c, const double *__restrict__ A, double *VH, double *wbuf,
+                         const double *ybuf, double *abuf, double *dv, double *ev, double *tauv, double *red,
+                         double *sh) {
+  const int n = S.n, tid = threadIdx.x, nt = blockDim.x;
   double *a = abuf + S.nv;
   const double *arow = A + S.nn + (int64_t)c * n;
   if (c > 0) {
@@ -106,24 +101,24 @@ __global__ __launch_bounds__(kT) void trd_row_kernel(const ESec *__restrict__ se
     const double *y = ybuf + S.nv;
     double *w = wbuf + S.nv;
     double part = 0;
-    for (int r = c + tid; r < n; r += kT) part += y[r] * vp[r];
+    for (int r = c + tid; r < n; r += nt) part += y[r] * vp[r];
     const double kcoef = -0.5 * tau * tau * block_sum(part, red);
-    for (int r = c + tid; r < n; r += kT) {
+    for (int r = c + tid; r < n; r += nt) {
       const double x = tau * y[r] + kcoef * vp[r];
       w[r] = x;
       if (r == c) sh[0] = x;
     }
     __syncthreads();
     const double wc = sh[0], vc = vp[c];
-    for (int r = c + tid; r < n; r += kT) a[r] = arow[r] - vc * w[r] - wc * vp[r];
+    for (int r = c + tid; r < n; r += nt) a[r] = arow[r] - vc * w[r] - wc * vp[r];
   } else {
-    for (int r = tid; r < n; r += kT) a[r] = arow[r];
+    for (int r = tid; r < n; r += nt) a[r] = arow[r];
   }
   __syncthreads();
   if (tid == 0) dv[S.nv + c] = a[c];
   if (c >= n - 1) return;
   double ss = 0;
-  for (int r = c + 2 + tid; r < n; r += kT) ss += a[r] * a[r];
+  for (int r = c + 2 + tid; r < n; r += nt) ss += a[r] * a[r];
   const double xn = sqrt(block_sum(ss, red));
   const double alpha = a[c + 1];
   double beta, tau, sc;
@@ -137,17 +132,32 @@ __global__ __launch_bounds__(kT) void trd_row_kernel(const ESec *__restrict__ se
     sc = 1.0 / (alpha - beta);
   }
   double *vh = VH + S.nn + (int64_t)c * n;
-  for (int r = c + 1 + tid; r < n; r += kT) vh[r] = r == c + 1 ? 1.0 : a[r] * sc;
+  for (int r = c + 1 + tid; r < n; r += nt) vh[r] = r == c + 1 ? 1.0 : a[r] * sc;
   if (tid == 0) {
     ev[S.nv + c] = beta;
     tauv[S.nv + c] = tau;
   }
 }
 
+// Column step c of every sector (one workgroup of kTR threads per sector).
+constexpr int kTR = 1024;
+__global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ secs, int c,
+                                                      const double *__restrict__ A, double *VH, double *wbuf,
+                                                      const double *__restrict__ ybuf, double *abuf, double *dv,
+                                                      double *ev, double *tauv) {
+  __shared__ double red[kTR / 64];
+  __shared__ double sh[2];
+  const ESec S = secs[blockIdx.x];
+  if (c >= S.n) return;
+  row_step(S, c, A, VH, wbuf, ybuf, abuf, dv, ev, tauv, red, sh);
+}
+
 // Rows r > c of the trailing square, one pass: apply the rank-2 update of
 // column c-1, A[r][q] -= v[r] w[q] + w[r] v[q] (q > c), store, and form
-// y[r] = sum_{q > c} A[r][q] v_c[q].  kSymvRows rows per workgroup, two per
-// wavefront, lanes along the row.
+// y[r] = sum_{q > c} A[r][q] v_c[q]; kSymvRows rows per workgroup, kRowsPerWave
+// per wavefront, lanes along the row.  (A device-scope fence per workgroup,
+// to hand the column step to the last workgroup, would write back the whole
+// L2 this kernel dirties; the column step is a launch of its own instead.)
 __global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restrict__ secs, int nsec, int c, double *A,
                                                              const double *__restrict__ VH,
                                                              const double *__restrict__ wbuf, double *ybuf) {
@@ -167,39 +177,42 @@ __global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restr
   const double *v = VH + S.nn + (int64_t)c * n;
   const double *vp = VH + S.nn + (int64_t)(c > 0 ? c - 1 : 0) * n;
   const double *w = wbuf + S.nv;
-  const int r0 = c + 1 + b * kSymvRows + wv * 2, r1 = r0 + 1;
-  const bool ok0 = r0 < n, ok1 = r1 < n;
-  double *a0 = A + S.nn + (int64_t)(ok0 ? r0 : c + 1) * n;
-  double *a1 = A + S.nn + (int64_t)(ok1 ? r1 : c + 1) * n;
-  double s0 = 0, s1 = 0;
+  const int r0 = c + 1 + b * kSymvRows + wv * kRowsPerWave;
+  double *ar[kRowsPerWave];
+  double vr[kRowsPerWave], wr[kRowsPerWave], acc[kRowsPerWave];
+  bool ok[kRowsPerWave];
+#pragma unroll
+  for (int i = 0; i < kRowsPerWave; ++i) {
+    const int r = r0 + i;
+    ok[i] = r < n;
+    ar[i] = A + S.nn + (int64_t)(ok[i] ? r : c + 1) * n;
+    vr[i] = (ok[i] && c > 0) ? vp[r] : 0.0;
+    wr[i] = (ok[i] && c > 0) ? w[r] : 0.0;
+    acc[i] = 0.0;
+  }
   if (c > 0) {
-    const double v0 = ok0 ? vp[r0] : 0.0, w0 = ok0 ? w[r0] : 0.0;
-    const double v1 = ok1 ? vp[r1] : 0.0, w1 = ok1 ? w[r1] : 0.0;
     for (int col = c + 1 + lane; col < n; col += 64) {
       const double x = v[col], vq = vp[col], wq = w[col];
-      if (ok0) {
-        const double t = a0[col] - v0 * wq - w0 * vq;
-        a0[col] = t;
-        s0 += t * x;
-      }
-      if (ok1) {
-        const double t = a1[col] - v1 * wq - w1 * vq;
-        a1[col] = t;
-        s1 += t * x;
-      }
+#pragma unroll
+      for (int i = 0; i < kRowsPerWave; ++i)
+        if (ok[i]) {
+          const double t = ar[i][col] - vr[i] * wq - wr[i] * vq;
+          ar[i][col] = t;
+          acc[i] += t * x;
+        }
     }
   } else {
     for (int col = c + 1 + lane; col < n; col += 64) {
       const double x = v[col];
-      if (ok0) s0 += a0[col] * x;
-      if (ok1) s1 += a1[col] * x;
+#pragma unroll
+      for (int i = 0; i < kRowsPerWave; ++i)
+        if (ok[i]) acc[i] += ar[i][col] * x;
     }
   }
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-  if (lane == 0) {
-    if (ok0) ybuf[S.nv + r0] = s0;
-    if (ok1) ybuf[S.nv + r1] = s1;
+#pragma unroll
+  for (int i = 0; i < kRowsPerWave; ++i) {
+    const double sum = wave_sum(acc[i]);
+    if (lane == 0 && ok[i]) ybuf[S.nv + r0 + i] = sum;
   }
 }
 
@@ -442,9 +455,11 @@ __global__ __launch_bounds__(256) void dc_rotate_kernel(const ESec *__restrict__
 // bisection.
 __global__ __launch_bounds__(256) void dc_roots_kernel(const ESec *__restrict__ secs, const DcMerge *__restrict__ ms,
                                                        DcBufs B) {
+  // one wavefront per root, lanes split the pole sums
   const DcMerge M = ms[blockIdx.y];
   const ESec S = secs[M.q];
-  const int j = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
   const DcInfo inf = B.info[blockIdx.y];
   const int kk = inf.kk;
   if (j >= kk) return;
@@ -457,8 +472,9 @@ __global__ __launch_bounds__(256) void dc_roots_kernel(const ESec *__restrict__ 
   if (!last) {
     const double gap = dK[j + 1] - dK[j];
     const double mid = 0.5 * gap;
-    double f = 1.0;
-    for (int i = 0; i < kk; ++i) f += rho * zK[i] * zK[i] / ((dK[i] - dK[j]) - mid);
+    double f = 0.0;
+    for (int i = lane; i < kk; i += 64) f += rho * zK[i] * zK[i] / ((dK[i] - dK[j]) - mid);
+    f = 1.0 + wave_sum(f);
     if (f >= 0) {
       o = j;
       lo = 0.0;
@@ -470,7 +486,8 @@ __global__ __launch_bounds__(256) void dc_roots_kernel(const ESec *__restrict__ 
     }
   } else {
     double zz = 0;
-    for (int i = 0; i < kk; ++i) zz += zK[i] * zK[i];
+    for (int i = lane; i < kk; i += 64) zz += zK[i] * zK[i];
+    zz = wave_sum(zz);
     o = j;
     lo = 0.0;
     hi = rho * zz;
@@ -480,17 +497,19 @@ __global__ __launch_bounds__(256) void dc_roots_kernel(const ESec *__restrict__ 
   double tau = 0.5 * (lo + hi);
   for (int it = 0; it < 100; ++it) {
     double psi = 0, dpsi = 0, phi = 0, dphi = 0;
-    for (int i = 0; i < kk; ++i) {
+    for (int i = lane; i < kk; i += 64) {
       const double inv = 1.0 / ((dK[i] - dO) - tau);
       const double w = rho * zK[i] * zK[i] * inv;
-      if (i <= j) {
-        psi += w;
-        dpsi += w * inv;
-      } else {
-        phi += w;
-        dphi += w * inv;
-      }
+      const bool left = i <= j;
+      psi += left ? w : 0.0;
+      dpsi += left ? w * inv : 0.0;
+      phi += left ? 0.0 : w;
+      dphi += left ? 0.0 : w * inv;
     }
+    psi = wave_sum(psi);
+    dpsi = wave_sum(dpsi);
+    phi = wave_sum(phi);
+    dphi = wave_sum(dphi);
     const double f = 1.0 + psi + phi;
     if (f == 0.0) break;
     if (f > 0.0) hi = tau;
@@ -523,8 +542,10 @@ __global__ __launch_bounds__(256) void dc_roots_kernel(const ESec *__restrict__ 
     tau = tn;
     if (step <= 2.0 * kEps * fabs(tau) || hi - lo <= 2.0 * kEps * fmax(fabs(lo), fabs(hi))) break;
   }
-  B.org[vo + j] = o;
-  B.tau[vo + j] = tau;
+  if (lane == 0) {
+    B.org[vo + j] = o;
+    B.tau[vo + j] = tau;
+  }
 }
 
 // Gu-Eisenstat: zhat_i^2 = (lambda_i - d_i) / rho * prod_{j != i} (lambda_j - d_i) / (d_j - d_i).
@@ -691,7 +712,7 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
   k_fill(c, tauv, nv, 0.0);
   // ---- 1. tridiagonalisation, one column per step for every sector
   for (int col = 0; col < maxn; ++col) {
-    trd_row_kernel<<<nsec, kT, 0, c->stream>>>(SD, col, G, VH, dp<double>(bw), yb, dp<double>(ba), dv, ev, tauv);
+    trd_row_kernel<<<nsec, kTR, 0, c->stream>>>(SD, col, G, VH, dp<double>(bw), yb, dp<double>(ba), dv, ev, tauv);
     int64_t nwg = 0;
     for (int q = 0; q < nsec; ++q)
       if (col < ns[q] - 1) nwg += (ns[q] - col - 1 + kSymvRows - 1) / kSymvRows;
@@ -801,7 +822,7 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
     dc_setup_kernel<<<nm, kT, 0, c->stream>>>(SD, MS, B);
     const dim3 g256(nblk(kmax), nm), g4(nblk(kmax, 4), nm);
     dc_rotate_kernel<<<g256, 256, 0, c->stream>>>(SD, MS, B);
-    dc_roots_kernel<<<g256, 256, 0, c->stream>>>(SD, MS, B);
+    dc_roots_kernel<<<g4, 256, 0, c->stream>>>(SD, MS, B);
     dc_zhat_kernel<<<g256, 256, 0, c->stream>>>(SD, MS, B);
     dc_vec_kernel<<<g4, 256, 0, c->stream>>>(SD, MS, B);
     // C (kk x k row-major) = U (kk x kk) P (kk x k): the merged vectors, column-major k x kk

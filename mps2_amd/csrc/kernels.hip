@@ -37,8 +37,7 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   red_partial.reset();
   dev_scalars.reset();
-  for (auto &kv : free_)
-    for (void *p : kv.second) (void)hipFree(p);
+  for (auto &kv : free_) (void)hipFree(kv.second);
   free_.clear();
   if (host_scalars) (void)hipHostFree(host_scalars);
   if (stream) (void)hipStreamDestroy(stream);
@@ -55,15 +54,21 @@ size_t Ctx::round_bytes(size_t b) {
   return (b + g - 1) / g * g;
 }
 
+// Stream-ordered reuse: a released block may be handed out again at once,
+// since every later use is queued on the same stream.  Best fit among cached
+// blocks of at most 1.25x the request (+ one granule), so the varying tensor
+// sizes of a sweep reuse memory instead of growing the cache.
 BufP Ctx::alloc(size_t bytes) {
   size_t rb = round_bytes(bytes);
   void *p = nullptr;
-  auto it = free_.find(rb);
-  if (it != free_.end() && !it->second.empty()) {
-    p = it->second.back();
-    it->second.pop_back();
+  auto it = free_.lower_bound(rb);
+  if (it != free_.end() && it->first <= rb + rb / 4 + (2u << 20)) {
+    p = it->second;
+    rb = it->first;
+    free_.erase(it);
     cached_bytes -= rb;
   } else {
+    count("hip_malloc");
     hipError_t e = hipMalloc(&p, rb);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -85,15 +90,15 @@ BufP Ctx::alloc(size_t bytes) {
 }
 
 void Ctx::release(void *p, size_t bytes) {
-  free_[bytes].push_back(p);
+  free_.emplace(bytes, p);
   cached_bytes += bytes;
   live_bytes -= bytes;
 }
 
 void Ctx::trim() {
+  count("alloc_trim");
   (void)hipStreamSynchronize(stream);
-  for (auto &kv : free_)
-    for (void *p : kv.second) (void)hipFree(p);
+  for (auto &kv : free_) (void)hipFree(kv.second);
   free_.clear();
   cached_bytes = 0;
 }
@@ -218,14 +223,18 @@ __global__ __launch_bounds__(kRedThreads) void reduce_final_kernel(const double 
   if (threadIdx.x == 0) *out = r;
 }
 
+// Max |x| per block; blockIdx.y splits a block into chunks, combined by a
+// 64-bit atomic max on the bit pattern (monotone for non-negative doubles,
+// order-independent).  out must start at zero.
 __global__ __launch_bounds__(kRedThreads) void block_maxabs_kernel(const double *x, const int64_t *off,
                                                                    const int64_t *size, double *out) {
   __shared__ double sh[8];
-  int b = blockIdx.x;
+  const int b = blockIdx.x;
   const double *p = x + off[b];
-  int64_t n = size[b];
+  const int64_t n = size[b];
   double m = 0;
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = fmax(m, fabs(p[i]));
+  for (int64_t i = blockIdx.y * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.y * blockDim.x)
+    m = fmax(m, fabs(p[i]));
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) sh[w] = m;
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(kRedThreads) void block_maxabs_kernel(const double 
   if (threadIdx.x == 0) {
     double r = 0;
     for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
-    out[b] = r;
+    if (r > 0) atomicMax(reinterpret_cast<unsigned long long *>(out) + b, (unsigned long long)__double_as_longlong(r));
   }
 }
 
@@ -309,7 +318,9 @@ void k_dot(Ctx *c, int64_t n, const double *x, const double *y, int slot) {
                                                         static_cast<double *>(c->dev_scalars->p) + slot);
 }
 void k_block_maxabs(Ctx *c, const double *x, const int64_t *d_off, const int64_t *d_size, int nblk, double *d_out) {
-  if (nblk > 0) block_maxabs_kernel<<<nblk, 256, 0, c->stream>>>(x, d_off, d_size, d_out);
+  if (nblk <= 0) return;
+  fill_kernel<<<grid_for(nblk, 256), 256, 0, c->stream>>>(d_out, nblk, 0.0);
+  block_maxabs_kernel<<<dim3(nblk, 64), 256, 0, c->stream>>>(x, d_off, d_size, d_out);
 }
 void k_lincomb(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *d_coef, double *out) {
   if (n > 0) lincomb_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, nv, d_vecs, d_coef, out);

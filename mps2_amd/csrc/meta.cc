@@ -186,13 +186,34 @@ std::shared_ptr<ContractPlan> plan_contract(const MetaP &a, const MetaP &b, cons
     if (!ac[i]) afree.push_back(i);
   for (int i = 0; i < rb; ++i)
     if (!bc[i]) bfree.push_back(i);
-  // Operand layouts.
+  // Operand layouts.  A leg whose sectors all have dimension 1 (a spin-1/2 or
+  // fermion site, for instance) leaves the memory order of every block
+  // unchanged wherever it sits, so it is left out of the test: a contraction
+  // whose other contracted legs lead or trail the block reads it in place.
+  auto unit = [](const Leg &l) {
+    for (auto d : l.dim)
+      if (d != 1) return false;
+    return true;
+  };
+  std::vector<int> apos(ra, -1), bpos(rb, -1);
+  int na = 0, nb = 0;
+  for (int i = 0; i < ra; ++i)
+    if (!unit(a->legs[i])) apos[i] = na++;
+  for (int i = 0; i < rb; ++i)
+    if (!unit(b->legs[i])) bpos[i] = nb++;
+  std::vector<int> axn, bxn;
+  for (int i = 0; i < k; ++i)
+    if (apos[ax[i]] >= 0) {
+      axn.push_back(apos[ax[i]]);
+      bxn.push_back(bpos[bx[i]]);
+    }
+  const int kn = (int)axn.size();
   bool a_trail = true, a_lead = true, b_lead = true, b_trail = true;
-  for (int i = 0; i < k; ++i) {
-    a_trail &= ax[i] == ra - k + i;
-    a_lead &= ax[i] == i;
-    b_lead &= bx[i] == i;
-    b_trail &= bx[i] == rb - k + i;
+  for (int i = 0; i < kn; ++i) {
+    a_trail &= axn[i] == na - kn + i;
+    a_lead &= axn[i] == i;
+    b_lead &= bxn[i] == i;
+    b_trail &= bxn[i] == nb - kn + i;
   }
   if (a_trail) P->la = kNormal;
   else if (a_lead) P->la = kTrans;

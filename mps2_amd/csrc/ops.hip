@@ -350,7 +350,10 @@ void heff_mul(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, cons
     t1.reset();
     auto t3 = contract(t2, W2, {4, 1}, {0, 1});
     t2.reset();
-    contract_into(t3, R, {4, 1}, {1, 0}, out);
+    // (t3 leg 1 (r) with R leg 0 (ket), t3 leg 4 (mpo) with R leg 1): the
+    // reference's pairing {4, 1} x {1, 0}, listed in R's storage order so R,
+    // which is the same for every Lanczos step, is read in place.
+    contract_into(t3, R, {1, 4}, {0, 1}, out);
   } else if (where == "lend") {
     auto t1 = contract(psi, W1, {0}, {0});
     auto t2 = contract(t1, W2, {0, 2}, {1, 0});

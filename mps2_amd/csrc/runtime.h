@@ -64,7 +64,7 @@ struct Ctx {
   hipStream_t stream = nullptr;
   int num_cu = 0;
   // caching allocator
-  std::map<size_t, std::vector<void *>> free_;
+  std::multimap<size_t, void *> free_;  // cached blocks by size (best fit)
   size_t cached_bytes = 0, live_bytes = 0, peak_bytes = 0;
   // pinned host staging for scalars
   double *host_scalars = nullptr;   // 256 doubles

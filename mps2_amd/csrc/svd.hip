@@ -13,12 +13,14 @@
 // so both factors of the product used by the sweep are exact even for
 // (near-)zero singular values.
 //
-// Eigen-decomposition: batched block Jacobi.  Each sector's Gram is padded to
-// nb*b (b = 32, nb even); per round of a round-robin tournament over the nb
-// column blocks, every block pair (I, J) is diagonalised in LDS by cyclic
-// Jacobi (64 x 64), then the rotations are applied to rows, then to columns
-// and to the accumulated eigenvectors.  Sweeps repeat until a sweep performs
-// no rotation above 1e-15 * trace(G_q).
+// Eigen-decomposition (eigh.hip): Householder tridiagonalisation of every
+// sector's Gram matrix, tridiagonal divide and conquer, and back-
+// transformation of the kept eigenvectors only, all sectors batched per
+// launch.  The batched block Jacobi method below (each Gram padded to nb*b,
+// b = 32; per round of a round-robin tournament every block pair (I, J)
+// diagonalised in LDS, then applied to G and the accumulated eigenvectors
+// on MFMA; sweeps until no rotation exceeds 1e-15 * trace(G_q)) remains
+// selectable with MPS2_EIGH=jacobi.
 //
 // Truncation (host, on the singular values only):
 //  1. all singular values, sorted descending, ties by (sector, index);
@@ -34,6 +36,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <chrono>
 #include <map>
 
@@ -672,7 +675,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   else raw_gemm(c, kTrans, kNormal, Md, Md, Gd, gb, gp);
   // traces for the rotation threshold
   std::vector<double> traces(ns, 0.0);
-  {
+  if (use_jacobi()) {
     auto dd = c->alloc(std::max<int64_t>(gtot, 1) * sizeof(double));
     std::vector<std::vector<double>> dg(ns);
     for (int q = 0; q < ns; ++q) {
@@ -693,7 +696,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   EighResult er;
   if (jac) eo = batched_eigh(c, Gd, goff, gn, traces);
   else er = eigh_batched(c, Gd, goff, gn);
-  const auto &evals = jac ? eo.evals : er.evals;
+  std::vector<std::vector<double>> evals = jac ? eo.evals : er.evals;
   c->count("svd_eigh_ms", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te0).count());
   {
     int maxn = 0;
@@ -704,19 +707,191 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   // 5. singular values: top min(m, n) eigenvalues per sector, descending
   std::vector<std::vector<double>> sv(ns);
   std::vector<std::vector<int32_t>> order(ns);
-  for (int q = 0; q < ns; ++q) {
-    std::vector<int32_t> idx(gn[q]);
-    for (int i = 0; i < gn[q]; ++i) idx[i] = i;
-    const auto &ev = evals[q];
-    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return ev[x] > ev[y]; });
-    int64_t rq = std::min(secs[q].m, secs[q].n);
-    idx.resize(rq);
-    order[q] = idx;
-    sv[q].resize(rq);
-    for (int64_t j = 0; j < rq; ++j) sv[q][j] = std::sqrt(std::max(ev[idx[j]], 0.0));
-  }
+  auto singular_values = [&]() {
+    for (int q = 0; q < ns; ++q) {
+      std::vector<int32_t> idx(gn[q]);
+      for (int i = 0; i < gn[q]; ++i) idx[i] = i;
+      const auto &ev = evals[q];
+      std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return ev[x] > ev[y]; });
+      int64_t rq = std::min(secs[q].m, secs[q].n);
+      idx.resize(rq);
+      order[q] = idx;
+      sv[q].resize(rq);
+      for (int64_t j = 0; j < rq; ++j) sv[q][j] = std::sqrt(std::max(ev[idx[j]], 0.0));
+    }
+  };
+  singular_values();
   SvdOut out;
   truncate_rule(sv, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
+  // Tail refinement (tridiagonal path).  An eigenvalue of G is accurate to
+  // about n eps lambda_max absolutely, i.e. a singular value s only to a
+  // relative (s_max / s)^2 eps, while the truncation (and its 1e-6 tie
+  // window) needs the values near the cut to a relative error like a direct
+  // SVD's, eps s_max / s.  Eigenvalues above kRefine * lambda_max = 1e-7
+  // lambda_max are resolved to ~1e-9 relative.  When the cut lies below that
+  // in a sector, the eigenpairs of that sector below kRefine * lambda_max (the
+  // tail S) are recomputed from the Gram of the projected tail,
+  // W_S = E_S^T M (left) or M E_S (right): its rows carry the true small
+  // singular values up to eps s_max absolute, and its eigensolver's rounding
+  // now scales with the tail's own maximum.  Repeated while the cut still
+  // lies below kRefine times the refined tail's maximum.
+  std::vector<int> nsub(ns, 0);
+  std::vector<int64_t> esoff(ns, 0);
+  BufP Esub;
+  if (!jac) {
+    const auto tr0 = std::chrono::steady_clock::now();
+    constexpr double kRefine = 1e-7;
+    std::vector<double> lam_top(ns, 0.0);  // scale of each sector's current problem
+    for (int q = 0; q < ns; ++q)
+      for (double x : evals[q]) lam_top[q] = std::max(lam_top[q], x);
+    for (int round = 0; round < 3; ++round) {
+      std::vector<double> all;
+      for (auto &v : sv) all.insert(all.end(), v.begin(), v.end());
+      if (out.D <= 0 || out.D >= (int64_t)all.size()) break;
+      std::nth_element(all.begin(), all.begin() + (out.D - 1), all.end(), std::greater<double>());
+      const double cut2 = all[out.D - 1] * all[out.D - 1];
+      std::vector<int> rs, rn;
+      for (int q = 0; q < ns; ++q) {
+        const double lim = kRefine * lam_top[q];
+        if (!(cut2 < lim)) continue;
+        if (round > 0) {
+          // the tail basis prepared by the previous round
+          if (nsub[q] > 0) {
+            rs.push_back(q);
+            rn.push_back(nsub[q]);
+          }
+          continue;
+        }
+        int r = 0;
+        while (r < gn[q] && evals[q][r] < lim) ++r;  // ascending
+        if (r < 1 || evals[q][r - 1] < 1e-4 * cut2) continue;  // no value near the cut
+        rs.push_back(q);
+        rn.push_back(r);
+      }
+      if (rs.empty()) break;
+      c->count("svd_refine_sectors", (double)rs.size());
+      c->count("svd_refine_rounds");
+      const int nr = (int)rs.size();
+      if (round == 0) {
+        // explicit tail bases from the tridiagonal solution
+        std::vector<std::vector<int32_t>> cols(ns);
+        int64_t tot = 0;
+        std::fill(nsub.begin(), nsub.end(), 0);
+        for (int i = 0; i < nr; ++i) {
+          const int q = rs[i];
+          cols[q].resize(rn[i]);
+          for (int j = 0; j < rn[i]; ++j) cols[q][j] = j;
+          esoff[q] = tot;
+          nsub[q] = rn[i];
+          tot += (int64_t)gn[q] * rn[i];
+        }
+        Esub = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
+        eigh_vectors_all(c, er, cols, static_cast<double *>(Esub->p), esoff);
+      }
+      const double *Es = static_cast<const double *>(Esub->p);
+      // W_S and the Gram of the projected tail
+      std::vector<int64_t> woff(nr), g2off(nr);
+      int64_t wtot = 0, gtot2 = 0;
+      for (int i = 0; i < nr; ++i) {
+        const int q = rs[i], r = rn[i];
+        woff[i] = wtot;
+        wtot += (int64_t)r * (side == kOrthLeft ? secs[q].n : secs[q].m);
+        g2off[i] = gtot2;
+        gtot2 += (int64_t)r * r;
+      }
+      auto bW = c->alloc(std::max<int64_t>(wtot, 1) * sizeof(double));
+      auto bG = c->alloc(std::max<int64_t>(gtot2, 1) * sizeof(double));
+      double *W_ = static_cast<double *>(bW->p), *G_ = static_cast<double *>(bG->p);
+      std::vector<BlkD> wb, gb2;
+      std::vector<PairD> wp, gp2;
+      for (int i = 0; i < nr; ++i) {
+        const int q = rs[i], r = rn[i];
+        const int32_t sm = (int32_t)secs[q].m, sn = (int32_t)secs[q].n;
+        if (side == kOrthLeft) {
+          wb.push_back(BlkD{woff[i], r, sn, i, i + 1});  // W = ES^T M  (r x n_cols)
+          wp.push_back(PairD{esoff[q], moff[q], sm, 0});
+          gb2.push_back(BlkD{g2off[i], r, r, i, i + 1});  // G2 = W W^T
+          gp2.push_back(PairD{woff[i], woff[i], sn, 0});
+        } else {
+          wb.push_back(BlkD{woff[i], sm, r, i, i + 1});  // W = M ES  (m_rows x r)
+          wp.push_back(PairD{moff[q], esoff[q], sn, 0});
+          gb2.push_back(BlkD{g2off[i], r, r, i, i + 1});  // G2 = W^T W
+          gp2.push_back(PairD{woff[i], woff[i], sm, 0});
+        }
+      }
+      if (side == kOrthLeft) {
+        raw_gemm(c, kTrans, kNormal, Es, Md, W_, wb, wp);
+        raw_gemm(c, kNormal, kTrans, W_, W_, G_, gb2, gp2);
+      } else {
+        raw_gemm(c, kNormal, kNormal, Md, Es, W_, wb, wp);
+        raw_gemm(c, kTrans, kNormal, W_, W_, G_, gb2, gp2);
+      }
+      EighResult er2 = eigh_batched(c, G_, g2off, rn);
+      for (int i = 0; i < nr; ++i) {
+        const int q = rs[i], r = rn[i];
+        for (int j = 0; j < r; ++j) evals[q][j] = er2.evals[i][j];
+        lam_top[q] = 0.0;
+        for (int j = 0; j < r; ++j) lam_top[q] = std::max(lam_top[q], er2.evals[i][j]);
+      }
+      singular_values();
+      truncate_rule(sv, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
+      // The kept vectors stay the tridiagonal solution's (the products S Vt =
+      // U^T M, or U S = M V, are exact projections for any basis of the kept
+      // span); refined vectors are formed only for a tail that a further
+      // round must resolve.
+      if (round == 2) break;
+      all.clear();
+      for (auto &v : sv) all.insert(all.end(), v.begin(), v.end());
+      if (out.D <= 0 || out.D >= (int64_t)all.size()) break;
+      std::nth_element(all.begin(), all.begin() + (out.D - 1), all.end(), std::greater<double>());
+      const double cut2n = all[out.D - 1] * all[out.D - 1];
+      std::vector<std::vector<int32_t>> cols2(nr);
+      std::vector<int> r2(nr, 0);
+      bool more = false;
+      for (int i = 0; i < nr; ++i) {
+        const int q = rs[i];
+        const double lim = kRefine * lam_top[q];
+        int r = 0;
+        if (cut2n < lim)
+          while (r < rn[i] && er2.evals[i][r] < lim) ++r;
+        if (r >= 1 && er2.evals[i][r - 1] >= 1e-4 * cut2n) {
+          r2[i] = r;
+          cols2[i].resize(r);
+          for (int j = 0; j < r; ++j) cols2[i][j] = j;
+          more = true;
+        }
+      }
+      if (!more) break;
+      // next tail bases: ES2 = ES B2[:, 0:r2]
+      std::vector<int64_t> b2off(nr), e2off(ns, 0);
+      int64_t btot = 0, etot2 = 0;
+      for (int i = 0; i < nr; ++i) {
+        b2off[i] = btot;
+        btot += (int64_t)rn[i] * r2[i];
+        e2off[rs[i]] = etot2;
+        etot2 += (int64_t)gn[rs[i]] * r2[i];
+      }
+      auto bB = c->alloc(std::max<int64_t>(btot, 1) * sizeof(double));
+      eigh_vectors_all(c, er2, cols2, static_cast<double *>(bB->p), b2off);
+      auto En = c->alloc(std::max<int64_t>(etot2, 1) * sizeof(double));
+      std::vector<BlkD> eb;
+      std::vector<PairD> ep;
+      std::vector<int> nsub2(ns, 0);
+      for (int i = 0; i < nr; ++i) {
+        if (r2[i] == 0) continue;
+        const int q = rs[i], idx = (int)eb.size();
+        eb.push_back(BlkD{e2off[q], gn[q], r2[i], idx, idx + 1});
+        ep.push_back(PairD{esoff[q], b2off[i], rn[i], 0});
+        nsub2[q] = r2[i];
+      }
+      raw_gemm(c, kNormal, kNormal, Es, static_cast<double *>(bB->p), static_cast<double *>(En->p), eb, ep);
+      Esub = En;
+      esoff = e2off;
+      nsub = nsub2;
+    }
+    c->count("svd_refine_ms",
+             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
+  }
   // 6. new bond index
   Leg knew;
   knew.dir = kOut;

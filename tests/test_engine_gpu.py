@@ -115,6 +115,46 @@ def test_tridiag_parity(ctx, n):
     assert abs(abs(float(v_g @ v_o)) - 1) < 1e-10
 
 
+def graded_block_matrix(rng, dims, qns, decades, multiplet=True):
+    """Block-diagonal 2-leg tensor whose sector matrices are U diag(s) V^T with
+    s_j = 10^(-decades j / n): singular values spread over `decades` orders
+    of magnitude, the sectors +q and -q carrying identical spectra (exact
+    multiplets across sectors, as SU(2) symmetry produces)."""
+    l = H.bond(IN, dims, qns)
+    r = H.bond(OUT, dims, qns)
+    t = BTensor([l, r])
+    spectra = {}
+    for i, (q, n) in enumerate(zip(qns, dims)):
+        key = abs(q) if multiplet else q
+        if key not in spectra:
+            spectra[key] = 10.0 ** (-decades * np.arange(n) / n) * (1 + 0.3 * rng.random())
+        s = spectra[key][:n]
+        u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        t.blocks[(i, i)] = (u * s) @ v.T
+    return t
+
+
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+def test_svd_deep_cut_parity(ctx, orth):
+    """Cut deep in a graded spectrum (s ~ 1e-8 s_max) with exact multiplets:
+    the kept counts per sector must match the oracle's direct SVD, which
+    needs singular values accurate far below the Gram matrix's resolution."""
+    rng = np.random.default_rng(7 + orth)
+    psi = graded_block_matrix(rng, (300, 420, 300), (-2, 0, 2), 12.0)
+    dpsi = ctx.put(psi)
+    for dmax in (520, 700):
+        Uo, So, Vo, eo, Do = O.svd(psi, 1, (0,), 0.0, 1, dmax)
+        u, s, vt, eg, Dg = E.svd(dpsi, 1, (0,), 0.0, 1, dmax, orth)
+        Sg = s.get()
+        assert Dg == Do
+        assert u.get().legs[-1].key() == Uo.legs[-1].key()
+        so, sg = np.diag(So.dense()), np.diag(Sg.dense())
+        assert so.min() < 1e-6 * so.max()  # the cut lies deep in the spectrum
+        assert np.abs(so - sg).max() < 1e-13 * so.max()
+        np.testing.assert_allclose(sg, so, rtol=1e-6)
+
+
 @pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
 @pytest.mark.parametrize("dims", [(3, 2, 4), (40, 70, 25), (150, 260, 120)])
 def test_svd_parity(ctx, orth, dims):
