@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -56,6 +58,18 @@ def cpu_baseline(tensors, flops, budget_s):
                        f"measurement), {t:.2f} s each, OpenMP over {cores} host threads")
 
 
+def measured_traffic(args):
+    """HBM bytes per launch of the roofline kernel, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled
+    by the same counter's reading for the calibration copy); None when no
+    summary exists for this configuration."""
+    path = os.path.join(ROOT, "profiles", "matvec_stage4_f32_j1j2_4x24_D4096_traffic.json")
+    if args.dmax != 4096 or args.lx != 24 or args.width != 4 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +86,9 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--profile", action="store_true",
+                    help="profiler runs: skip the timed sweeps and the CPU baseline (state, matvec bench and the "
+                         "calibration copy as in the default run)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -110,7 +127,7 @@ def main():
     model.checkpoint()
 
     res = {}
-    for name, prec in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED)):
+    for name, prec in (() if args.profile else (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED))):
         model.restore()
         engine.check(engine.lib().mps2_ctx_sync(G.lib().gqmps2_engine_ctx()))
         G.engine_reset_stats()
@@ -128,11 +145,12 @@ def main():
             f"mean Lanczos iters {r['iters'].mean():.1f}, gemm flops {st['gemm_flops']:.3e}, "
             f"peak HBM {st['peak_bytes'] / 2**30:.1f} GiB")
 
-    e64, emx = res["fp64"]["r"]["energy"], res["mixed"]["r"]["energy"]
-    de_rel = abs(emx - e64) / abs(e64)
-    upd_rel = float(max(abs(a - b) / abs(b) for a, b in zip(res["mixed"]["r"]["energies"],
-                                                           res["fp64"]["r"]["energies"])))
-    dims_equal = bool((res["mixed"]["r"]["dims"] == res["fp64"]["r"]["dims"]).all())
+    if res:
+        e64, emx = res["fp64"]["r"]["energy"], res["mixed"]["r"]["energy"]
+        de_rel = abs(emx - e64) / abs(e64)
+        upd_rel = float(max(abs(a - b) / abs(b) for a, b in zip(res["mixed"]["r"]["energies"],
+                                                               res["fp64"]["r"]["energies"])))
+        dims_equal = bool((res["mixed"]["r"]["dims"] == res["fp64"]["r"]["dims"]).all())
 
     centre = N // 2 - 1
     t0 = time.perf_counter()
@@ -142,17 +160,71 @@ def main():
         f"({mb['mixed']['flops'] / mb['mixed']['ms'] / 1e9:.2f} TFLOP/s), stage ms fp64 {mb['fp64']['stage_ms']} "
         f"mixed {mb['mixed']['stage_ms']}")
 
+    # attainable copy bandwidth; the same kernel scales the profiler's byte counters
+    copy_bytes = 1 << 30
+    copy_ms = engine.copy_bench(G.lib().gqmps2_engine_ctx(), copy_bytes, reps=5)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+    log(f"calibration copy: {copy_bytes >> 20} MiB in {copy_ms:.3f} ms = {copy_gbs:.0f} GB/s (read + write)")
+
     def tflops(b):
         return b["flops"] / (b["ms"] * 1e-3) / 1e12
 
     dom = mb["mixed"]
     dom_ach = dom["dom_flops"] / (dom["dom_ms"] * 1e-3) / 1e12 if dom["dom_ms"] > 0 else 0.0
     roofline = dict(bound="mfma", achieved=round(dom_ach, 3), peak=PEAK["fp32"], unit="TFLOP/s",
-                    frac=round(dom_ach / PEAK["fp32"], 4), traffic=None,
-                    kernel=f"grouped_gemm_kernel<float> stage {dom['dom_stage'] + 1} of the centre-bond matvec",
+                    frac=round(dom_ach / PEAK["fp32"], 4), traffic=measured_traffic(args),
+                    kernel=f"grouped_gemm_kernel<float, ..., 1> (stage {dom['dom_stage'] + 1} of the centre-bond "
+                           f"matvec, run as the tagged instance)",
                     flops_per_launch=dom["dom_flops"], ms_per_launch=dom["dom_ms"])
     d64 = mb["fp64"]
     d64_ach = d64["dom_flops"] / (d64["dom_ms"] * 1e-3) / 1e12 if d64["dom_ms"] > 0 else 0.0
+
+    if args.profile:
+        print(json.dumps({"metric": METRIC, "profile": True, "matvec_ms": {k: mb[k]["ms"] for k in mb},
+                          "roofline": roofline, "copy_gbs": round(copy_gbs, 1)}), flush=True)
+        return
+
+    L_, W1_, W2_, R_, psi_ = model.bench_tensors()
+    ctxh = G.lib().gqmps2_engine_ctx()
+    # one GPU's share: the dominant stage split over 8 owners of its output blocks
+    st = dom["dom_stage"]
+    rows = [engine.heff_bench_shard(L_, W1_, W2_, R_, psi_, "cent", engine.PREC_MIXED, st, 8, r, reps=3)
+            for r in range(8)]
+    sh_ms = [x[0] for x in rows]
+    shard = dict(owners=8, stage=st + 1, precision="mixed", assignment="output blocks, longest-processing-time on flops",
+                 ms_per_owner=[round(m, 4) for m in sh_ms], flops_per_owner=[x[1] for x in rows],
+                 stage_flops=rows[0][2], full_stage_ms=round(dom["dom_ms"], 4), max_owner_ms=round(max(sh_ms), 4),
+                 balance=round(sum(x[1] for x in rows) / 8 / max(x[1] for x in rows), 4),
+                 owner_tflops=[round(x[1] / (x[0] * 1e-3) / 1e12, 2) for x in rows],
+                 speedup_vs_one_gpu=round(dom["dom_ms"] / max(sh_ms), 3))
+    log(f"8-owner share of stage {st + 1}: max {max(sh_ms):.3f} ms per owner vs {dom['dom_ms']:.3f} ms whole "
+        f"(speedup {shard['speedup_vs_one_gpu']}, flop balance {shard['balance']})")
+    # FP64 vs mixed Lanczos on the centre bond from one random start (seed 42)
+    rng = np.random.default_rng(42)
+    bt = psi_.get()
+    for k in bt.blocks:
+        bt.blocks[k] = rng.standard_normal(bt.blocks[k].shape)
+    lz = {}
+    for name, prec in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED)):
+        init = engine.put_on(ctxh, bt)
+        engine.check(engine.lib().mps2_ctx_sync(ctxh))
+        t0 = time.perf_counter()
+        it, e, gs = engine.lanczos(L_, W1_, W2_, R_, init, "cent", le, lm, prec)
+        engine.check(engine.lib().mps2_ctx_sync(ctxh))
+        lz[name] = dict(iters=it, energy=e, s=time.perf_counter() - t0, gs=gs)
+    ov = abs(engine.dot(lz["fp64"]["gs"], lz["mixed"]["gs"])) / np.sqrt(
+        engine.dot(lz["fp64"]["gs"], lz["fp64"]["gs"]) * engine.dot(lz["mixed"]["gs"], lz["mixed"]["gs"]))
+    lanczos_cmp = dict(start="random normal (seed 42) in the centre-bond state's blocks",
+                       iters={k: lz[k]["iters"] for k in lz}, energy={k: lz[k]["energy"] for k in lz},
+                       seconds={k: round(lz[k]["s"], 3) for k in lz},
+                       speedup_mixed_vs_fp64=round(lz["fp64"]["s"] / lz["mixed"]["s"], 3),
+                       dE_rel=abs(lz["mixed"]["energy"] - lz["fp64"]["energy"]) / abs(lz["fp64"]["energy"]),
+                       overlap=float(ov))
+    log(f"centre-bond Lanczos from a random start: fp64 {lz['fp64']['iters']} iters {lz['fp64']['s']:.2f}s, "
+        f"mixed {lz['mixed']['iters']} iters {lz['mixed']['s']:.2f}s, dE_rel {lanczos_cmp['dE_rel']:.2e}, "
+        f"overlap {ov:.15f}")
+    for v in lz.values():
+        v["gs"].free()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -194,10 +266,15 @@ def main():
                               frac=round(d64_ach / PEAK["fp64"], 4)),
         "energy": {"fp64": e64, "mixed": emx, "dE_rel_final": de_rel, "dE_rel_max_update": upd_rel,
                    "kept_dims_identical": dims_equal},
+        "truncation_error": {k: {"max": float(res[k]["r"]["trunc_errs"].max()),
+                                 "mean": float(res[k]["r"]["trunc_errs"].mean())} for k in res},
+        "centre_bond_lanczos": lanczos_cmp,
+        "one_gpu_share": shard,
         "lanczos_iters_mean": {"fp64": float(res["fp64"]["r"]["iters"].mean()),
                                "mixed": float(res["mixed"]["r"]["iters"].mean())},
         "peak_hbm_gib": round(max(res[k]["stats"]["peak_bytes"] for k in res) / 2**30, 2),
         "roofline": roofline,
+        "hbm_copy_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

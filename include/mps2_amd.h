@@ -140,11 +140,30 @@ int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init
  * launch with its own events.  out[0] ms per matvec, out[1] algorithmic
  * flops per matvec (sum over QN-matched block pairs of 2*M*N*K), out[2] ms
  * per matvec inside grouped GEMMs, out[3] index of the stage with the longest
- * GEMM time, out[4] that stage's flops per launch, out[5] its ms per launch,
+ * GEMM time, out[4] that stage's flops per launch, out[5] its ms per launch
+ * measured in a second pass of `reps` matvecs in which that stage runs as a
+ * separately named kernel instance (grouped_gemm_kernel<..., 1>), so a
+ * profiler's per-kernel statistics of that instance cover these launches,
  * out[6] FP64 bytes of L, W1, W2, R and two states (the fused-ideal traffic),
  * out[7..10] ms per launch of stages 1..4. */
 int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
                     int precision, int warmup, int reps, double *out);
+
+/* One GPU's share of a sharded matvec stage (no reference counterpart:
+ * measurement of the owner-computes design).  The output blocks of matvec
+ * stage `stage` (0-based) are assigned to `nshard` owners by
+ * longest-processing-time on their flops; `reps` matvecs run with that stage
+ * computing only rank `shard`'s blocks (the output is then incomplete).
+ * out[0] ms per launch of the share, out[1] the share's algorithmic flops,
+ * out[2] the whole stage's flops. */
+int mps2_heff_bench_shard(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                          int precision, int stage, int nshard, int shard, int reps, double *out);
+
+/* Plain device-to-device copy of `bytes` (a multiple of 16) repeated `reps`
+ * times by a streaming kernel (calib_copy_kernel); ms per copy from HIP
+ * events.  Measurement only (no reference counterpart): attainable HBM
+ * bandwidth, and the known-size kernel that scales profiler byte counters. */
+int mps2_copy_bench(mps2_ctx ctx, int64_t bytes, int reps, double *ms_per_copy);
 
 /* TridiagGsSolver (lanczos_solver_impl.h:259-312): lowest eigenpair of the
  * symmetric tridiagonal (a[0..n-1], b[0..n-2]), solved on the device.  vec may

@@ -381,6 +381,38 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     size_t dom = 0;
     for (size_t k = 1; k < per; ++k)
       if (st_ms[k] > st_ms[dom]) dom = k;
+    // Second pass: the dominant stage's launches run as the separately named
+    // kernel instance (tag 1); its ms per launch comes from these launches,
+    // which a profiler's statistics for that instance cover exactly.
+    double dom_ms = per ? st_ms[dom] / reps : 0;
+    if (per) {
+      c->tag_stage = (int)dom;
+      c->time_gemms = true;
+      try {
+        for (int i = 0; i < reps; ++i) mps2::heff_mul(Lp, W1p, W2p, Rp, x, w, y, p);
+      } catch (...) {
+        c->time_gemms = false;
+        c->tag_stage = -1;
+        throw;
+      }
+      c->time_gemms = false;
+      c->tag_stage = -1;
+      c->sync();
+      double tms = 0;
+      int cnt = 0;
+      for (size_t k = 0; k < c->gemm_timings.size(); ++k) {
+        if (k % per == dom) {
+          float t = 0;
+          MPS2_HIP(hipEventElapsedTime(&t, c->gemm_timings[k].start, c->gemm_timings[k].stop));
+          tms += t;
+          ++cnt;
+        }
+        (void)hipEventDestroy(c->gemm_timings[k].start);
+        (void)hipEventDestroy(c->gemm_timings[k].stop);
+      }
+      c->gemm_timings.clear();
+      if (cnt) dom_ms = tms / cnt;
+    }
     double bytes = 0;
     for (auto &t : {opt(L), W1->t, W2->t, opt(R)})
       if (t) bytes += 8.0 * t->n();
@@ -391,9 +423,99 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     out[2] = gemm_ms / reps;
     out[3] = (double)dom;
     out[4] = per ? st_fl[dom] : 0;
-    out[5] = per ? st_ms[dom] / reps : 0;
+    out[5] = dom_ms;
     out[6] = bytes;
     for (size_t k = 0; k < per && k < 4; ++k) out[7 + k] = st_ms[k] / reps;
+  });
+}
+
+int mps2_heff_bench_shard(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                          int precision, int stage, int nshard, int shard, int reps, double *out) {
+  return guard([&] {
+    need(W1, "W1");
+    need(W2, "W2");
+    need(psi, "psi");
+    need(where, "where");
+    need(out, "out");
+    if (reps < 1 || nshard < 1 || shard < 0 || shard >= nshard || stage < 0)
+      throw mps2::Error(mps2::kErrArg, "heff_bench_shard: bad reps, stage or shard");
+    std::string w(where);
+    if (w == "cent" || w == "rend") need(L, "L");
+    if (w == "cent" || w == "lend") need(R, "R");
+    mps2::Ctx *c = psi->t->ctx;
+    auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
+    mps2::Prec p = precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64;
+    auto cast = [&](const mps2::TenP &t) -> mps2::TenP {
+      if (!t || p == mps2::kF64) return t;
+      auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
+      mps2::k_cast_d2f(c, t->n(), t->d(), r->f());
+      return r;
+    };
+    auto Lp = cast(opt(L)), W1p = cast(W1->t), W2p = cast(W2->t), Rp = cast(opt(R));
+    auto x = cast(mps2::embed(psi->t, full));
+    auto y = mps2::ten_alloc(c, full, p);
+    struct Reset {
+      mps2::Ctx *c;
+      ~Reset() {
+        c->tag_stage = -1;
+        c->shard_g = 1;
+        c->shard_r = 0;
+        c->time_gemms = false;
+      }
+    } reset{c};
+    c->tag_stage = stage;
+    c->shard_g = nshard;
+    c->shard_r = shard;
+    mps2::heff_mul(Lp, W1p, W2p, Rp, x, w, y, p);  // warm-up (builds the share's tiles)
+    c->sync();
+    c->gemm_timings.clear();
+    c->time_gemms = true;
+    for (int i = 0; i < reps; ++i) mps2::heff_mul(Lp, W1p, W2p, Rp, x, w, y, p);
+    c->time_gemms = false;
+    c->sync();
+    const size_t per = c->gemm_timings.size() / reps;
+    double tms = 0, full_flops = 0;
+    int cnt = 0;
+    for (size_t k = 0; k < c->gemm_timings.size(); ++k) {
+      if (per && k % per == (size_t)stage) {
+        float t = 0;
+        MPS2_HIP(hipEventElapsedTime(&t, c->gemm_timings[k].start, c->gemm_timings[k].stop));
+        tms += t;
+        full_flops = c->gemm_timings[k].flops;
+        ++cnt;
+      }
+      (void)hipEventDestroy(c->gemm_timings[k].start);
+      (void)hipEventDestroy(c->gemm_timings[k].stop);
+    }
+    c->gemm_timings.clear();
+    if (!cnt) throw mps2::Error(mps2::kErrArg, "heff_bench_shard: stage out of range");
+    out[0] = tms / cnt;
+    out[1] = c->shard_flops;
+    out[2] = full_flops;
+  });
+}
+
+int mps2_copy_bench(mps2_ctx ctx, int64_t bytes, int reps, double *ms_per_copy) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(ms_per_copy, "ms_per_copy");
+    if (bytes < 16 || bytes % 16 || reps < 1) throw mps2::Error(mps2::kErrArg, "bytes must be a positive multiple of 16");
+    mps2::Ctx *c = ctx->c.get();
+    auto a = c->alloc(bytes), b = c->alloc(bytes);
+    mps2::k_fill(c, static_cast<double *>(a->p), bytes / 8, 1.0);
+    mps2::k_calib_copy(c, a->p, b->p, bytes);  // warm-up
+    hipEvent_t e0, e1;
+    MPS2_HIP(hipEventCreate(&e0));
+    MPS2_HIP(hipEventCreate(&e1));
+    MPS2_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < reps; ++i) mps2::k_calib_copy(c, a->p, b->p, bytes);
+    MPS2_HIP(hipEventRecord(e1, c->stream));
+    MPS2_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MPS2_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms_per_copy = ms / reps;
   });
 }
 

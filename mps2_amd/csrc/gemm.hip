@@ -123,7 +123,8 @@ struct Slab {
 // the global loads of the next slab (possibly of the next pair) overlap the
 // matrix-core work on the current one: registers -> LDS double buffer, one
 // barrier per slab.
-template <typename T, int BM, int BN, bool TA, bool TB>
+// TAG only names the instance (profiling: isolates one stage's launches).
+template <typename T, int BM, int BN, bool TA, bool TB, int TAG>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
                                                                  T *__restrict__ C, const TileD *__restrict__ tiles,
                                                                  const BlkD *__restrict__ blks,
@@ -143,6 +144,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   const int tix = blockIdx.x;
   if (tix >= ntiles) return;
   const TileD tile = tiles[tix];
+  if (tile.blk < 0) return;  // padding of a shorter per-XCD queue
   const BlkD blk = blks[tile.blk];
   const int M = blk.M, N = blk.N;
   const int m0 = tile.m0, n0 = tile.n0;
@@ -213,34 +215,47 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
     }
 }
 
-template <typename T, int BM, int BN>
+template <typename T, int BM, int BN, int TAG>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                   const BlkD *blks, const PairD *pairs, int sub) {
   dim3 grid(ntiles), block(kThreads);
   if (la == 0 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, false, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, false, false, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
+                                                                                     ntiles, sub);
   else if (la == 0 && lb == 1)
-    grouped_gemm_kernel<T, BM, BN, false, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, false, true, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
+                                                                                    ntiles, sub);
   else if (la == 1 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, true, false><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, true, false, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
+                                                                                    ntiles, sub);
   else
-    grouped_gemm_kernel<T, BM, BN, true, true><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, true, true, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
+                                                                                   ntiles, sub);
 }
 
 }  // namespace
 
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub) {
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag) {
   if (ntiles <= 0) return;
-  if (tile_m == 128 && tile_n == 128) launch_tiles<T, 128, 128>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
-  else launch_tiles<T, 64, 64>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  const bool big = tile_m == 128 && tile_n == 128;
+  if (tag == 1) {
+    if (big) launch_tiles<T, 128, 128, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    else launch_tiles<T, 64, 64, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  } else if (tag == 2) {
+    if (big) launch_tiles<T, 128, 128, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    else launch_tiles<T, 64, 64, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  } else {
+    if (big) launch_tiles<T, 128, 128, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    else launch_tiles<T, 64, 64, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  }
   c->gemm_launches++;
 }
 
 template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *,
-                                          int, const BlkD *, const PairD *, int, int, int);
+                                          int, const BlkD *, const PairD *, int, int, int, int);
 template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
-                                         const BlkD *, const PairD *, int, int, int);
+                                         const BlkD *, const PairD *, int, int, int, int);
 
 }  // namespace mps2

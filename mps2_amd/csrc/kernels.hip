@@ -291,7 +291,21 @@ __global__ void copy2d_kernel(const double *src, double *dst, const Copy2D *desc
   }
 }
 
+// Plain streaming copy (16 bytes per thread per step), for bandwidth
+// calibration.
+__global__ void calib_copy_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+void k_calib_copy(Ctx *c, const void *src, void *dst, int64_t bytes) {
+  const int64_t n = bytes / 16;
+  if (n > 0)
+    calib_copy_kernel<<<grid_for(n, 256, 65536), 256, 0, c->stream>>>(static_cast<const double2 *>(src),
+                                                                      static_cast<double2 *>(dst), n);
+}
 
 void k_fill(Ctx *c, double *p, int64_t n, double v) {
   fill_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);

@@ -81,6 +81,17 @@ struct Ctx {
     double flops;
   };
   std::vector<GemmTiming> gemm_timings;
+  // Profiling aid: within each effective-Hamiltonian matvec, the grouped GEMM
+  // of stage tag_stage (0-based; -1 = none) runs as a separately named kernel
+  // instance, so a profiler's per-kernel statistics isolate its launches.
+  int tag_stage = -1;
+  int stage_idx = 0;
+  // Benchmark of one GPU's share: with shard_g > 1 the tagged stage computes
+  // only the output blocks owned by rank shard_r of shard_g (blocks assigned
+  // by longest-processing-time on their flops); shard_flops records the
+  // share's algorithmic flops of the last such launch.
+  int shard_g = 1, shard_r = 0;
+  double shard_flops = 0;
   // named event counters (svd calls, Jacobi sweeps, host syncs, ...)
   std::map<std::string, double> counters;
   void count(const char *name, double v = 1.0) { counters[name] += v; }
@@ -145,6 +156,8 @@ struct Copy2D {
   double scale;   // multiply by scale (1 for plain copies)
 };
 void k_copy2d(Ctx *c, const double *src, double *dst, const Copy2D *d_desc, int n);
+// bytes (a multiple of 16) copied by a plain streaming kernel (calibration)
+void k_calib_copy(Ctx *c, const void *src, void *dst, int64_t bytes);
 
 // Grouped GEMM over a contraction plan (gemm.hip).
 struct TileD {
@@ -163,7 +176,7 @@ struct PairD {
 };
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0);
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0);
 
 // ------------------------------------------------------------------ ops (ops.hip)
 // Device-side contraction plan (descriptors uploaded once, cached by layout).
@@ -174,6 +187,13 @@ struct DevPlan {
   int tile_m = 64, tile_n = 64;
   BufP perm_a, perm_b, perm_a_start, perm_b_start;   // permutation descriptors
   int nperm_a = 0, nperm_b = 0;
+  // tiles of one owner's output blocks, by (owners, rank): tiles, count, flops
+  struct Share {
+    BufP tiles;
+    int ntiles = 0;
+    double flops = 0;
+  };
+  mutable std::map<std::pair<int, int>, Share> shares;
 };
 
 std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,

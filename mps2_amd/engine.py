@@ -51,6 +51,9 @@ SIGNATURES = {
     "mps2_lanczos": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_double, C.c_int64, C.c_int, _i64p, _dp,
                                C.POINTER(_vp)]),
     "mps2_tridiag_lowest": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "mps2_copy_bench": (C.c_int, [_vp, C.c_int64, C.c_int, _dp]),
+    "mps2_heff_bench_shard": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, _dp]),
     "mps2_heff_bench": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]),
 }
 
@@ -128,6 +131,16 @@ class Context:
         check(lib().mps2_tridiag_lowest(self.h, len(a), a.ctypes.data_as(_dp), b.ctypes.data_as(_dp), C.byref(ev),
                                         vec.ctypes.data_as(_dp) if want_vec else None))
         return ev.value, vec
+
+
+def put_on(ctx_handle, t: BTensor) -> "DTen":
+    """Import a tensor into the context behind a raw handle (e.g. the DMRG layer's)."""
+    meta, data = t.to_flat()
+    if data.size == 0:
+        data = np.zeros(1)
+    h = _vp()
+    check(lib().mps2_ten_import(ctx_handle, meta.ctypes.data_as(_i32p), data.ctypes.data_as(_dp), C.byref(h)))
+    return DTen(h, None)
 
 
 class DTen:
@@ -216,6 +229,21 @@ def lanczos(L, W1, W2, R, init, where, error, max_iter=200, precision=PREC_FP64)
     check(lib().mps2_lanczos(_h(L), W1.h, W2.h, _h(R), init.h, where.encode(), error, max_iter, precision,
                              C.byref(it), C.byref(e), C.byref(gs)))
     return it.value, e.value, DTen(gs, init.ctx)
+
+
+def heff_bench_shard(L, W1, W2, R, psi, where, precision, stage, nshard, shard, reps=5):
+    """(ms per launch, share flops, stage flops) of rank `shard`'s output blocks of one matvec stage."""
+    out = np.zeros(3)
+    check(lib().mps2_heff_bench_shard(_h(L), W1.h, W2.h, _h(R), psi.h, where.encode(), precision, stage, nshard,
+                                      shard, reps, out.ctypes.data_as(_dp)))
+    return float(out[0]), float(out[1]), float(out[2])
+
+
+def copy_bench(ctx_handle, nbytes, reps=5):
+    """ms per plain device-to-device copy of nbytes (calib_copy_kernel)."""
+    ms = C.c_double()
+    check(lib().mps2_copy_bench(ctx_handle, int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def heff_bench(L, W1, W2, R, psi, where, precision=PREC_FP64, warmup=2, reps=10):
