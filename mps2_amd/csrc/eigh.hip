@@ -45,6 +45,12 @@ constexpr int kRowsPerWave = 2;
 constexpr int kSymvRows = 4 * kRowsPerWave;  // 4 wavefronts per workgroup
 constexpr double kEps = 2.220446049250313e-16;
 
+// Order key of eigenvalue estimates in the merges: NaN sorts as +inf, so the
+// merge positions (dc_setup) and ranks (dc_rank) form permutations for every
+// input and the gathers indexed through them stay inside their blocks even
+// when the data is not finite (the host then rejects the eigenvalues).
+__device__ __forceinline__ double okey(double v) { return v == v ? v : INFINITY; }
+
 struct ESec {
   int64_t nn;   // offset in n*n buffers
   int64_t nv;   // offset in n-vectors
@@ -95,6 +101,9 @@ __device__ void row_step(const ESec &S, int c, const double *__restrict__ A, dou
   const int n = S.n, tid = threadIdx.x, nt = blockDim.x;
   double *a = abuf + S.nv;
   const double *arow = A + S.nn + (int64_t)c * n;
+  // One pass forms w_{c-1}, the corrected row c and the squared norm of its
+  // tail; w_{c-1}[c] is recomputed by every thread instead of broadcast.
+  double ss = 0;
   if (c > 0) {
     const double tau = tauv[S.nv + c - 1];
     const double *vp = VH + S.nn + (int64_t)(c - 1) * n;
@@ -103,22 +112,23 @@ __device__ void row_step(const ESec &S, int c, const double *__restrict__ A, dou
     double part = 0;
     for (int r = c + tid; r < n; r += nt) part += y[r] * vp[r];
     const double kcoef = -0.5 * tau * tau * block_sum(part, red);
+    const double vc = vp[c], wc = tau * y[c] + kcoef * vc;
     for (int r = c + tid; r < n; r += nt) {
       const double x = tau * y[r] + kcoef * vp[r];
       w[r] = x;
-      if (r == c) sh[0] = x;
+      const double av = arow[r] - vc * x - wc * vp[r];
+      a[r] = av;
+      if (r >= c + 2) ss += av * av;
     }
-    __syncthreads();
-    const double wc = sh[0], vc = vp[c];
-    for (int r = c + tid; r < n; r += nt) a[r] = arow[r] - vc * w[r] - wc * vp[r];
   } else {
-    for (int r = tid; r < n; r += nt) a[r] = arow[r];
+    for (int r = tid; r < n; r += nt) {
+      const double av = arow[r];
+      a[r] = av;
+      if (r >= 2) ss += av * av;
+    }
   }
-  __syncthreads();
-  if (tid == 0) dv[S.nv + c] = a[c];
+  if (tid == 0) dv[S.nv + c] = a[c];  // a[c] was written by this thread
   if (c >= n - 1) return;
-  double ss = 0;
-  for (int r = c + 2 + tid; r < n; r += nt) ss += a[r] * a[r];
   const double xn = sqrt(block_sum(ss, red));
   const double alpha = a[c + 1];
   double beta, tau, sc;
@@ -337,7 +347,7 @@ __global__ __launch_bounds__(kT) void dc_setup_kernel(const ESec *__restrict__ s
       int lo = 0, hi = n2;  // count of l2 < D
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (l2[mid] < D) lo = mid + 1;
+        if (okey(l2[mid]) < okey(D)) lo = mid + 1;
         else hi = mid;
       }
       pos = i + lo;
@@ -348,7 +358,7 @@ __global__ __launch_bounds__(kT) void dc_setup_kernel(const ESec *__restrict__ s
       int lo = 0, hi = n1;  // count of l1 <= D
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (l1[mid] <= D) lo = mid + 1;
+        if (okey(l1[mid]) <= okey(D)) lo = mid + 1;
         else hi = mid;
       }
       pos = t + lo;
@@ -608,11 +618,11 @@ __global__ __launch_bounds__(256) void dc_rank_kernel(const ESec *__restrict__ s
   auto val = [&](int w) {
     return w < kk ? B.dK[vo + B.org[vo + w]] + B.tau[vo + w] : B.Dp[vo + B.Didx[vo + w - kk]];
   };
-  const double x = val(u);
+  const double x = val(u), kx = okey(x);
   int rank = 0;
   for (int w = 0; w < k; ++w) {
-    const double y = val(w);
-    rank += (y < x) || (y == x && w < u);
+    const double ky = okey(val(w));
+    rank += (ky < kx) || (ky == kx && w < u);
   }
   // lam of this block was consumed by the setup kernel; write the merged values
   B.zp[vo + rank] = x;    // staged: copied into lam by dc_write_kernel
@@ -839,6 +849,9 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
                             hipMemcpyDeviceToHost, c->stream));
   }
   c->sync();
+  for (int q = 0; q < nsec; ++q)
+    for (double x : res.evals[q])
+      if (!std::isfinite(x)) throw Error(kErrNumeric, "eigh: non-finite eigenvalue (input holds NaN or Inf)");
   return res;
 }
 

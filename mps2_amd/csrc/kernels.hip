@@ -232,17 +232,20 @@ __global__ __launch_bounds__(kRedThreads) void block_maxabs_kernel(const double 
   const int b = blockIdx.x;
   const double *p = x + off[b];
   const int64_t n = size[b];
+  // NaN-propagating maximum (fmax would drop NaN): a block holding NaN
+  // reports NaN, whose bit pattern (sign cleared) orders above +inf.
+  auto nmax = [](double a, double v) { return (v != v || v > a) ? v : a; };
   double m = 0;
   for (int64_t i = blockIdx.y * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.y * blockDim.x)
-    m = fmax(m, fabs(p[i]));
-  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_down(m, o, 64));
+    m = nmax(m, fabs(p[i]));
+  for (int o = 32; o > 0; o >>= 1) m = nmax(m, __shfl_down(m, o, 64));
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) sh[w] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = 0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmax(r, sh[i]);
-    if (r > 0) atomicMax(reinterpret_cast<unsigned long long *>(out) + b, (unsigned long long)__double_as_longlong(r));
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = nmax(r, sh[i]);
+    if (!(r <= 0)) atomicMax(reinterpret_cast<unsigned long long *>(out) + b, (unsigned long long)__double_as_longlong(r));
   }
 }
 

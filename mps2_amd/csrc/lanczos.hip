@@ -289,6 +289,7 @@ LanczosOut lanczos(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R,
   c->count("lanczos_calls");
   c->count("matvec_fp64", (double)r.matvecs_hi);
   c->count("matvec_fp32", (double)r.matvecs_lo);
+  if (!std::isfinite(r.energy)) throw Error(kErrNumeric, "Lanczos: non-finite energy (environment or state holds NaN or Inf)");
   return r;
 }
 

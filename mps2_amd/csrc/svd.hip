@@ -22,7 +22,9 @@
 // on MFMA; sweeps until no rotation exceeds 1e-15 * trace(G_q)) remains
 // selectable with MPS2_EIGH=jacobi.
 //
-// Truncation (host, on the singular values only):
+// Truncation (host, on the singular values only; every nonzero value below
+// kResolve s_max = 1e-11 s_max, which no FP64 SVD orders reproducibly,
+// counts as kResolve s_max):
 //  1. all singular values, sorted descending, ties by (sector, index);
 //  2. D_err = smallest D >= 1 with sum_{i>=D} s_i^2 <= trunc_err * sum s_i^2;
 //  3. D = min(max(D_err, Dmin), Dmax, count);
@@ -50,6 +52,13 @@ namespace {
 constexpr int kB = 32;        // Jacobi block width
 constexpr int kP = 2 * kB;    // subproblem size
 constexpr double kDegTol = 1e-6;
+// Singular values below kResolve * s_max (s_max over all sectors, weights
+// below 1e-22) carry no reproducible order: a direct SVD's absolute error is
+// p(n) eps s_max, p(n) up to a few hundred, and the refined Gram tail below
+// keeps artifacts of up to eps s_max^2 / s_r ~ 1e-12 s_max (leakage of the
+// values s_r >= 3e-4 s_max just above the refinement threshold into the tail
+// basis).  The truncation treats them as tied.
+constexpr double kResolve = 1e-11;
 
 // Jacobi sweeps per 2b x 2b subproblem and per outer round (MPS2_JACOBI_INNER,
 // default 1), and the rotation threshold relative to trace(G_q)
@@ -88,7 +97,7 @@ BufP upload(Ctx *c, const void *h, size_t bytes) {
 
 // Grouped GEMM from host-side block / pair lists (not cached).
 void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *C, const std::vector<BlkD> &blks,
-              const std::vector<PairD> &pairs) {
+              const std::vector<PairD> &pairs, int sub = 0) {
   std::vector<TileD> tiles;
   for (size_t i = 0; i < blks.size(); ++i)
     for (int m0 = 0; m0 < blks[i].M; m0 += 64)
@@ -98,7 +107,7 @@ void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *
   auto db = upload(c, blks.data(), blks.size() * sizeof(BlkD));
   auto dp = upload(c, pairs.data(), pairs.size() * sizeof(PairD));
   launch_grouped_gemm<double>(c, la, lb, A, B, C, static_cast<const TileD *>(dt->p), (int)tiles.size(),
-                              static_cast<const BlkD *>(db->p), static_cast<const PairD *>(dp->p), 64, 64);
+                              static_cast<const BlkD *>(db->p), static_cast<const PairD *>(dp->p), 64, 64, sub);
   c->sync();
 }
 
@@ -586,6 +595,19 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     MPS2_HIP(hipMemcpyAsync(bmax.data(), dout->p, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->sync();
   }
+  // The Gram matrices square the entries and the tridiagonalisation squares
+  // them again, so unnormalised inputs (a canonicalisation sweep over a
+  // random start carries the product of all site norms, 1e80 at 100 sites)
+  // would overflow.  The sector matrices are assembled scaled by 2^-e with
+  // 2^e <= max|t| < 2^(e+1): exact in binary floating point, so U and V^T
+  // are unchanged and S is scaled back by 2^e.
+  double gmax = 0.0;
+  for (int b = 0; b < nblk; ++b) {
+    if (!std::isfinite(bmax[b])) throw Error(kErrNumeric, "SVD: input tensor holds NaN or Inf");
+    gmax = std::max(gmax, bmax[b]);
+  }
+  const int sexp = gmax > 0.0 ? std::max(-1000, std::min(1000, std::ilogb(gmax))) : 0;
+  const double pre = std::ldexp(1.0, -sexp), post = std::ldexp(1.0, sexp);
   // 2. sector structure
   struct Sec {
     QN qn;
@@ -645,7 +667,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       for (int i = 0; i < ldims; ++i) bm *= m.legs[i].dim[cc[i]];
       for (int i = ldims; i < r; ++i) bn *= m.legs[i].dim[cc[i]];
       cp.push_back(Copy2D{m.offset[b], moff[q] + ro * s.n + co, (int32_t)bm, (int32_t)bn, (int32_t)bn,
-                          (int32_t)s.n, 1.0});
+                          (int32_t)s.n, pre});
     }
     if (!cp.empty()) {
       auto dcp = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
@@ -704,25 +726,48 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     c->count("svd_sectors", ns);
     c->count("svd_max_sector_sum", maxn);
   }
-  // 5. singular values: top min(m, n) eigenvalues per sector, descending
-  std::vector<std::vector<double>> sv(ns);
+  // 5. singular values: top min(m, n) eigenvalues per sector, descending.
+  // sv holds sqrt(max(lambda, 0)) (S and the factors); svr holds what the
+  // truncation rule sees.  Below kResolve * s_max a direct SVD returns
+  // rounding noise, tiny but positive, and the Gram eigenvalues there are
+  // +-noise.  svr counts every nonzero value below kResolve * s_max
+  // (negative lambda included) as exactly kResolve * s_max: with trunc_err = 0
+  // all of them are kept, as a direct SVD keeps its positive noise, and a cut
+  // among them is a tie, decided by (sector, index) instead of by rounding.
+  // An exactly zero lambda (a structural zero, which deflation produces
+  // exactly) stays zero and is cut.
+  std::vector<std::vector<double>> sv(ns), svr(ns);
   std::vector<std::vector<int32_t>> order(ns);
   auto singular_values = [&]() {
+    double top = 0.0;
+    for (int q = 0; q < ns; ++q)
+      for (double x : evals[q]) top = std::max(top, x);
+    const double thr = kResolve * std::sqrt(top);
     for (int q = 0; q < ns; ++q) {
       std::vector<int32_t> idx(gn[q]);
       for (int i = 0; i < gn[q]; ++i) idx[i] = i;
       const auto &ev = evals[q];
-      std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) { return ev[x] > ev[y]; });
+      auto rule = [&](int32_t i) {
+        return ev[i] > 0.0 ? std::max(std::sqrt(ev[i]), thr) : (ev[i] < 0.0 ? thr : 0.0);
+      };
+      std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+        const double rx = rule(x), ry = rule(y);
+        return rx != ry ? rx > ry : ev[x] > ev[y];
+      });
       int64_t rq = std::min(secs[q].m, secs[q].n);
       idx.resize(rq);
       order[q] = idx;
       sv[q].resize(rq);
-      for (int64_t j = 0; j < rq; ++j) sv[q][j] = std::sqrt(std::max(ev[idx[j]], 0.0));
+      svr[q].resize(rq);
+      for (int64_t j = 0; j < rq; ++j) {
+        sv[q][j] = std::sqrt(std::max(ev[idx[j]], 0.0));
+        svr[q][j] = rule(idx[j]);
+      }
     }
   };
   singular_values();
   SvdOut out;
-  truncate_rule(sv, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
+  truncate_rule(svr, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
   // Tail refinement (tridiagonal path).  An eigenvalue of G is accurate to
   // about n eps lambda_max absolutely, i.e. a singular value s only to a
   // relative (s_max / s)^2 eps, while the truncation (and its 1e-6 tie
@@ -738,6 +783,11 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   std::vector<int> nsub(ns, 0);
   std::vector<int64_t> esoff(ns, 0);
   BufP Esub;
+  // Large directions of the refined sectors (level-1 values above the tail):
+  // F_L = E_L^T M (left) or M E_L (right), and 1/s^2 per direction.
+  std::vector<int> nL(ns, 0);
+  std::vector<int64_t> fLoff(ns, 0), iLoff(ns, 0);
+  BufP FLb, ILb;
   if (!jac) {
     const auto tr0 = std::chrono::steady_clock::now();
     constexpr double kRefine = 1e-7;
@@ -746,10 +796,15 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       for (double x : evals[q]) lam_top[q] = std::max(lam_top[q], x);
     for (int round = 0; round < 3; ++round) {
       std::vector<double> all;
-      for (auto &v : sv) all.insert(all.end(), v.begin(), v.end());
+      for (auto &v : svr) all.insert(all.end(), v.begin(), v.end());
       if (out.D <= 0 || out.D >= (int64_t)all.size()) break;
       std::nth_element(all.begin(), all.begin() + (out.D - 1), all.end(), std::greater<double>());
-      const double cut2 = all[out.D - 1] * all[out.D - 1];
+      // A cut below kResolve * s_max is a tie among the values counted as
+      // kResolve * s_max; what must be resolved then is which values lie
+      // above that level, so the threshold takes the cut's place.
+      const double cut =
+          std::max(all[out.D - 1], kResolve * *std::max_element(all.begin(), all.end()));
+      const double cut2 = cut * cut;
       std::vector<int> rs, rn;
       for (int q = 0; q < ns; ++q) {
         const double lim = kRefine * lam_top[q];
@@ -787,6 +842,52 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         }
         Esub = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
         eigh_vectors_all(c, er, cols, static_cast<double *>(Esub->p), esoff);
+        // The computed tail basis leans into each large direction u_r by
+        // theta ~ eps lambda_max / lambda_r, which puts rows theta s_r v_r^T
+        // into W_S and lifts true tail values to about eps s_max^2 / s_r
+        // (1e-12 s_max for s_r near the tail threshold).  The v_r = M^T u_r / s_r
+        // of the large values are accurate (their tail admixture is scaled by
+        // s_tail / s_r), so W_S is projected off them below, every round.
+        std::vector<std::vector<int32_t>> colsL(ns);
+        std::vector<int64_t> eLoff(ns, 0);
+        int64_t totE = 0, totF = 0, totI = 0;
+        std::vector<double> is2;
+        for (int i = 0; i < nr; ++i) {
+          const int q = rs[i], kl = gn[q] - rn[i];
+          nL[q] = kl;
+          if (kl == 0) continue;
+          colsL[q].resize(kl);
+          for (int j = 0; j < kl; ++j) colsL[q][j] = rn[i] + j;
+          eLoff[q] = totE;
+          totE += (int64_t)gn[q] * kl;
+          fLoff[q] = totF;
+          totF += (int64_t)kl * (side == kOrthLeft ? secs[q].n : secs[q].m);
+          iLoff[q] = totI;
+          totI += kl;
+          for (int j = 0; j < kl; ++j) is2.push_back(1.0 / evals[q][rn[i] + j]);
+        }
+        if (totE > 0) {
+          auto ELb = c->alloc(totE * sizeof(double));
+          eigh_vectors_all(c, er, colsL, static_cast<double *>(ELb->p), eLoff);
+          FLb = c->alloc(totF * sizeof(double));
+          std::vector<BlkD> fb;
+          std::vector<PairD> fp;
+          for (int q = 0; q < ns; ++q) {
+            if (nL[q] == 0) continue;
+            const int32_t kl = nL[q], sm = (int32_t)secs[q].m, sn = (int32_t)secs[q].n, idx = (int32_t)fb.size();
+            if (side == kOrthLeft) {
+              fb.push_back(BlkD{fLoff[q], kl, sn, idx, idx + 1});  // F_L = E_L^T M
+              fp.push_back(PairD{eLoff[q], moff[q], sm, 0});
+            } else {
+              fb.push_back(BlkD{fLoff[q], sm, kl, idx, idx + 1});  // F_L = M E_L
+              fp.push_back(PairD{moff[q], eLoff[q], sn, 0});
+            }
+          }
+          if (side == kOrthLeft) raw_gemm(c, kTrans, kNormal, static_cast<double *>(ELb->p), Md,
+                                          static_cast<double *>(FLb->p), fb, fp);
+          else raw_gemm(c, kNormal, kNormal, Md, static_cast<double *>(ELb->p), static_cast<double *>(FLb->p), fb, fp);
+          ILb = upload(c, is2.data(), is2.size() * sizeof(double));
+        }
       }
       const double *Es = static_cast<const double *>(Esub->p);
       // W_S and the Gram of the projected tail
@@ -819,13 +920,53 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
           gp2.push_back(PairD{woff[i], woff[i], sm, 0});
         }
       }
-      if (side == kOrthLeft) {
-        raw_gemm(c, kTrans, kNormal, Es, Md, W_, wb, wp);
-        raw_gemm(c, kNormal, kTrans, W_, W_, G_, gb2, gp2);
-      } else {
-        raw_gemm(c, kNormal, kNormal, Md, Es, W_, wb, wp);
-        raw_gemm(c, kTrans, kNormal, W_, W_, G_, gb2, gp2);
+      if (side == kOrthLeft) raw_gemm(c, kTrans, kNormal, Es, Md, W_, wb, wp);
+      else raw_gemm(c, kNormal, kNormal, Md, Es, W_, wb, wp);
+      if (FLb) {
+        // W_S -= (W_S F_L^T) diag(1/s^2) F_L   (left; right: F_L diag(1/s^2) F_L^T W_S)
+        const double *FL = static_cast<const double *>(FLb->p);
+        std::vector<int64_t> poff(nr, 0);
+        int64_t ptot = 0;
+        for (int i = 0; i < nr; ++i) {
+          poff[i] = ptot;
+          ptot += (int64_t)rn[i] * nL[rs[i]];
+        }
+        auto bP = c->alloc(std::max<int64_t>(ptot, 1) * sizeof(double));
+        double *P = static_cast<double *>(bP->p);
+        std::vector<BlkD> pb, ub;
+        std::vector<PairD> pp, up;
+        std::vector<ScaleD> sd;
+        for (int i = 0; i < nr; ++i) {
+          const int q = rs[i];
+          const int32_t r = rn[i], kl = nL[q], sm = (int32_t)secs[q].m, sn = (int32_t)secs[q].n;
+          if (kl == 0) continue;
+          const int32_t idx = (int32_t)pb.size();
+          if (side == kOrthLeft) {
+            pb.push_back(BlkD{poff[i], r, kl, idx, idx + 1});  // P = W F_L^T
+            pp.push_back(PairD{woff[i], fLoff[q], sn, 0});
+            sd.push_back(ScaleD{poff[i], iLoff[q], r, kl, 1, 0});
+            ub.push_back(BlkD{woff[i], r, sn, idx, idx + 1});  // W -= P F_L
+            up.push_back(PairD{poff[i], fLoff[q], kl, 0});
+          } else {
+            pb.push_back(BlkD{poff[i], kl, r, idx, idx + 1});  // P = F_L^T W
+            pp.push_back(PairD{fLoff[q], woff[i], sm, 0});
+            sd.push_back(ScaleD{poff[i], iLoff[q], kl, r, 0, 0});
+            ub.push_back(BlkD{woff[i], sm, r, idx, idx + 1});  // W -= F_L P
+            up.push_back(PairD{fLoff[q], poff[i], kl, 0});
+          }
+        }
+        if (!pb.empty()) {
+          if (side == kOrthLeft) raw_gemm(c, kNormal, kTrans, W_, FL, P, pb, pp);
+          else raw_gemm(c, kTrans, kNormal, FL, W_, P, pb, pp);
+          auto dsd = upload(c, sd.data(), sd.size() * sizeof(ScaleD));
+          scale_batch_kernel<<<dim3(256, (unsigned)sd.size()), 256, 0, c->stream>>>(
+              P, static_cast<const double *>(ILb->p), static_cast<const ScaleD *>(dsd->p));
+          if (side == kOrthLeft) raw_gemm(c, kNormal, kNormal, P, FL, W_, ub, up, 1);
+          else raw_gemm(c, kNormal, kNormal, FL, P, W_, ub, up, 1);
+        }
       }
+      if (side == kOrthLeft) raw_gemm(c, kNormal, kTrans, W_, W_, G_, gb2, gp2);
+      else raw_gemm(c, kTrans, kNormal, W_, W_, G_, gb2, gp2);
       EighResult er2 = eigh_batched(c, G_, g2off, rn);
       for (int i = 0; i < nr; ++i) {
         const int q = rs[i], r = rn[i];
@@ -834,17 +975,19 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         for (int j = 0; j < r; ++j) lam_top[q] = std::max(lam_top[q], er2.evals[i][j]);
       }
       singular_values();
-      truncate_rule(sv, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
+      truncate_rule(svr, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
       // The kept vectors stay the tridiagonal solution's (the products S Vt =
       // U^T M, or U S = M V, are exact projections for any basis of the kept
       // span); refined vectors are formed only for a tail that a further
       // round must resolve.
       if (round == 2) break;
       all.clear();
-      for (auto &v : sv) all.insert(all.end(), v.begin(), v.end());
+      for (auto &v : svr) all.insert(all.end(), v.begin(), v.end());
       if (out.D <= 0 || out.D >= (int64_t)all.size()) break;
       std::nth_element(all.begin(), all.begin() + (out.D - 1), all.end(), std::greater<double>());
-      const double cut2n = all[out.D - 1] * all[out.D - 1];
+      const double cutn =
+          std::max(all[out.D - 1], kResolve * *std::max_element(all.begin(), all.end()));
+      const double cut2n = cutn * cutn;
       std::vector<std::vector<int32_t>> cols2(nr);
       std::vector<int> r2(nr, 0);
       bool more = false;
@@ -935,7 +1078,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       if (secpos[q] < 0) continue;
       int32_t b = sm->find(Coord{secpos[q], secpos[q]});
       int64_t kq = out.kept[q];
-      for (int64_t k = 0; k < kq; ++k) sh[sm->offset[b] + k * kq + k] = sv[q][k];
+      for (int64_t k = 0; k < kq; ++k) sh[sm->offset[b] + k * kq + k] = sv[q][k] * post;
     }
     out.s = ten_upload(c, sm, sh.data());
   }

@@ -623,6 +623,18 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
     }
   }
   int ns = (int)secs.size();
+  // LAPACK's gesdd, behind gqten's SVD, is accurate at any finite scale (it
+  // rescales out-of-range matrices); the Jacobi sums of squares below are
+  // not, so the sector matrices are taken scaled by 2^-e, 2^e <= max|t| <
+  // 2^(e+1) (exact, so U and V are unchanged), and S is scaled back.
+  double gmax = 0.0;
+  for (auto &kv : t.blk)
+    for (double v : kv.second) {
+      if (!std::isfinite(v)) throw std::runtime_error("svd: input holds NaN or Inf");
+      gmax = std::max(gmax, std::fabs(v));
+    }
+  const int sexp = gmax > 0.0 ? std::max(-1000, std::min(1000, std::ilogb(gmax))) : 0;
+  const double pre = std::ldexp(1.0, -sexp), post = std::ldexp(1.0, sexp);
   std::vector<std::vector<double>> Us(ns), Ss(ns), Vs(ns);
   for (int q = 0; q < ns; ++q) {
     auto &s = secs[q];
@@ -637,12 +649,26 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
       for (int i = 0; i < ldims; ++i) bm *= t.legs[i].dim[kv.first[i]];
       for (int i = ldims; i < r; ++i) bn *= t.legs[i].dim[kv.first[i]];
       for (int64_t i = 0; i < bm; ++i)
-        for (int64_t j = 0; j < bn; ++j) M[(size_t)(ir->second + i) * s.n + ic->second + j] = kv.second[i * bn + j];
+        for (int64_t j = 0; j < bn; ++j) M[(size_t)(ir->second + i) * s.n + ic->second + j] = kv.second[i * bn + j] * pre;
     }
     svd_jacobi(s.m, s.n, M.data(), Us[q], Ss[q], Vs[q]);
   }
+  // Below 1e-11 s_max (s_max over all sectors; weights below 1e-22) FP64
+  // SVD values carry no reproducible order (a direct SVD's absolute error is
+  // p(n) eps s_max; a Gram-based one keeps larger artifacts).  The rule
+  // counts every nonzero value below 1e-11 s_max as exactly 1e-11 s_max, so
+  // a cut among them is a tie decided by (sector, index), not by rounding;
+  // exact zeros stay zero and are cut.
+  double smax_all = 0.0;
+  for (auto &v : Ss)
+    for (double x : v) smax_all = std::max(smax_all, x);
+  const double thr = 1e-11 * smax_all;
+  std::vector<std::vector<double>> Sr(Ss);
+  for (auto &v : Sr)
+    for (double &x : v)
+      if (x > 0.0 && x < thr) x = thr;
   std::vector<int64_t> kept;
-  truncate_rule(Ss, trunc_err, dmin, dmax, kept, err, D);
+  truncate_rule(Sr, trunc_err, dmin, dmax, kept, err, D);
   Leg knew;
   knew.dir = 1;
   std::vector<int> secpos(ns, -1);
@@ -688,7 +714,7 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
       Vt.blk[c] = std::move(b);
     }
     std::vector<double> sd((size_t)kq * kq, 0.0);
-    for (int64_t k = 0; k < kq; ++k) sd[k * kq + k] = Ss[q][k];
+    for (int64_t k = 0; k < kq; ++k) sd[k * kq + k] = Ss[q][k] * post;
     S.blk[Coord{secpos[q], secpos[q]}] = std::move(sd);
   }
 }

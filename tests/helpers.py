@@ -9,6 +9,7 @@ mpogen_impl.h:359,377,398; test_lanczos_solver.cc:121-168).
 import numpy as np
 
 from mps2_amd.flat import BTensor, Leg, IN, OUT
+from oracle import oracle_py as O
 
 
 def bond(dir, dims, qns, qw=1):
@@ -115,3 +116,17 @@ def max_rel_diff(a: BTensor, b: BTensor):
     da, db = a.dense(), b.dense()
     s = max(np.abs(db).max(), 1e-300)
     return float(np.abs(da - db).max() / s)
+
+
+def rank_deficient_state():
+    """psi = A B through a bond of total dimension 12 (sectors -3, -1, 1, 3 of
+    dimensions 2, 4, 3, 3): 12 resolved singular values, the other 112 at
+    rounding level."""
+    rng = np.random.default_rng(5)
+    l0 = bond(IN, (20, 30, 12), (0, 2, -2))
+    p = spin_phys(OUT)
+    k = bond(OUT, (3, 4, 3, 2), (1, -1, 3, -3))
+    r = bond(OUT, (21, 31, 13, 2), (0, 2, -2, 4))
+    A = BTensor.random([l0, p, k], (0,), rng)
+    B = BTensor.random([k.inverse(), p, r], (0,), rng)
+    return O.contract(A, B, [[2], [0]])

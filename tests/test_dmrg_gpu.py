@@ -95,3 +95,52 @@ def test_mixed_sweep_matches_fp64():
     rel = np.abs(gmx["energies"] - g64["energies"]) / np.abs(g64["energies"])
     assert rel.max() < 1e-9, rel.max()
     assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
+
+
+def test_heisenberg_l100_config2_ramp():
+    """Heisenberg OBC L=100, U(1) Sz=0, D ramped 256,256,512,512,768,768,1024,1024
+    over 8 sweeps (FP64 on the GPU), from a seeded random MPS.  No reference
+    output exists at this size, so the checks are what the physics fixes:
+    the energy is variational (non-increasing over the last sweeps, within the
+    Lanczos tolerance), D reaches 1024, and the converged energy agrees with
+    the DMRG value of this chain, E0 = -44.12773924, to 1e-5.
+
+    Then mixed precision against FP64, one sweep each from the same state.  At
+    Dmax 1024 this chain's cuts fall among Schmidt values of 1e-11 .. 1e-20
+    s_max, below what a state converged to a 1e-10 energy tolerance fixes, so
+    the per-sector split of those bonds (and the count of the next bond, which
+    follows from it) may differ between the two runs, as between any two
+    solvers; energies must agree per update to 1e-9 and finally to 1e-10, and
+    kept counts may differ only at updates discarding less than 1e-16 (values
+    below 1e-11 s_max enter the reported weight as 1e-22 each).  At Dmax 256
+    the cuts are resolved and the kept counts must be identical."""
+    N = 100
+    m = HM.heisenberg_chain(N)
+    m.init_mps([[i % 2 for i in range(N)], [(i + 1) % 2 for i in range(N)]], enlarged_dim=8, seed=0x5EED)
+    sched = [256, 256, 512, 512, 768, 768, 1024, 1024]
+    r = m.dmrg(len(sched), 1, 1024, 0.0, 1e-10, 200, dmax_schedule=sched, verbose=True)
+    per = 2 * (N - 1)
+    e_sweep = r["energies"].reshape(len(sched), per)[:, -1]
+    dmax_sweep = r["dims"].reshape(len(sched), per).max(axis=1)
+    info = f"energies per sweep {e_sweep.tolist()}, max D per sweep {dmax_sweep.tolist()}"
+    assert np.all(np.diff(e_sweep[-4:]) <= 1e-9), info
+    assert dmax_sweep[-1] == 1024, info
+    assert abs(r["energy"] - (-44.12773924)) < 1e-5, info
+    m.checkpoint()
+    for dmax in (1024, 256):
+        m.restore()
+        g64 = m.dmrg(1, 1, dmax, 0.0, 1e-10, 200, verbose=True)
+        m.restore()
+        gmx = m.dmrg(1, 1, dmax, 0.0, 1e-10, 200, precision=1, verbose=True)
+        rel = np.abs(gmx["energies"] - g64["energies"]) / np.abs(g64["energies"])
+        assert rel.max() < 1e-9, (dmax, rel.max())
+        assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
+        diff = np.nonzero(gmx["dims"] != g64["dims"])[0]
+        msg = (dmax, diff.tolist(), g64["dims"][diff].tolist(), gmx["dims"][diff].tolist(),
+               g64["trunc_errs"][diff].tolist())
+        if dmax == 256:
+            assert diff.size == 0, msg
+        else:
+            # an update whose count differs keeps everything or cuts among
+            # values the converged state does not fix
+            assert np.all(np.maximum(g64["trunc_errs"][diff], gmx["trunc_errs"][diff]) < 1e-16), msg

@@ -193,6 +193,82 @@ def test_svd_parity(ctx, orth, dims):
             assert np.abs(rec - psi.dense()).max() < 1e-11
 
 
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+def test_svd_rank_deficient_parity(ctx, orth):
+    """psi = A B through a bond of total dimension 12: every sector matrix is
+    numerically rank-deficient (112 of 124 singular values at rounding level,
+    1e-17 relative).  Values below 1e-11 s_max have no reproducible order in
+    any SVD; the rule keeps them all with trunc_err = 0 (a direct SVD returns
+    them as tiny positive numbers) and breaks a cut among them by (sector,
+    index).  The Gram path's +-noise eigenvalues there must give the
+    oracle's kept counts per sector: all kept (Dmax 10000), a cut at the
+    rank (12), and cuts inside the unresolved values (13, 16).  The product
+    used by the sweep reconstructs psi and the orthonormal side is
+    orthonormal."""
+    psi = H.rank_deficient_state()
+    dpsi = ctx.put(psi)
+    for dmax in (10_000, 9, 12, 13, 16):
+        Uo, So, Vo, eo, Do = O.svd(psi, 2, (0,), 0.0, 1, dmax)
+        u, s, vt, eg, Dg = E.svd(dpsi, 2, (0,), 0.0, 1, dmax, orth)
+        Ug, Sg, Vg = u.get(), s.get(), vt.get()
+        assert Dg == Do, (dmax, Dg, Do)
+        assert Ug.legs[-1].key() == Uo.legs[-1].key(), (dmax, Ug.legs[-1].key(), Uo.legs[-1].key())
+        so, sg = np.sort(np.diag(So.dense())), np.sort(np.diag(Sg.dense()))
+        big = so > 1e-6 * so.max()
+        assert np.abs(so[big] - sg[big]).max() < 1e-10 * so.max()
+        assert np.all(sg[~big] < 1e-6 * so.max())
+        if orth == E.ORTH_LEFT:
+            u2 = Ug.dense().reshape(-1, Ug.dense().shape[-1])
+            assert np.abs(u2.T @ u2 - np.eye(u2.shape[1])).max() < 1e-12
+        else:
+            v2 = Vg.dense().reshape(Vg.dense().shape[0], -1)
+            assert np.abs(v2 @ v2.T - np.eye(v2.shape[0])).max() < 1e-12
+        if dmax >= 12:
+            rec = np.einsum("abk,kl,lcd->abcd", Ug.dense(), Sg.dense(), Vg.dense(), optimize=True)
+            assert np.abs(rec - psi.dense()).max() < 1e-11 * np.abs(psi.dense()).max()
+
+
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+def test_svd_scale_invariance(ctx, orth):
+    """Unnormalised inputs: a right-canonicalisation of a random 100-site
+    start carries the product of the site norms (~1e80), whose Gram entries
+    (1e160) and their squares overflow FP64 unless the SVD rescales.  The
+    power-of-two scaling is exact: scaling the input by 2^300 leaves U and
+    V^T bit-identical and multiplies S by exactly 2^300.  Inputs of scale
+    1e150 and 1e-150 (not powers of two) agree with the oracle's SVD of the
+    same tensor to FP64 accuracy; a NaN input raises instead of running."""
+    rng = np.random.default_rng(77 + orth)
+    l0 = H.bond(IN, (30, 50, 20), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (31, 51, 21, 2), (0, 2, -2, 4))
+    psi = BTensor.random([l0, p, p, r], (0,), rng)
+    dmax = 40
+    u1, s1, v1, e1, D1 = E.svd(ctx.put(psi), 2, (0,), 0.0, 1, dmax, orth)
+    big = BTensor(psi.legs, psi.qw, {c: b * 2.0 ** 300 for c, b in psi.blocks.items()})
+    u2, s2, v2, e2, D2 = E.svd(ctx.put(big), 2, (0,), 0.0, 1, dmax, orth)
+    assert D2 == D1 and e2 == e1
+    for a, b in ((u1.get(), u2.get()), (v1.get(), v2.get())):
+        assert sorted(a.blocks) == sorted(b.blocks)
+        for k in a.blocks:
+            assert np.array_equal(a.blocks[k], b.blocks[k])
+    S1, S2 = s1.get(), s2.get()
+    for k in S1.blocks:
+        assert np.array_equal(S1.blocks[k] * 2.0 ** 300, S2.blocks[k])
+    for scale in (1e150, 1e-150):
+        t = BTensor(psi.legs, psi.qw, {c: b * scale for c, b in psi.blocks.items()})
+        Uo, So, Vo, eo, Do = O.svd(t, 2, (0,), 0.0, 1, dmax)
+        u, s, vt, eg, Dg = E.svd(ctx.put(t), 2, (0,), 0.0, 1, dmax, orth)
+        assert Dg == Do and u.get().legs[-1].key() == Uo.legs[-1].key()
+        so, sg = np.diag(So.dense()), np.diag(s.get().dense())
+        assert np.all(np.isfinite(sg))
+        assert np.abs(so - sg).max() < 1e-10 * so.max()
+        assert abs(eg - eo) < 1e-10
+    bad = BTensor(psi.legs, psi.qw, {c: b.copy() for c, b in psi.blocks.items()})
+    next(iter(bad.blocks.values())).flat[0] = np.nan
+    with pytest.raises(E.EngineError, match="NaN"):
+        E.svd(ctx.put(bad), 2, (0,), 0.0, 1, dmax, orth)
+
+
 def test_linear_combine_and_normalize(ctx):
     rng = np.random.default_rng(41)
     l0 = H.bond(IN, (5, 6), (0, 2))

@@ -145,6 +145,25 @@ def test_oracle_sweep_reproduces_golden_energy(name):
     assert abs(r["energy"] - case["e0"]) < case["tol"], (r["energy"], case["e0"])
 
 
+def test_oracle_centralize_l100_random_start_is_right_canonical():
+    """Centralize(0) of the seeded random 100-site start: the tensors it
+    right-canonicalises carry the product of the site norms (~1e80 at
+    site 1), past the range where sums of squares of their entries stay in
+    FP64.  Every site i >= 1 must still come out right-orthonormal
+    (sum_{s,r} A_i[l,s,r] A_i[l',s,r] = delta_{l l'}) and the state's norm
+    must sit in the head tensor."""
+    N = 100
+    m = HM.heisenberg_chain(N)
+    m.init_mps([[i % 2 for i in range(N)], [(i + 1) % 2 for i in range(N)]], enlarged_dim=8, seed=0x5EED,
+               centralize=False)
+    mps = O.centralize0([m.mps(i) for i in range(N)])
+    for i in range(1, N):
+        a = mps[i].dense()
+        a2 = a.reshape(a.shape[0], -1)
+        assert np.abs(a2 @ a2.T - np.eye(a2.shape[0])).max() < 1e-12, i
+    assert 1e79 < mps[0].norm() < 1e82
+
+
 def test_oracle_config1_heisenberg_l16_matches_ed():
     """Heisenberg OBC L=16, U(1) Sz=0, Dmax=64, 4 sweeps, Lanczos tol 1e-10:
     the oracle's final energy against exact diagonalisation."""

@@ -148,3 +148,54 @@ def test_heff_cent_matches_dense():
     hp = O.heff_mul(L, W1, W2, R, psi, "cent")
     ref = np.einsum("abc,bdef,fghi,jik,adgj->cehk", L.dense(), W1.dense(), W2.dense(), R.dense(), psi.dense())
     assert np.abs(hp.dense() - ref).max() < 1e-12
+
+
+def test_svd_scale_equivariance():
+    """SVD(c A) = U (c S) V^T: scaling by 2^300 (exact in binary floating
+    point) leaves U and V^T bit-identical and S scaled exactly; scales
+    1e150 and 1e-150, where unscaled sums of squares would leave the FP64
+    range, agree with the unscaled result to rounding and keep the same
+    dimensions and truncation error."""
+    rng = np.random.default_rng(77)
+    l0 = H.bond(IN, (30, 50, 20), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (31, 51, 21, 2), (0, 2, -2, 4))
+    psi = BTensor.random([l0, p, p, r], (0,), rng)
+    U0, S0, V0, e0, D0 = O.svd(psi, 2, (0,), 0.0, 1, 40)
+    s0 = np.diag(S0.dense())
+    for sc in (2.0 ** 300, 1e150, 1e-150):
+        t = BTensor(psi.legs, psi.qw, {c: b * sc for c, b in psi.blocks.items()})
+        U, S, V, e, D = O.svd(t, 2, (0,), 0.0, 1, 40)
+        assert D == D0 and U.legs[-1].key() == U0.legs[-1].key()
+        s = np.diag(S.dense())
+        if sc == 2.0 ** 300:
+            assert np.array_equal(s, s0 * sc) and e == e0
+            assert all(np.array_equal(U.blocks[k], U0.blocks[k]) for k in U0.blocks)
+            assert all(np.array_equal(V.blocks[k], V0.blocks[k]) for k in V0.blocks)
+        else:
+            assert np.abs(s / sc - s0).max() < 1e-13 * s0.max()
+            assert abs(e - e0) < 1e-13
+    bad = BTensor(psi.legs, psi.qw, {c: b.copy() for c, b in psi.blocks.items()})
+    next(iter(bad.blocks.values())).flat[0] = np.nan
+    with pytest.raises(RuntimeError, match="NaN"):
+        O.svd(bad, 2, (0,), 0.0, 1, 40)
+
+
+def test_svd_unresolved_values_rule():
+    """Values below 1e-11 s_max have no reproducible order: with trunc_err = 0
+    they are all kept (a direct SVD returns them as tiny positive numbers),
+    and a cut among them is a tie decided by (sector, index), so one state
+    beyond the rank goes to the lowest-QN sector.  A cut above them is
+    decided by value (sector ranks 2, 4, 3, 3)."""
+    psi = H.rank_deficient_state()
+    U, S, V, e, D = O.svd(psi, 2, (0,), 0.0, 1, 10_000)
+    s = np.diag(S.dense())
+    assert D == 124 and (s < 1e-11 * s.max()).sum() == 112
+    U, S, V, e, D = O.svd(psi, 2, (0,), 0.0, 1, 12)
+    assert U.legs[-1].key() == (1, (((-3,), 2), ((-1,), 4), ((1,), 3), ((3,), 3)))
+    U, S, V, e, D = O.svd(psi, 2, (0,), 0.0, 1, 13)
+    assert U.legs[-1].key() == (1, (((-3,), 3), ((-1,), 4), ((1,), 3), ((3,), 3)))
+    U, S, V, e, D = O.svd(psi, 2, (0,), 0.0, 1, 16)
+    assert U.legs[-1].key() == (1, (((-3,), 6), ((-1,), 4), ((1,), 3), ((3,), 3)))
+    # the discarded weight of the unresolved values is negligible
+    assert e < 1e-20
