@@ -111,6 +111,34 @@ void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *
   c->sync();
 }
 
+// Largest |x| over n device doubles (NaN propagates); host value.
+double dev_maxabs(Ctx *c, const double *x, int64_t n) {
+  if (n <= 0) return 0.0;
+  const int64_t off = 0;
+  auto doff = upload(c, &off, sizeof(int64_t));
+  auto dsz = upload(c, &n, sizeof(int64_t));
+  auto dout = c->alloc(sizeof(double));
+  k_block_maxabs(c, x, static_cast<const int64_t *>(doff->p), static_cast<const int64_t *>(dsz->p), 1,
+                 static_cast<double *>(dout->p));
+  double h = 0;
+  MPS2_HIP(hipMemcpyAsync(&h, dout->p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  c->sync();
+  return h;
+}
+
+// Eigen-decomposition with the stage named in any numeric error.
+EighResult eigh_named(Ctx *c, double *G, const std::vector<int64_t> &goff, const std::vector<int> &ns,
+                      const std::string &stage) {
+  try {
+    return eigh_batched(c, G, goff, ns);
+  } catch (const Error &e) {
+    int nmax = 0;
+    for (int x : ns) nmax = std::max(nmax, x);
+    throw Error(e.code, "SVD " + stage + " (" + std::to_string(ns.size()) + " sectors, largest " +
+                            std::to_string(nmax) + "): " + e.what());
+  }
+}
+
 struct SecDesc {     // per sector, device
   int64_t g_off;     // offset of padded Gram / eigenvector matrices
   int32_t np;        // padded size nb * kB
@@ -379,6 +407,34 @@ __global__ void scale_batch_kernel(double *X, const double *sc, const ScaleD *de
   }
 }
 
+// Sums of squares of the rows (mode 0: one wavefront per row, lanes along
+// the row) or columns (mode 1: one thread per column) of each descriptor's
+// block (rows x cols, row-major at off); results at out[sc_off + i].
+__global__ __launch_bounds__(256) void sumsq_batch_kernel(const double *__restrict__ X, double *out,
+                                                          const ScaleD *__restrict__ desc) {
+  const ScaleD d = desc[blockIdx.y];
+  if (d.mode == 0) {
+    const int lane = threadIdx.x & 63;
+    for (int i = blockIdx.x * 4 + (threadIdx.x >> 6); i < d.rows; i += gridDim.x * 4) {
+      const double *row = X + d.off + (int64_t)i * d.cols;
+      double acc = 0;
+      for (int j = lane; j < d.cols; j += 64) acc += row[j] * row[j];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+      if (lane == 0) out[d.sc_off + i] = acc;
+    }
+  } else {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.cols; j += gridDim.x * blockDim.x) {
+      double acc = 0;
+      for (int i = 0; i < d.rows; ++i) {
+        const double x = X[d.off + (int64_t)i * d.cols + j];
+        acc += x * x;
+      }
+      out[d.sc_off + j] = acc;
+    }
+  }
+}
+
 // Transposed block copies, one descriptor per blockIdx.x:
 // dst[dst_off + c * ldd + r] = src[src_off + r * lds + c], r < rows, c < cols.
 __global__ void copy2d_t_kernel(const double *src, double *dst, const Copy2D *desc) {
@@ -595,12 +651,16 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     MPS2_HIP(hipMemcpyAsync(bmax.data(), dout->p, nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     c->sync();
   }
-  // The Gram matrices square the entries and the tridiagonalisation squares
-  // them again, so unnormalised inputs (a canonicalisation sweep over a
-  // random start carries the product of all site norms, 1e80 at 100 sites)
-  // would overflow.  The sector matrices are assembled scaled by 2^-e with
-  // 2^e <= max|t| < 2^(e+1): exact in binary floating point, so U and V^T
-  // are unchanged and S is scaled back by 2^e.
+  // The Gram matrices square the entries and the eigensolver squares and
+  // inverts them again, so unnormalised inputs (a canonicalisation sweep
+  // over a random start carries the product of all site norms, 1e80 at 100
+  // sites) would overflow, and sectors of negligible weight (1e-63 of the
+  // largest entry at the ends of a long chain) underflow.  Each sector matrix
+  // is assembled scaled by 2^-e_q with 2^e_q <= max|M_q| < 2^(e_q+1): exact
+  // in binary floating point, so U and V^T are unchanged.  Host-side values
+  // are kept in one unit, the tensor scaled by 2^-e with e from the largest
+  // entry overall (S is scaled back by 2^e); a sector's device values differ
+  // from it by the exact factor 2^(e - e_q) (eigenvalues 4^(e - e_q)).
   double gmax = 0.0;
   for (int b = 0; b < nblk; ++b) {
     if (!std::isfinite(bmax[b])) throw Error(kErrNumeric, "SVD: input tensor holds NaN or Inf");
@@ -650,6 +710,18 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     moff[q] = moff_tot;
     moff_tot += s.m * s.n;
   }
+  // per-sector exponents dq = e_q - e (<= 0): device value = host value * 2^-dq
+  std::vector<int> dq(ns, 0);
+  {
+    std::vector<double> smax(ns, 0.0);
+    for (int b = 0; b < nblk; ++b) {
+      if (!(bmax[b] > 0.0)) continue;
+      const int q = (int)std::distance(smap.begin(), smap.find(kqn(m.coords[b])));
+      smax[q] = std::max(smax[q], bmax[b]);
+    }
+    for (int q = 0; q < ns; ++q)
+      dq[q] = smax[q] > 0.0 ? std::max(-1000, std::min(1000, std::ilogb(smax[q]))) - sexp : 0;
+  }
   // 3. assemble sector matrices
   auto Mb = c->alloc(std::max<int64_t>(moff_tot, 1) * sizeof(double));
   double *Md = static_cast<double *>(Mb->p);
@@ -667,7 +739,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       for (int i = 0; i < ldims; ++i) bm *= m.legs[i].dim[cc[i]];
       for (int i = ldims; i < r; ++i) bn *= m.legs[i].dim[cc[i]];
       cp.push_back(Copy2D{m.offset[b], moff[q] + ro * s.n + co, (int32_t)bm, (int32_t)bn, (int32_t)bn,
-                          (int32_t)s.n, pre});
+                          (int32_t)s.n, std::ldexp(pre, -dq[q])});
     }
     if (!cp.empty()) {
       auto dcp = upload(c, cp.data(), cp.size() * sizeof(Copy2D));
@@ -717,8 +789,10 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   EigOut eo;
   EighResult er;
   if (jac) eo = batched_eigh(c, Gd, goff, gn, traces);
-  else er = eigh_batched(c, Gd, goff, gn);
+  else er = eigh_named(c, Gd, goff, gn, "level-1 eigensolve");
   std::vector<std::vector<double>> evals = jac ? eo.evals : er.evals;
+  for (int q = 0; q < ns; ++q)
+    for (double &x : evals[q]) x = std::ldexp(x, 2 * dq[q]);
   c->count("svd_eigh_ms", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - te0).count());
   {
     int maxn = 0;
@@ -772,14 +846,14 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   // about n eps lambda_max absolutely, i.e. a singular value s only to a
   // relative (s_max / s)^2 eps, while the truncation (and its 1e-6 tie
   // window) needs the values near the cut to a relative error like a direct
-  // SVD's, eps s_max / s.  Eigenvalues above kRefine * lambda_max = 1e-7
-  // lambda_max are resolved to ~1e-9 relative.  When the cut lies below that
-  // in a sector, the eigenpairs of that sector below kRefine * lambda_max (the
-  // tail S) are recomputed from the Gram of the projected tail,
-  // W_S = E_S^T M (left) or M E_S (right): its rows carry the true small
-  // singular values up to eps s_max absolute, and its eigensolver's rounding
-  // now scales with the tail's own maximum.  Repeated while the cut still
-  // lies below kRefine times the refined tail's maximum.
+  // SVD's, eps s_max / s.  Eigenvalues above kAccept * lambda_max are
+  // resolved to ~1e-9 relative.  When the cut lies below that in a sector,
+  // the eigenpairs of that sector below the tail threshold (the tail S) are
+  // recomputed from the Gram of the projected tail, W_S = E_S^T M (left) or
+  // M E_S (right): its rows carry the true small singular values up to
+  // eps s_max absolute, and its eigensolver's rounding now scales with the
+  // tail's own maximum.  Repeated while the cut still lies below kAccept
+  // times the refined tail's maximum.
   std::vector<int> nsub(ns, 0);
   std::vector<int64_t> esoff(ns, 0);
   BufP Esub;
@@ -790,7 +864,15 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   BufP FLb, ILb;
   if (!jac) {
     const auto tr0 = std::chrono::steady_clock::now();
-    constexpr double kRefine = 1e-7;
+    // A cut below kAccept times the current problem's largest eigenvalue is
+    // not resolved to the tie window and triggers a round; the round's tail
+    // holds the eigenvalues below max(kTail * top, kMargin * cut^2).  The
+    // level-1 eigenvalues are accurate to ~0.2 eps * top absolutely (graded
+    // spectra of known values, n = 1000), so a tail boundary at 1e-10 top
+    // mixes the tail with the large directions by theta < 1e-6, a relative
+    // error theta^2 in the refined values; the tail's own maximum, ~1e-10 top,
+    // then resolves a cut at 1e-8 s_max in one round.
+    constexpr double kAccept = 1e-7, kTail = 1e-10, kMargin = 1e3;
     std::vector<double> lam_top(ns, 0.0);  // scale of each sector's current problem
     for (int q = 0; q < ns; ++q)
       for (double x : evals[q]) lam_top[q] = std::max(lam_top[q], x);
@@ -807,8 +889,8 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       const double cut2 = cut * cut;
       std::vector<int> rs, rn;
       for (int q = 0; q < ns; ++q) {
-        const double lim = kRefine * lam_top[q];
-        if (!(cut2 < lim)) continue;
+        if (!(cut2 < kAccept * lam_top[q])) continue;
+        const double lim = std::max(kTail * lam_top[q], kMargin * cut2);
         if (round > 0) {
           // the tail basis prepared by the previous round
           if (nsub[q] > 0) {
@@ -864,7 +946,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
           totF += (int64_t)kl * (side == kOrthLeft ? secs[q].n : secs[q].m);
           iLoff[q] = totI;
           totI += kl;
-          for (int j = 0; j < kl; ++j) is2.push_back(1.0 / evals[q][rn[i] + j]);
+          for (int j = 0; j < kl; ++j) is2.push_back(std::ldexp(1.0 / evals[q][rn[i] + j], 2 * dq[q]));
         }
         if (totE > 0) {
           auto ELb = c->alloc(totE * sizeof(double));
@@ -887,6 +969,27 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
                                           static_cast<double *>(FLb->p), fb, fp);
           else raw_gemm(c, kNormal, kNormal, Md, static_cast<double *>(ELb->p), static_cast<double *>(FLb->p), fb, fp);
           ILb = upload(c, is2.data(), is2.size() * sizeof(double));
+          // The large values themselves: |u_r^T M|^2 (|M v_r|^2), the Rayleigh
+          // quotient of the exact M M^T, whose error is second order in the
+          // eigenvector's (~theta^2 times the gap, 1e-14 relative), where the
+          // eigenvalue carries the absolute ~eps * top of G.
+          std::vector<ScaleD> nd;
+          for (int q = 0; q < ns; ++q) {
+            if (nL[q] == 0) continue;
+            if (side == kOrthLeft) nd.push_back(ScaleD{fLoff[q], iLoff[q], nL[q], (int32_t)secs[q].n, 0, 0});
+            else nd.push_back(ScaleD{fLoff[q], iLoff[q], (int32_t)secs[q].m, nL[q], 1, 0});
+          }
+          auto dnd = upload(c, nd.data(), nd.size() * sizeof(ScaleD));
+          auto bn = c->alloc(std::max<int64_t>(totI, 1) * sizeof(double));
+          sumsq_batch_kernel<<<dim3(64, (unsigned)nd.size()), 256, 0, c->stream>>>(
+              static_cast<const double *>(FLb->p), static_cast<double *>(bn->p), static_cast<const ScaleD *>(dnd->p));
+          std::vector<double> lv(totI);
+          MPS2_HIP(hipMemcpyAsync(lv.data(), bn->p, totI * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+          c->sync();
+          for (double x : lv)
+            if (!std::isfinite(x)) throw Error(kErrNumeric, "SVD refinement: non-finite large-direction values");
+          for (int q = 0; q < ns; ++q)
+            for (int j = 0; j < nL[q]; ++j) evals[q][gn[q] - nL[q] + j] = std::ldexp(lv[iLoff[q] + j], 2 * dq[q]);
         }
       }
       const double *Es = static_cast<const double *>(Esub->p);
@@ -967,9 +1070,12 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       }
       if (side == kOrthLeft) raw_gemm(c, kNormal, kTrans, W_, W_, G_, gb2, gp2);
       else raw_gemm(c, kTrans, kNormal, W_, W_, G_, gb2, gp2);
-      EighResult er2 = eigh_batched(c, G_, g2off, rn);
+      if (!std::isfinite(dev_maxabs(c, G_, gtot2)))
+        throw Error(kErrNumeric, "SVD refinement round " + std::to_string(round) + ": non-finite Gram of the projected tail");
+      EighResult er2 = eigh_named(c, G_, g2off, rn, "refinement round " + std::to_string(round));
       for (int i = 0; i < nr; ++i) {
         const int q = rs[i], r = rn[i];
+        for (double &x : er2.evals[i]) x = std::ldexp(x, 2 * dq[q]);
         for (int j = 0; j < r; ++j) evals[q][j] = er2.evals[i][j];
         lam_top[q] = 0.0;
         for (int j = 0; j < r; ++j) lam_top[q] = std::max(lam_top[q], er2.evals[i][j]);
@@ -993,9 +1099,9 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       bool more = false;
       for (int i = 0; i < nr; ++i) {
         const int q = rs[i];
-        const double lim = kRefine * lam_top[q];
+        const double lim = std::max(kTail * lam_top[q], kMargin * cut2n);
         int r = 0;
-        if (cut2n < lim)
+        if (cut2n < kAccept * lam_top[q])
           while (r < rn[i] && er2.evals[i][r] < lim) ++r;
         if (r >= 1 && er2.evals[i][r - 1] >= 1e-4 * cut2n) {
           r2[i] = r;
@@ -1098,7 +1204,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     ftot += kq * (side == kOrthLeft ? secs[q].n : secs[q].m);
     ioff[q] = itot;
     itot += kq;
-    for (int64_t k = 0; k < kq; ++k) inv.push_back(sv[q][k] > 0 ? 1.0 / sv[q][k] : 0.0);
+    for (int64_t k = 0; k < kq; ++k) inv.push_back(sv[q][k] > 0 ? std::ldexp(1.0 / sv[q][k], dq[q]) : 0.0);
   }
   auto Eb = c->alloc(std::max<int64_t>(etot, 1) * sizeof(double));
   auto Fb = c->alloc(std::max<int64_t>(ftot, 1) * sizeof(double));

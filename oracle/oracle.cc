@@ -651,7 +651,14 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
       for (int64_t i = 0; i < bm; ++i)
         for (int64_t j = 0; j < bn; ++j) M[(size_t)(ir->second + i) * s.n + ic->second + j] = kv.second[i * bn + j] * pre;
     }
+    // each sector additionally at its own scale 2^-e_q (exact): sectors of
+    // negligible weight would otherwise underflow in the sums of squares
+    double mq = 0.0;
+    for (double v : M) mq = std::max(mq, std::fabs(v));
+    const int eq = mq > 0.0 ? std::max(-1000, std::min(1000, std::ilogb(mq))) : 0;
+    for (double &v : M) v = std::ldexp(v, -eq);
     svd_jacobi(s.m, s.n, M.data(), Us[q], Ss[q], Vs[q]);
+    for (double &v : Ss[q]) v = std::ldexp(v, eq);
   }
   // Below 1e-11 s_max (s_max over all sectors; weights below 1e-22) FP64
   // SVD values carry no reproducible order (a direct SVD's absolute error is
