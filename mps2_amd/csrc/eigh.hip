@@ -1,15 +1,13 @@
 // Batched symmetric eigen-decomposition of the sector Gram matrices of the
 // truncated SVD, on the device, FP64:
 //
-//  1. Householder tridiagonalisation, blocked by panels of kNB columns, lower
-//     form (the dsytrd/dlatrd scheme: A = Q T Q^T, Q = H_0 H_1 ... H_{n-2},
-//     H_c = I - tau_c v_c v_c^T, v_c[c+1] = 1).  Within a panel the matrix
-//     stays as it was at the panel start; per column, many workgroups form
-//     y = A_trail v (one symv), then one workgroup per sector turns y into
-//     w = tau (y - V W^T v - W V^T v) - tau/2 (w^T v) v and builds the next
-//     reflector from row c+1 corrected by the panel's rank-2j term.  At a
-//     panel end the trailing square takes the rank-2kNB update
-//     A -= V W^T + W V^T on FP64 MFMA.
+//  1. Householder tridiagonalisation, lower form (A = Q T Q^T,
+//     Q = H_0 H_1 ... H_{n-2}, H_c = I - tau_c v_c v_c^T, v_c[c+1] = 1), one
+//     column per step for every sector at once: one workgroup per sector
+//     finishes w_{c-1} = tau y - tau^2/2 (y^T v) v, forms row c of the updated
+//     matrix and builds the reflector v_c; then many workgroups apply the
+//     rank-2 update of column c-1 to the trailing square and form
+//     y = A_trail v_c in the same pass.
 //  2. Tridiagonal divide and conquer (Cuppen), bottom-up from 1x1 blocks,
 //     every merge of one level in the same launches.  A merge of adjacent
 //     blocks diag(T1', T2') + rho u u^T (u = e_last + sign(beta) e_first,
@@ -20,15 +18,14 @@
 //     z from the roots (orthogonal vectors), and the product of the
 //     sub-block eigenvectors with the merge's vectors on MFMA.
 //  3. Back-transformation of selected eigenvectors: E <- Q E panel by panel
-//     in reverse, Q_p = I - V_p T_p V_p^T (compact WY), two MFMA GEMMs per
-//     panel.
+//     in reverse, Q_p = I - V_p T_p V_p^T (compact WY over kNB = 128
+//     reflectors), two MFMA GEMMs per panel.
 //
 // Layouts (per sector of size n; offsets in the sector descriptors):
 //   A      n x n row-major (the Gram matrix, overwritten),
 //   VH, WT n x n column-major (column c = v_c; WT = V_p T_p per panel),
 //   Z      n x n column-major eigenvectors of T (block-diagonal during the
 //          merges), lam ascending eigenvalues,
-//   V, W   n x kNB column-major panel buffers.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +36,7 @@
 namespace mps2 {
 namespace {
 
-constexpr int kNB = 32;   // panel width
+constexpr int kNB = 128;  // compact-WY panel width of the back-transformation
 constexpr int kT = 256;   // threads of the per-sector kernels
 constexpr int kRowsPerWave = 2;
 constexpr int kSymvRows = 4 * kRowsPerWave;  // 4 wavefronts per workgroup
@@ -54,7 +51,7 @@ __device__ __forceinline__ double okey(double v) { return v == v ? v : INFINITY;
 struct ESec {
   int64_t nn;   // offset in n*n buffers
   int64_t nv;   // offset in n-vectors
-  int64_t np;   // offset in n*kNB panel buffers
+  int64_t np;   // unused
   int64_t tb;   // offset in T blocks (npanels * kNB * kNB)
   int n;
   int pad;
@@ -228,32 +225,34 @@ __global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restr
 
 // T block of panel p (compact WY, forward columnwise) from the Gram of its
 // reflectors G = V_p^T V_p:  T[j][j] = tau_j,  T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j].
-// One wavefront per (sector, panel).
+// One workgroup of kNB threads per (sector, panel); T is built in LDS
+// (column-major, T[col * kNB + row]) and then stored.
 struct PanelRef {
   int q, p;
   int64_t goff;  // G block (jn x jn row-major) in the panel-Gram buffer
 };
-__global__ __launch_bounds__(64) void trd_tblock_kernel(const ESec *__restrict__ secs, const PanelRef *__restrict__ pr,
-                                                        const double *__restrict__ Gp, const double *__restrict__ tauv,
-                                                        double *Tb) {
+__global__ __launch_bounds__(kNB) void trd_tblock_kernel(const ESec *__restrict__ secs, const PanelRef *__restrict__ pr,
+                                                         const double *__restrict__ Gp, const double *__restrict__ tauv,
+                                                         double *Tb) {
+  __shared__ double Ts[kNB * kNB];
   const PanelRef P = pr[blockIdx.x];
   const ESec S = secs[P.q];
   const int n = S.n, lane = threadIdx.x;
   const int k = P.p * kNB, jn = min(kNB, n - k);
   double *T = Tb + S.tb + (int64_t)P.p * kNB * kNB;
   const double *G = Gp + P.goff;
-  for (int i = lane; i < kNB * kNB; i += 64) T[i] = 0.0;
+  for (int i = lane; i < kNB * kNB; i += kNB) Ts[i] = 0.0;
   __syncthreads();
   for (int j = 0; j < jn; ++j) {
     const double tj = (k + j < n - 1) ? tauv[S.nv + k + j] : 0.0;
     double s = 0;
     if (lane < j)
-      for (int l = lane; l < j; ++l) s += T[l * kNB + lane] * G[l * jn + j];
-    __syncthreads();
-    if (lane < j) T[j * kNB + lane] = -tj * s;
-    if (lane == j) T[j * kNB + j] = tj;
+      for (int l = lane; l < j; ++l) s += Ts[l * kNB + lane] * G[l * jn + j];
+    if (lane < j) Ts[j * kNB + lane] = -tj * s;
+    if (lane == j) Ts[j * kNB + j] = tj;
     __syncthreads();
   }
+  for (int i = lane; i < kNB * kNB; i += kNB) T[i] = Ts[i];
 }
 
 // WT[:, k + j] = sum_{l <= j} VH[:, k + l] T_p[l][j]  (column-major, per panel).
@@ -742,7 +741,8 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
         prs.push_back(PanelRef{q, p, goff});
         gb.push_back(BlkD{goff, jn, jn, idx, idx + 1});
         gp.push_back(PairD{st->secs[q].nn + (int64_t)k * ns[q], st->secs[q].nn + (int64_t)k * ns[q], ns[q], 0});
-        gt.push_back(TileD{idx, 0, 0, 0});
+        for (int m0 = 0; m0 < jn; m0 += 64)
+          for (int n0 = 0; n0 < jn; n0 += 64) gt.push_back(TileD{idx, m0, n0, 0});
         goff += (int64_t)jn * jn;
       }
     auto dpr = upload_(c, prs.data(), prs.size() * sizeof(PanelRef));
@@ -752,7 +752,7 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
     auto bg = c->alloc(std::max<int64_t>(goff, 1) * sizeof(double));
     launch_grouped_gemm<double>(c, 0, 1, VH, VH, dp<double>(bg), dp<TileD>(dgt), (int)gt.size(), dp<BlkD>(dgb),
                                 dp<PairD>(dgp), 64, 64);
-    trd_tblock_kernel<<<(int)prs.size(), 64, 0, c->stream>>>(SD, dp<PanelRef>(dpr), dp<double>(bg), tauv, Tb);
+    trd_tblock_kernel<<<(int)prs.size(), kNB, 0, c->stream>>>(SD, dp<PanelRef>(dpr), dp<double>(bg), tauv, Tb);
     trd_wt_kernel<<<nblk(nn), 256, 0, c->stream>>>(SD, nsec, VH, Tb, dp<double>(st->WT), nn);
     c->sync();
   }
@@ -894,7 +894,8 @@ void eigh_vectors_all(Ctx *c, const EighResult &r, const std::vector<std::vector
       const int jn = std::min(kNB, n - k), bi = (int)blks.size();
       blks.push_back(BlkD{xoff[q], jn, kq, (int32_t)pairs.size(), (int32_t)pairs.size() + 1});
       pairs.push_back(PairD{S.nn + (int64_t)k * n, eoff[q], n, 0});
-      for (int n0 = 0; n0 < kq; n0 += 64) tiles.push_back(TileD{bi, 0, n0, 0});
+      for (int m0 = 0; m0 < jn; m0 += 64)
+        for (int n0 = 0; n0 < kq; n0 += 64) tiles.push_back(TileD{bi, m0, n0, 0});
     }
     t1e[p] = (int)tiles.size();
     t2b[p] = (int)tiles.size();

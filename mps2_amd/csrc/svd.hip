@@ -435,6 +435,25 @@ __global__ __launch_bounds__(256) void sumsq_batch_kernel(const double *__restri
   }
 }
 
+// Kept eigenvectors of a refined sector from the level-1 vectors computed
+// for its refinement: E[e_off + i*kq + t] = map[t] >= 0 ? ES[i][map[t]]
+// (tail basis, n x r) : EL[i][-1 - map[t]] (large directions, n x nl).
+struct KeepD {
+  int64_t e_off, s_off, l_off, map_off;
+  int32_t n, kq, r, nl;
+};
+__global__ void gather_kept_kernel(const double *__restrict__ ES, const double *__restrict__ EL,
+                                   const int32_t *__restrict__ map, const KeepD *__restrict__ desc, double *E) {
+  const KeepD d = desc[blockIdx.y];
+  const int64_t tot = (int64_t)d.n * d.kq;
+  for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < tot; g += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = g / d.kq;
+    const int t = (int)(g - i * d.kq);
+    const int m = map[d.map_off + t];
+    E[d.e_off + g] = m >= 0 ? ES[d.s_off + i * d.r + m] : EL[d.l_off + i * d.nl + (-1 - m)];
+  }
+}
+
 // Transposed block copies, one descriptor per blockIdx.x:
 // dst[dst_off + c * ldd + r] = src[src_off + r * lds + c], r < rows, c < cols.
 __global__ void copy2d_t_kernel(const double *src, double *dst, const Copy2D *desc) {
@@ -862,6 +881,11 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
   std::vector<int> nL(ns, 0);
   std::vector<int64_t> fLoff(ns, 0), iLoff(ns, 0);
   BufP FLb, ILb;
+  // level-1 vectors of the sectors refined in round 0: tail (r0 columns) and
+  // large directions (nL columns), reused for the kept vectors
+  BufP ES0, EL0;
+  std::vector<int> r0(ns, -1);
+  std::vector<int64_t> es0off(ns, 0), el0off(ns, 0);
   if (!jac) {
     const auto tr0 = std::chrono::steady_clock::now();
     // A cut below kAccept times the current problem's largest eigenvalue is
@@ -924,6 +948,9 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         }
         Esub = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
         eigh_vectors_all(c, er, cols, static_cast<double *>(Esub->p), esoff);
+        ES0 = Esub;
+        es0off = esoff;
+        for (int i = 0; i < nr; ++i) r0[rs[i]] = rn[i];
         // The computed tail basis leans into each large direction u_r by
         // theta ~ eps lambda_max / lambda_r, which puts rows theta s_r v_r^T
         // into W_S and lifts true tail values to about eps s_max^2 / s_r
@@ -951,6 +978,8 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         if (totE > 0) {
           auto ELb = c->alloc(totE * sizeof(double));
           eigh_vectors_all(c, er, colsL, static_cast<double *>(ELb->p), eLoff);
+          EL0 = ELb;
+          el0off = eLoff;
           FLb = c->alloc(totF * sizeof(double));
           std::vector<BlkD> fb;
           std::vector<PairD> fp;
@@ -1220,7 +1249,27 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
           static_cast<int32_t *>(keep.back()->p), (int)kq, Ed + eoff[q]);
     }
   } else {
-    eigh_vectors_all(c, er, kcols, Ed, eoff);
+    // refined sectors: copy from the level-1 tail / large-direction vectors;
+    // the others are back-transformed
+    std::vector<std::vector<int32_t>> bcols(kcols);
+    std::vector<KeepD> kd;
+    std::vector<int32_t> kmap;
+    for (int q = 0; q < ns; ++q) {
+      if (secpos[q] < 0 || r0[q] < 0) continue;
+      const int32_t kq = (int32_t)out.kept[q];
+      kd.push_back(KeepD{eoff[q], es0off[q], el0off[q], (int64_t)kmap.size(), gn[q], kq, r0[q], nL[q]});
+      for (int32_t idx : kcols[q]) kmap.push_back(idx < r0[q] ? idx : -1 - (idx - r0[q]));
+      bcols[q].clear();
+    }
+    eigh_vectors_all(c, er, bcols, Ed, eoff);
+    if (!kd.empty()) {
+      keep.push_back(upload(c, kmap.data(), kmap.size() * sizeof(int32_t)));
+      const int32_t *dmap = static_cast<const int32_t *>(keep.back()->p);
+      keep.push_back(upload(c, kd.data(), kd.size() * sizeof(KeepD)));
+      gather_kept_kernel<<<dim3(256, (unsigned)kd.size()), 256, 0, c->stream>>>(
+          ES0 ? static_cast<const double *>(ES0->p) : nullptr, EL0 ? static_cast<const double *>(EL0->p) : nullptr,
+          dmap, static_cast<const KeepD *>(keep.back()->p), Ed);
+    }
   }
   {
     std::vector<BlkD> fb;
