@@ -10,6 +10,11 @@
 //
 // Matrix core: v_mfma_f64_16x16x4f64 (FP64) or v_mfma_f32_16x16x4f32 (FP32),
 // 4 wavefronts of 64 lanes in a 2x2 arrangement, LDS-staged BK=16 slabs.
+// FP32: the instruction's k index is the lane group g = lane / 16; within a
+// slab, MFMA step s feeds k = 4g + s, so each lane's four k values of a
+// fragment are contiguous in k-major LDS rows and come in one 16-byte read.
+// FP64 (one wavefront per SIMD at 128x128) keeps m/n-major rows and one
+// scalar read per fragment and step, which measured faster there.
 #include <hip/hip_runtime.h>
 
 #include "runtime.h"
@@ -46,13 +51,18 @@ constexpr int kThreads = 256;
 constexpr int kBK = 16;
 
 // Per-thread share of one BK-deep slab of A (BM x BK) and B (BK x BN),
-// loaded from global memory into registers, then stored to LDS as
-// As[k][m], Bs[k][n].  Out-of-range elements read as zero.
+// loaded from global memory into registers, then stored to LDS: FP32 as
+// As[m][k], Bs[n][k] with row stride SK (k contiguous, rows 16-byte
+// aligned), FP64 as As[k][m], Bs[k][n] with strides PA, PB.  Out-of-range
+// elements read as zero.
 template <typename T, int BM, int BN, bool TA, bool TB>
 struct Slab {
+  static constexpr bool kQ = sizeof(T) == 4;
   static constexpr int NA = BM * kBK / kThreads;
   static constexpr int NB = BN * kBK / kThreads;
+  static constexpr int SK = kBK + 4;
   static constexpr int PA = BM + 1, PB = BN + 1;
+  static constexpr int LA = kQ ? BM * SK : kBK * PA, LB = kQ ? BN * SK : kBK * PB;  // LDS elements
   T ra[NA], rb[NB];
 
   __device__ __forceinline__ void load(const T *__restrict__ Ab, const T *__restrict__ Bb, int M, int N, int K,
@@ -93,29 +103,42 @@ struct Slab {
     }
   }
 
+  // LDS index of A(m, k) and B(k, n) within a slab
+  static __device__ __forceinline__ int ia(int m, int k) { return kQ ? m * SK + k : k * PA + m; }
+  static __device__ __forceinline__ int ib(int k, int n) { return kQ ? n * SK + k : k * PB + n; }
+
   __device__ __forceinline__ void store(T *As, T *Bs, int tid) const {
     if (!TA) {
       constexpr int RSTEP = kThreads / kBK;
       const int kk = tid % kBK;
 #pragma unroll
-      for (int i = 0; i < NA; ++i) As[kk * PA + tid / kBK + i * RSTEP] = ra[i];
+      for (int i = 0; i < NA; ++i) As[ia(tid / kBK + i * RSTEP, kk)] = ra[i];
     } else {
       constexpr int KSTEP = kThreads / BM;
       const int mm = tid % BM;
 #pragma unroll
-      for (int i = 0; i < NA; ++i) As[(tid / BM + i * KSTEP) * PA + mm] = ra[i];
+      for (int i = 0; i < NA; ++i) As[ia(mm, tid / BM + i * KSTEP)] = ra[i];
     }
     if (!TB) {
       constexpr int KSTEP = kThreads / BN;
       const int nn = tid % BN;
 #pragma unroll
-      for (int i = 0; i < NB; ++i) Bs[(tid / BN + i * KSTEP) * PB + nn] = rb[i];
+      for (int i = 0; i < NB; ++i) Bs[ib(tid / BN + i * KSTEP, nn)] = rb[i];
     } else {
       constexpr int RSTEP = kThreads / kBK;
       const int kk = tid % kBK;
 #pragma unroll
-      for (int i = 0; i < NB; ++i) Bs[kk * PB + tid / kBK + i * RSTEP] = rb[i];
+      for (int i = 0; i < NB; ++i) Bs[ib(kk, tid / kBK + i * RSTEP)] = rb[i];
     }
+  }
+};
+
+// Four consecutive k values of one FP32 LDS row (16-byte aligned).
+struct Quad {
+  float v[4];
+  __device__ __forceinline__ void load(const float *p) {
+    const float4 q = *reinterpret_cast<const float4 *>(p);
+    v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
   }
 };
 
@@ -133,9 +156,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   using SlabT = Slab<T, BM, BN, TA, TB>;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  constexpr int PA = SlabT::PA, PB = SlabT::PB;
-  __shared__ T As[2][kBK * PA];
-  __shared__ T Bs[2][kBK * PB];
+  __shared__ __align__(16) T As[2][SlabT::LA];
+  __shared__ __align__(16) T Bs[2][SlabT::LB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -174,20 +196,38 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
     }
     const bool has_next = np < blk.pe;
     if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
-    const T *as = As[cur];
-    const T *bs = Bs[cur];
+    if constexpr (SlabT::kQ) {
+      constexpr int SK = SlabT::SK;
+      const float *as = As[cur] + (wm * WM + (lane & 15)) * SK + 4 * (lane >> 4);
+      const float *bs = Bs[cur] + (wn * WN + (lane & 15)) * SK + 4 * (lane >> 4);
+      Quad af[FM], bf[FN];
 #pragma unroll
-    for (int ks = 0; ks < kBK; ks += 4) {
-      const int kr = ks + (lane >> 4);
-      T af[FM], bf[FN];
+      for (int i = 0; i < FM; ++i) af[i].load(as + i * 16 * SK);
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = as[kr * PA + wm * WM + i * 16 + (lane & 15)];
+      for (int j = 0; j < FN; ++j) bf[j].load(bs + j * 16 * SK);
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j] = bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
+      for (int st = 0; st < 4; ++st)
 #pragma unroll
-      for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i].v[st], bf[j].v[st], acc[i][j]);
+    } else {
+      constexpr int PA = SlabT::PA, PB = SlabT::PB;
+      const T *as = As[cur];
+      const T *bs = Bs[cur];
+#pragma unroll
+      for (int ks = 0; ks < kBK; ks += 4) {
+        const int kr = ks + (lane >> 4);
+        T af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = as[kr * PA + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = bs[kr * PB + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+      }
     }
     if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
     __syncthreads();
