@@ -96,8 +96,78 @@ __device__ void row_step(const ESec &S, int c, const double *__restrict__ A, dou
                          const double *ybuf, double *abuf, double *dv, double *ev, double *tauv, double *red,
                          double *sh) {
   const int n = S.n, tid = threadIdx.x, nt = blockDim.x;
-  double *a = abuf + S.nv;
   const double *arow = A + S.nn + (int64_t)c * n;
+  constexpr int P = 4;
+  if (n - c <= P * nt) {
+    // Rows r = c + tid + k nt (k < P) stay in this thread's registers: their
+    // loads are issued before the first reduction, and the reflector is
+    // written from the same registers (no round trip through memory).
+    double ar[P], yr[P], vr[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int r = c + tid + k * nt;
+      ar[k] = r < n ? arow[r] : 0.0;
+    }
+    double ss = 0;
+    if (c > 0) {
+      const double tau = tauv[S.nv + c - 1];
+      const double *vp = VH + S.nn + (int64_t)(c - 1) * n;
+      const double *y = ybuf + S.nv;
+      double *w = wbuf + S.nv;
+      const double vc = vp[c], yc = y[c];
+      double part = 0;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int r = c + tid + k * nt;
+        yr[k] = r < n ? y[r] : 0.0;
+        vr[k] = r < n ? vp[r] : 0.0;
+        part += yr[k] * vr[k];
+      }
+      const double kcoef = -0.5 * tau * tau * block_sum(part, red);
+      const double wc = tau * yc + kcoef * vc;
+#pragma unroll
+      for (int k = 0; k < P; ++k) {
+        const int r = c + tid + k * nt;
+        if (r < n) {
+          const double x = tau * yr[k] + kcoef * vr[k];
+          w[r] = x;
+          ar[k] = ar[k] - vc * x - wc * vr[k];
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int r = c + tid + k * nt;
+      if (r < n && r >= c + 2) ss += ar[k] * ar[k];
+    }
+    if (tid == 0) dv[S.nv + c] = ar[0];  // row c is thread 0's first
+    if (c >= n - 1) return;
+    if (tid == 1) sh[0] = ar[0];  // a[c+1], read by all after the barrier below
+    const double xn = sqrt(block_sum(ss, red));
+    const double alpha = sh[0];
+    double beta, tau, sc;
+    if (xn == 0.0) {
+      beta = alpha;
+      tau = 0.0;
+      sc = 0.0;
+    } else {
+      beta = -copysign(hypot(alpha, xn), alpha);
+      tau = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    double *vh = VH + S.nn + (int64_t)c * n;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const int r = c + tid + k * nt;
+      if (r > c && r < n) vh[r] = r == c + 1 ? 1.0 : ar[k] * sc;
+    }
+    if (tid == 0) {
+      ev[S.nv + c] = beta;
+      tauv[S.nv + c] = tau;
+    }
+    return;
+  }
+  double *a = abuf + S.nv;
   // One pass forms w_{c-1}, the corrected row c and the squared norm of its
   // tail; w_{c-1}[c] is recomputed by every thread instead of broadcast.
   double ss = 0;
