@@ -38,7 +38,7 @@ namespace {
 
 constexpr int kNB = 128;  // compact-WY panel width of the back-transformation
 constexpr int kT = 256;   // threads of the per-sector kernels
-constexpr int kRowsPerWave = 2;
+constexpr int kRowsPerWave = 1;
 constexpr int kSymvRows = 4 * kRowsPerWave;  // 4 wavefronts per workgroup
 constexpr double kEps = 2.220446049250313e-16;
 
@@ -267,8 +267,35 @@ __global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restr
     wr[i] = (ok[i] && c > 0) ? w[r] : 0.0;
     acc[i] = 0.0;
   }
+  // Columns in chunks of four 64-wide strides: the loads of a chunk are all
+  // issued before its arithmetic (memory-level parallelism for a wave that
+  // is otherwise alone on its SIMD); each lane still sums its columns in
+  // increasing order.  Rows that are not ok point at row c + 1 (< n), so the
+  // chunk loads stay in bounds; only ok rows are written.
+  constexpr int U = 4;
+  int col = c + 1 + lane;
   if (c > 0) {
-    for (int col = c + 1 + lane; col < n; col += 64) {
+    for (; col + (U - 1) * 64 < n; col += U * 64) {
+      double x[U], vq[U], wq[U], t[U][kRowsPerWave];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        x[u] = v[col + 64 * u];
+        vq[u] = vp[col + 64 * u];
+        wq[u] = w[col + 64 * u];
+#pragma unroll
+        for (int i = 0; i < kRowsPerWave; ++i) t[u][i] = ar[i][col + 64 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < kRowsPerWave; ++i)
+          if (ok[i]) {
+            const double tt = t[u][i] - vr[i] * wq[u] - wr[i] * vq[u];
+            ar[i][col + 64 * u] = tt;
+            acc[i] += tt * x[u];
+          }
+    }
+    for (; col < n; col += 64) {
       const double x = v[col], vq = vp[col], wq = w[col];
 #pragma unroll
       for (int i = 0; i < kRowsPerWave; ++i)
@@ -279,7 +306,21 @@ __global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restr
         }
     }
   } else {
-    for (int col = c + 1 + lane; col < n; col += 64) {
+    for (; col + (U - 1) * 64 < n; col += U * 64) {
+      double x[U], t[U][kRowsPerWave];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        x[u] = v[col + 64 * u];
+#pragma unroll
+        for (int i = 0; i < kRowsPerWave; ++i) t[u][i] = ar[i][col + 64 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int i = 0; i < kRowsPerWave; ++i)
+          if (ok[i]) acc[i] += t[u][i] * x[u];
+    }
+    for (; col < n; col += 64) {
       const double x = v[col];
 #pragma unroll
       for (int i = 0; i < kRowsPerWave; ++i)
