@@ -185,6 +185,8 @@ struct DevPlan {
   BufP tiles, blks, pairs;
   int ntiles = 0;
   int tile_m = 64, tile_n = 64;
+  BufP tiles64;  // the same output tiled 64x64 (FP64 launches)
+  int ntiles64 = 0;
   BufP perm_a, perm_b, perm_a_start, perm_b_start;   // permutation descriptors
   int nperm_a = 0, nperm_b = 0;
   // tiles of one owner's output blocks, by (owners, rank): tiles, count, flops
