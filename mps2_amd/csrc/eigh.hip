@@ -97,7 +97,7 @@ __device__ void row_step(const ESec &S, int c, const double *__restrict__ A, dou
                          double *sh) {
   const int n = S.n, tid = threadIdx.x, nt = blockDim.x;
   const double *arow = A + S.nn + (int64_t)c * n;
-  constexpr int P = 4;
+  constexpr int P = 8;
   if (n - c <= P * nt) {
     // Rows r = c + tid + k nt (k < P) stay in this thread's registers: their
     // loads are issued before the first reduction, and the reflector is
@@ -217,7 +217,7 @@ __device__ void row_step(const ESec &S, int c, const double *__restrict__ A, dou
 }
 
 // Column step c of every sector (one workgroup of kTR threads per sector).
-constexpr int kTR = 1024;
+constexpr int kTR = 512;
 __global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ secs, int c,
                                                       const double *__restrict__ A, double *VH, double *wbuf,
                                                       const double *__restrict__ ybuf, double *abuf, double *dv,
