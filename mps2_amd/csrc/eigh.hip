@@ -436,14 +436,23 @@ __global__ void dc_init_kernel(const ESec *__restrict__ secs, int nsec, DcBufs B
 }
 
 // One workgroup per merge: z, D in merged sorted order, deflation (one
-// thread, sequential over the sorted order), compaction.
+// thread, sequential over the sorted order), compaction.  With lds_cap > 0
+// (every merge of the launch has k <= lds_cap) D, z and the flags live in
+// dynamic LDS during the sequential scan and D, z are written back after it;
+// otherwise they are used in place in global memory.
+constexpr int kDcLdsBytes = 128 * 1024;
 __global__ __launch_bounds__(kT) void dc_setup_kernel(const ESec *__restrict__ secs, const DcMerge *__restrict__ ms,
-                                                      DcBufs B) {
+                                                      DcBufs B, int lds_cap) {
   __shared__ double red[kT / 64];
+  extern __shared__ double dc_lds[];
   const DcMerge M = ms[blockIdx.x];
   const ESec S = secs[M.q];
   const int n = S.n, s = M.s, n1 = M.n1, n2 = M.n2, k = n1 + n2, tid = threadIdx.x;
   const int64_t vo = S.nv + s;
+  const bool in_lds = lds_cap > 0;
+  double *Dp = in_lds ? dc_lds : B.Dp + vo;
+  double *zp = in_lds ? dc_lds + lds_cap : B.zp + vo;
+  int *flag = in_lds ? reinterpret_cast<int *>(dc_lds + 2 * (int64_t)lds_cap) : B.flag + vo;
   const double beta = B.ev[S.nv + s + n1 - 1];
   const double sg = beta >= 0 ? 1.0 : -1.0;
   const double *l1 = B.lam + vo, *l2 = B.lam + vo + n1;
@@ -473,8 +482,8 @@ __global__ __launch_bounds__(kT) void dc_setup_kernel(const ESec *__restrict__ s
       }
       pos = t + lo;
     }
-    B.Dp[vo + pos] = D;
-    B.zp[vo + pos] = z;
+    Dp[pos] = D;
+    zp[pos] = z;
     B.perm[vo + pos] = i;
     zz += z * z;
     dm = fmax(dm, fabs(D));
@@ -486,8 +495,6 @@ __global__ __launch_bounds__(kT) void dc_setup_kernel(const ESec *__restrict__ s
   const double tol = 8.0 * kEps * fmax(dmax, rho);
   __syncthreads();
   if (tid == 0) {
-    double *Dp = B.Dp + vo, *zp = B.zp + vo;
-    int *flag = B.flag + vo;
     int pj = -1, nrot = 0;
     for (int i = 0; i < k; ++i) {
       const double zi = zp[i] / zn;
@@ -538,6 +545,13 @@ __global__ __launch_bounds__(kT) void dc_setup_kernel(const ESec *__restrict__ s
     B.info[blockIdx.x] = inf;
     B.gblk[blockIdx.x].M = kk;
     B.gpair[blockIdx.x].K = kk;
+  }
+  if (in_lds) {
+    __syncthreads();
+    for (int i = tid; i < k; i += kT) {
+      B.Dp[vo + i] = Dp[i];
+      B.zp[vo + i] = zp[i];
+    }
   }
 }
 
@@ -940,7 +954,13 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
     B.gblk = dp<BlkD>(dgb);
     B.gpair = dp<PairD>(dgp);
     const DcMerge *MS = dp<DcMerge>(dms);
-    dc_setup_kernel<<<nm, kT, 0, c->stream>>>(SD, MS, B);
+    // D, z, flags of the largest merge in LDS when they fit (20 bytes per entry)
+    const int lds_cap = (int64_t)kmax * 20 <= kDcLdsBytes ? kmax : 0;
+    static const bool lds_ok = hipFuncSetAttribute((const void *)dc_setup_kernel,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                   kDcLdsBytes) == hipSuccess;
+    const int cap = lds_ok ? lds_cap : 0;
+    dc_setup_kernel<<<nm, kT, (size_t)cap * 20, c->stream>>>(SD, MS, B, cap);
     const dim3 g256(nblk(kmax), nm), g4(nblk(kmax, 4), nm);
     dc_rotate_kernel<<<g256, 256, 0, c->stream>>>(SD, MS, B);
     dc_roots_kernel<<<g4, 256, 0, c->stream>>>(SD, MS, B);
