@@ -9,6 +9,7 @@
 
 #include "gqmps2/framework.h"
 #include "gqmps2/finite_mps.h"
+#include "gqmps2/finite_mps_measu.h"
 #include "gqmps2/mpogen.h"
 #include "gqmps2/algorithm.h"
 

@@ -88,6 +88,22 @@ int gqmps2_phase_times(gqmps2_model m, double *out, int64_t n);
 /* Console output of the host layer on (1, the reference's behaviour) or off. */
 int gqmps2_set_verbose(int verbose);
 
+/* MeasureOneSiteOp (finite_mps_measu.h:150-199): each of the nops operators
+ * at every site, the MPS centred there; avgs[j * N + i] = <op_j at site i>.
+ * Writes basenames[j] + ".json" (DumpMeasuRes, finite_mps_measu.h:556-582).
+ * Leaves the MPS centred at site N-1. */
+int gqmps2_measure_one_site(gqmps2_model m, int nops, const int32_t *op_ids, const char *const *basenames,
+                            double *avgs);
+/* MeasureMultiSiteOp (finite_mps_measu.h:290-361): event e has nphys[e] >= 2
+ * physical operators phys_op_ids[] at ascending sites[] (both back to back
+ * over the events) and nphys[e] - 1 entries of inst_op_ids[], the insertion
+ * operator repeated over each gap between consecutive physical operators
+ * (-1: the site's identity).  avgs[e] = <event e>; writes basename + ".json".
+ * MeasureTwoSiteOp is the nphys == 2 case. */
+int gqmps2_measure_multi_site(gqmps2_model m, int64_t nevents, const int32_t *nphys, const int64_t *sites,
+                              const int32_t *phys_op_ids, const int32_t *inst_op_ids, const char *basename,
+                              double *avgs);
+
 /* Engine context used by this library (for mps2_ctx_stats). */
 mps2_ctx gqmps2_engine_ctx(void);
 /* Message of this thread's last gqmps2_* failure. */

@@ -44,6 +44,8 @@ SIGNATURES = {
     "gqmps2_bench_bond": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int]),
     "gqmps2_phase_times": (C.c_int, [_vp, _dp, C.c_int64]),
     "gqmps2_set_verbose": (C.c_int, [C.c_int]),
+    "gqmps2_measure_one_site": (C.c_int, [_vp, C.c_int, _i32p, C.POINTER(C.c_char_p), _dp]),
+    "gqmps2_measure_multi_site": (C.c_int, [_vp, C.c_int64, _i32p, _i64p, _i32p, _i32p, C.c_char_p, _dp]),
     "gqmps2_engine_ctx": (_vp, []),
     "gqmps2_last_error": (C.c_char_p, []),
 }
@@ -184,6 +186,38 @@ class Model:
         arr = (_vp * 5)()
         check(lib().gqmps2_bench_tensors(self.h, arr))
         return [engine.DTen(_vp(arr[k]), None) if arr[k] else None for k in range(5)]
+
+    def measure_one_site(self, ops, basenames):
+        """MeasureOneSiteOp with a list of dense operators (one result file
+        each); returns an array [len(ops), N] of <op at site i>."""
+        ids = np.asarray([self.op(0, o) for o in ops], dtype=np.int32)
+        names = (C.c_char_p * len(ops))(*[str(b).encode() for b in basenames])
+        out = np.zeros((len(ops), self.N))
+        check(lib().gqmps2_measure_one_site(self.h, len(ops), ids.ctypes.data_as(_i32p), names,
+                                            out.ctypes.data_as(_dp)))
+        return out
+
+    def measure_multi_site(self, events, basename):
+        """MeasureMultiSiteOp.  events: list of (phys_ops, sites, inst_ops)
+        with dense matrices; inst_ops has one entry per gap (None = the
+        site's identity).  Returns the array of averages."""
+        nphys, st, pid, iid = [], [], [], []
+        for phys, sites, inst in events:
+            assert len(phys) == len(sites) >= 2 and len(inst) == len(sites) - 1
+            nphys.append(len(sites))
+            st.extend(sites)
+            pid.extend(self.op(s, o) for o, s in zip(phys, sites))
+            iid.extend(-1 if o is None else self.op(min(sites[k] + 1, self.N - 1), o) for k, o in enumerate(inst))
+        nphys = np.asarray(nphys, dtype=np.int32)
+        st = np.asarray(st, dtype=np.int64)
+        pid = np.asarray(pid, dtype=np.int32)
+        iid = np.asarray(iid if iid else [0], dtype=np.int32)
+        out = np.zeros(len(events))
+        check(lib().gqmps2_measure_multi_site(self.h, len(events), nphys.ctypes.data_as(_i32p),
+                                              st.ctypes.data_as(_i64p), pid.ctypes.data_as(_i32p),
+                                              iid.ctypes.data_as(_i32p), str(basename).encode(),
+                                              out.ctypes.data_as(_dp)))
+        return out
 
     def dmrg(self, sweeps, dmin, dmax, trunc_err, lanczos_error, lanczos_max_iter=200,
              precision=engine.PREC_FP64, dmax_schedule=None, verbose=False):

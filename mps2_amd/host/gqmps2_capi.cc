@@ -51,6 +51,9 @@ struct ModelBase {
                       int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te,
                       double *ut, double *st) = 0;
   virtual void bench_bond(int64_t site, int warmup, int reps) = 0;
+  virtual void measure_one_site(int nops, const int32_t *op_ids, const char *const *basenames, double *avgs) = 0;
+  virtual void measure_multi_site(int64_t nevents, const int32_t *nphys, const int64_t *sites, const int32_t *phys_ids,
+                                  const int32_t *inst_ids, const char *basename, double *avgs) = 0;
 };
 
 template <typename QNT>
@@ -278,6 +281,54 @@ struct Model : ModelBase {
     mps2_ten src[5] = {H[0]->Handle(), H[1]->Handle(), H[2]->Handle(), H[3]->Handle(), init.Handle()};
     for (int k = 0; k < 5; ++k) gqten::CheckMps2(mps2_ten_copy(src[k], &bench_tens[k]), "bench copy");
     bench_done = true;
+  }
+
+  void require_mps() const {
+    for (size_t i = 0; i < N; ++i)
+      if (!(*mps)(i)) throw std::runtime_error("MPS not initialised");
+  }
+
+  void measure_one_site(int nops, const int32_t *op_ids, const char *const *basenames, double *avgs) override {
+    require_mps();
+    if (nops < 1) throw std::invalid_argument("measure_one_site: no operator");
+    std::vector<TenT> o;
+    std::vector<std::string> names;
+    for (int j = 0; j < nops; ++j) {
+      o.push_back(ops.at(op_ids[j]));
+      if (!basenames[j]) throw std::invalid_argument("measure_one_site: NULL file basename");
+      names.emplace_back(basenames[j]);
+    }
+    auto set = gqmps2::MeasureOneSiteOp(*mps, o, names);
+    for (int j = 0; j < nops; ++j)
+      for (size_t i = 0; i < N; ++i) avgs[j * N + i] = set[j][i].avg;
+  }
+
+  void measure_multi_site(int64_t nevents, const int32_t *nphys, const int64_t *st, const int32_t *phys_ids,
+                          const int32_t *inst_ids, const char *basename, double *avgs) override {
+    require_mps();
+    if (!basename) throw std::invalid_argument("measure_multi_site: NULL file basename");
+    const auto &id_ops = mps->GetSitesInfo().id_ops;
+    gqmps2::TenVV<double, QNT> phys_set(nevents);
+    gqmps2::TenVVV<double, QNT> inst_set(nevents);
+    std::vector<std::vector<size_t>> sites_set(nevents);
+    size_t p = 0, g = 0;
+    for (int64_t e = 0; e < nevents; ++e) {
+      if (nphys[e] < 2) throw std::invalid_argument("measure_multi_site: an event needs two operators or more");
+      for (int k = 0; k < nphys[e]; ++k, ++p) {
+        if (st[p] < 0 || (size_t)st[p] >= N) throw std::out_of_range("measure_multi_site: site out of range");
+        phys_set[e].push_back(ops.at(phys_ids[p]));
+        sites_set[e].push_back((size_t)st[p]);
+      }
+      if (!gqmps2::IsOrderKept(sites_set[e])) throw std::invalid_argument("measure_multi_site: sites not ascending");
+      for (int k = 0; k + 1 < nphys[e]; ++k, ++g) {
+        std::vector<TenT> gap;
+        for (size_t s = sites_set[e][k] + 1; s < sites_set[e][k + 1]; ++s)
+          gap.push_back(inst_ids[g] < 0 ? id_ops[s] : ops.at(inst_ids[g]));
+        inst_set[e].push_back(std::move(gap));
+      }
+    }
+    auto res = gqmps2::MeasureMultiSiteOp(*mps, phys_set, inst_set, sites_set, basename);
+    for (int64_t e = 0; e < nevents; ++e) avgs[e] = res[e].avg;
   }
 
   double dmrg(int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *sched, double trunc_err, double lerr,
@@ -518,6 +569,23 @@ int gqmps2_bench_tensors(gqmps2_model m, mps2_ten *out5) {
       out5[k] = nullptr;
       if (m->m->bench_tens[k]) gqten::CheckMps2(mps2_ten_copy(m->m->bench_tens[k], &out5[k]), "bench copy");
     }
+  });
+}
+
+int gqmps2_measure_one_site(gqmps2_model m, int nops, const int32_t *op_ids, const char *const *basenames,
+                            double *avgs) {
+  return guard([&] {
+    if (!m || !op_ids || !basenames || !avgs) throw std::invalid_argument("NULL argument");
+    m->m->measure_one_site(nops, op_ids, basenames, avgs);
+  });
+}
+
+int gqmps2_measure_multi_site(gqmps2_model m, int64_t nevents, const int32_t *nphys, const int64_t *sites,
+                              const int32_t *phys_op_ids, const int32_t *inst_op_ids, const char *basename,
+                              double *avgs) {
+  return guard([&] {
+    if (!m || !nphys || !sites || !phys_op_ids || !inst_op_ids || !avgs) throw std::invalid_argument("NULL argument");
+    m->m->measure_multi_site(nevents, nphys, sites, phys_op_ids, inst_op_ids, basename, avgs);
   });
 }
 

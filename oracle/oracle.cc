@@ -955,6 +955,39 @@ struct Dmrg {
   }
 };
 
+// ---------------------------------------------------------------- measurements
+// <psi| prod_i op_i |psi> of the MPS as it stands (any gauge, no
+// normalisation): the transfer chain from site 0 to site N-1 with the identity
+// where ops[i] is NULL.  This is what MeasureOneSiteOp / MeasureMultiSiteOp
+// return (finite_mps_measu.h:150-554) without their canonical-form shortcut:
+// the reference centres the MPS at the first operator so that the chain left
+// of it and right of the last operator is the identity; here every site is
+// contracted, so the value does not depend on the gauge.
+// MPS legs (l IN, p OUT, r OUT), ends (p, r) and (l, p); op (p IN, p OUT).
+double expect(const std::vector<Ten> &mps, const std::vector<const Ten *> &ops) {
+  const int n = (int)mps.size();
+  if (n < 2 || (int)ops.size() != n) throw std::invalid_argument("expect: need N >= 2 tensors and N operator slots");
+  Ten E;
+  {
+    const Ten &A = mps[0];
+    Ten bra = dag(A);
+    if (ops[0]) E = contract(contract(A, *ops[0], {0}, {0}), bra, {1}, {0});
+    else E = contract(A, bra, {0}, {0});
+  }
+  for (int i = 1; i < n; ++i) {
+    const Ten &A = mps[i];
+    Ten bra = dag(A);
+    Ten t = contract(E, A, {0}, {0});   // (bra bond, p[, r])
+    if (ops[i]) t = contract(t, *ops[i], {1}, {0});   // (bra bond[, r], p')
+    const bool last = i == n - 1;
+    if (ops[i] && !last) E = contract(t, bra, {0, 2}, {0, 1});
+    else E = contract(t, bra, {0, 1}, {0, 1});
+  }
+  if (E.rank() != 0) throw std::logic_error("expect: chain did not close");
+  auto it = E.blk.find(Coord());
+  return it == E.blk.end() ? 0.0 : it->second[0];
+}
+
 template <class F>
 int guarded(F f) {
   try {
@@ -1084,6 +1117,18 @@ int oracle_dmrg(int n, oracle_ten *mps, oracle_ten *mpo, int64_t sweeps, int64_t
       delete T(mps[i]);
       mps[i] = new Ten(std::move(m[i]));
     }
+  });
+}
+
+int oracle_expect(int n, oracle_ten *mps, oracle_ten *ops, double *out) {
+  return guarded([&] {
+    std::vector<Ten> m;
+    std::vector<const Ten *> o;
+    for (int i = 0; i < n; ++i) {
+      m.push_back(*T(mps[i]));
+      o.push_back(ops && ops[i] ? T(ops[i]) : nullptr);
+    }
+    *out = expect(m, o);
   });
 }
 

@@ -83,6 +83,10 @@ int oracle_dmrg(int n, oracle_ten *mps, oracle_ten *mpo, int64_t sweeps,
                 double *energies, int64_t *dims, int64_t *iters,
                 double *trunc_errs, double *final_energy);
 
+/* <psi| prod_i ops[i] |psi> of the MPS as given (any gauge, unnormalised):
+ * the full transfer chain; ops[i] NULL (or ops NULL) = identity at site i. */
+int oracle_expect(int n, oracle_ten *mps, oracle_ten *ops, double *out);
+
 /* Threads used by the oracle's GEMM (OpenMP); returns the count. */
 int oracle_set_threads(int n);
 

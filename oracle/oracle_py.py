@@ -52,6 +52,7 @@ def lib():
                                   C.c_double, C.c_double, C.c_int64, C.POINTER(C.c_double),
                                   C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
+        L.oracle_expect.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp), C.POINTER(C.c_double)]
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_set_threads.restype = C.c_int
@@ -185,6 +186,20 @@ def dmrg(mps, mpo, sweeps, dmin, dmax, trunc_err, lanczos_error, lanczos_max_ite
         raise RuntimeError("oracle dmrg: " + _err())
     out_mps = [OTen(arr[i]).to_bt() for i in range(n)]
     return dict(energy=e0.value, energies=en, dims=dims, iters=its, trunc_errs=te, mps=out_mps)
+
+
+def expect(mps, ops):
+    """<psi| prod_i ops[i] |psi> of the MPS as given (ops[i] None = identity)."""
+    n = len(mps)
+    hs = [OTen.from_bt(t) for t in mps]
+    os_ = [OTen.from_bt(o) if o is not None else None for o in ops]
+    arr = (C.c_void_p * n)(*[h.h for h in hs])
+    oarr = (C.c_void_p * n)(*[o.h if o is not None else None for o in os_])
+    out = C.c_double()
+    rc = lib().oracle_expect(n, arr, oarr, C.byref(out))
+    if rc:
+        raise RuntimeError("oracle expect: " + _err())
+    return out.value
 
 
 def set_threads(n):

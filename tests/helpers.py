@@ -130,3 +130,84 @@ def rank_deficient_state():
     A = BTensor.random([l0, p, k], (0,), rng)
     B = BTensor.random([k.inverse(), p, r], (0,), rng)
     return O.contract(A, B, [[2], [0]])
+
+
+def mps_dense_state(mps):
+    """Dense state vector of an MPS (site 0 = slowest index)."""
+    psi = mps[0].dense()
+    for t in mps[1:]:
+        psi = np.tensordot(psi, t.dense(), axes=([-1], [0]))
+    return psi.reshape(-1)
+
+
+def dense_expect(psi, dims, ops):
+    """<psi| prod_i ops[i] |psi> for dense ops in the op(i = IN, j = OUT)
+    convention, i.e. (O psi)_j = sum_i op[i, j] psi_i; None = identity."""
+    v = psi.reshape(dims)
+    w = v
+    for k, o in enumerate(ops):
+        if o is not None:
+            w = np.moveaxis(np.tensordot(o, w, axes=([0], [k])), 0, k)
+    return float((v * w).sum())
+
+
+def random_spin_mps(rng, n, maxdim=3, div=0):
+    """Random (non-canonical, unnormalised) U(1) MPS of spin-1/2 sites; bond k
+    carries every charge of the right parity within |q| <= k + 1."""
+    p = spin_phys(OUT)
+    bonds = []
+    for k in range(n - 1):
+        qs = list(range(-(k + 1), k + 2, 2))
+        bonds.append(bond(OUT, [int(rng.integers(1, maxdim + 1)) for _ in qs], qs))
+    mps = [BTensor.random([p, bonds[0]], (0,), rng)]
+    for k in range(1, n - 1):
+        mps.append(BTensor.random([bonds[k - 1].inverse(), p, bonds[k]], (0,), rng))
+    mps.append(BTensor.random([bonds[-1].inverse(), p], (div,), rng))
+    return mps
+
+
+def product_mps(phys: Leg, labs):
+    """DirectStateInitMps (finite_mps_init.h:216-262) in numpy: one-dimensional
+    bonds whose charges follow the occupied sectors, every tensor = 1 at the
+    labelled state."""
+    n = len(labs)
+    offs = phys.offsets()
+
+    def sect(c):
+        for s, o in enumerate(offs):
+            if o <= c < o + phys.sectors[s][1]:
+                return s
+        raise ValueError(c)
+
+    q = [phys.sectors[sect(c)][0] for c in labs]
+    qw = len(q[0])
+    tot = tuple(sum(x[k] for x in q) for k in range(qw))
+    rq = tuple(tot[k] - q[0][k] for k in range(qw))
+    legs_r = Leg(OUT, [(rq, 1)])
+    mps = []
+    t = BTensor([phys, legs_r], qw)
+    t.blocks[(sect(labs[0]), 0)] = np.ones((1, 1))
+    mps.append(t)
+    for i in range(1, n - 1):
+        l = legs_r.inverse()
+        rq = tuple(l.sectors[0][0][k] - q[i][k] for k in range(qw))
+        legs_r = Leg(OUT, [(rq, 1)])
+        t = BTensor([l, phys, legs_r], qw)
+        t.blocks[(0, sect(labs[i]), 0)] = np.ones((1, 1, 1))
+        mps.append(t)
+    t = BTensor([legs_r.inverse(), phys], qw)
+    t.blocks[(0, sect(labs[-1]))] = np.ones((1, 1))
+    mps.append(t)
+    return mps
+
+
+def dense_op_tensor(phys: Leg, dense):
+    """Block tensor (phys IN, phys OUT) of a dense operator (zeros dropped)."""
+    t = BTensor([phys.inverse(), phys], len(phys.sectors[0][0]))
+    offs = phys.offsets()
+    for a, (qa, da) in enumerate(phys.sectors):
+        for b, (qb, db) in enumerate(phys.sectors):
+            blk = np.asarray(dense)[offs[a]:offs[a] + da, offs[b]:offs[b] + db]
+            if np.any(blk != 0):
+                t.blocks[(a, b)] = np.array(blk, dtype=np.float64)
+    return t

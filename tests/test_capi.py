@@ -40,3 +40,13 @@ def test_errors_are_reported_not_fatal():
     rc = lib.mps2_contract(None, None, 0, None, None, None)
     assert rc == engine.lib().mps2_version() * 0 + 1  # MPS2_ERR_ARG
     assert b"NULL" in lib.mps2_last_error()
+
+
+def test_host_layer_exports_every_declared_symbol():
+    from mps2_amd import gqmps2
+    lib = ctypes.CDLL(gqmps2.LIB_PATH)
+    names = declared(os.path.join(ROOT, "include", "gqmps2_capi.h"))
+    assert "gqmps2_measure_multi_site" in names and len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(gqmps2.SIGNATURES)

@@ -199,3 +199,97 @@ def test_svd_unresolved_values_rule():
     assert U.legs[-1].key() == (1, (((-3,), 6), ((-1,), 4), ((1,), 3), ((3,), 3)))
     # the discarded weight of the unresolved values is negligible
     assert e < 1e-20
+
+
+# ---------------------------------------------------------------- measurements
+SZ = np.diag([0.5, -0.5])
+SP = np.array([[0.0, 1.0], [0.0, 0.0]])
+SM = np.array([[0.0, 0.0], [1.0, 0.0]])
+
+
+def _expect_cases(n, rng):
+    cases = [[None] * n]                                   # the squared norm
+    for i in range(n):                                     # one-site
+        ops = [None] * n
+        ops[i] = SZ
+        cases.append(ops)
+    for (i, j) in [(0, 1), (0, n - 1), (1, 4), (2, 3)]:    # strings with and without insertions
+        ops = [None] * n
+        ops[i], ops[j] = SZ, SZ
+        cases.append(ops)
+        ops = [None] * n
+        ops[i], ops[j] = SP, SM
+        for k in range(i + 1, j):
+            ops[k] = 2 * SZ
+        cases.append(ops)
+    cases.append([np.diag(rng.uniform(-1, 1, 2)) for _ in range(n)])   # every site
+    return cases
+
+
+def test_expect_matches_dense_state():
+    """The oracle's transfer chain against the dense state vector of a random
+    (non-canonical, unnormalised) U(1) MPS: one-site values, strings with
+    identity and non-identity insertions, QN-changing pairs (S+ S-) and an
+    operator on every site.  A transposed operator, a skipped site or a
+    wrong contraction axis fails one of them."""
+    rng = np.random.default_rng(77)
+    n = 6
+    p = H.spin_phys(OUT)
+    for div in (0, 2):
+        mps = H.random_spin_mps(rng, n, div=div)
+        psi = H.mps_dense_state(mps)
+        norm2 = float(psi @ psi)
+        for ops in _expect_cases(n, rng):
+            want = H.dense_expect(psi, [2] * n, ops)
+            got = O.expect(mps, [H.dense_op_tensor(p, o) if o is not None else None for o in ops])
+            assert abs(got - want) <= 1e-13 * norm2, (ops, got, want)
+        # S+ alone changes the charge: zero, with no block left in the scalar
+        ops = [None] * n
+        ops[2] = SP
+        assert O.expect(mps, [H.dense_op_tensor(p, o) if o is not None else None for o in ops]) == 0.0
+
+
+def test_expect_gauge_invariant():
+    """Moving the gauge (Centralize(0) by untruncated SVDs) leaves every value
+    unchanged, the property the reference's canonical-form shortcut uses."""
+    rng = np.random.default_rng(78)
+    n = 6
+    p = H.spin_phys(OUT)
+    mps = H.random_spin_mps(rng, n)
+    cen = O.centralize0(mps)
+    for ops in _expect_cases(n, rng):
+        t = [H.dense_op_tensor(p, o) if o is not None else None for o in ops]
+        a, b = O.expect(mps, t), O.expect(cen, t)
+        assert abs(a - b) <= 1e-12 * abs(O.expect(mps, [None] * n)), (a, b)
+
+
+def test_expect_reference_product_states():
+    """test_finite_mps_measu.cc:43-170 (double cases): a 6-site chain of
+    (N=0, N=1) sites in the product states DirectStateInitMps builds; <n_i>,
+    <1 1> and <n_i n_j> with identity insertions, compared exactly."""
+    phys = H.Leg(OUT, [((0,), 1), ((1,), 1)])
+    ntot = H.dense_op_tensor(phys, np.diag([0.0, 1.0]))
+    idt = H.dense_op_tensor(phys, np.eye(2))
+    n = 6
+    labs1 = [1] * n
+    labs2 = [i % 2 for i in range(n)]
+    sites_set = [(0, 1), (0, 2), (0, 5), (1, 2), (1, 3), (4, 5)]
+
+    def two(mps, a_op, b_op, s):
+        ops = [None] * n
+        ops[s[0]], ops[s[1]] = a_op, b_op
+        for k in range(s[0] + 1, s[1]):
+            ops[k] = idt
+        return O.expect(mps, ops)
+
+    for labs in (labs1, labs2):
+        mps = H.product_mps(phys, labs)
+        for i in range(n):
+            ops = [None] * n
+            ops[i] = ntot
+            assert O.expect(mps, ops) == labs[i]
+    m1, m2 = H.product_mps(phys, labs1), H.product_mps(phys, labs2)
+    assert [two(m1, idt, idt, s) for s in sites_set] == [1.0] * 6
+    assert [two(m1, ntot, ntot, s) for s in sites_set] == [1.0] * 6
+    assert [two(m2, idt, idt, s) for s in sites_set] == [1.0] * 6
+    assert [two(m2, ntot, ntot, s) for s in sites_set] == [0, 0, 0, 0, 1, 0]
