@@ -5,6 +5,8 @@ and per-update parity with the CPU FP64 oracle sweep started from the same
 right-canonical MPS (energies to 1e-10 relative, identical kept bond
 dimensions, identical Lanczos iteration counts).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -177,3 +179,18 @@ def test_hubbard_cylinder_mixed_matches_fp64(tp, U):
     assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
     assert np.array_equal(gmx["dims"], g64["dims"]), (g64["dims"], gmx["dims"])
     assert g64["dims"].max() == 512
+
+
+@pytest.mark.gpu
+def test_cpp_spin_chain_cases_with_resume(tmp_path):
+    """test_two_site_update_finite_vmps.cc:91-222 as a C++ program over the
+    drop-in headers: Ising E0 and measurements of the loaded result, Heisenberg
+    E0 = -2.493577133888 to 1e-12, and the same energy when a second run
+    continues from the MPS files."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp",
+                       "test_two_site_update_finite_vmps")
+    assert os.path.exists(exe), "tests/cpp is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
