@@ -141,6 +141,14 @@ void CtrctMidTen(const FiniteMPS<TenElemT, QNT> &mps, const size_t site, const G
 
 namespace measu_detail {
 
+// Upload each caller operator once, before it is copied into per-event
+// lists: the copies then share that device tensor instead of uploading
+// their own.
+template <typename TenT>
+void Resident(const std::vector<TenT> &ops) {
+  for (auto &t : ops) t.ToDevice();
+}
+
 // Open the string at head_site of an MPS centred there: the head tensor with
 // ops[0] applied, closed against its conjugate on every leg but the right bond
 // (finite_mps_measu.h:458-484).
@@ -321,6 +329,10 @@ MeasuRes<TenElemT> MeasureMultiSiteOp(FiniteMPS<TenElemT, QNT> &mps, const TenVV
   const size_t ne = sites_set.size();
   if (phys_ops_set.size() != ne || inst_ops_set_set.size() != ne)
     throw std::invalid_argument("MeasureMultiSiteOp: operators and sites must list the same events");
+  for (size_t e = 0; e < ne; ++e) {
+    measu_detail::Resident(phys_ops_set[e]);
+    for (auto &gap : inst_ops_set_set[e]) measu_detail::Resident(gap);
+  }
   MeasuRes<TenElemT> res(ne);
   measu_detail::StringCache<TenElemT, QNT> cache;
   for (size_t e = 0; e < ne; ++e) {
@@ -345,6 +357,7 @@ MeasuRes<TenElemT> MeasureMultiSiteOp(FiniteMPS<TenElemT, QNT> &mps, const TenVV
   if (inst_ops_set.size() != ne) throw std::invalid_argument("MeasureMultiSiteOp: one insertion list per event");
   TenVVV<TenElemT, QNT> strings(ne);
   for (size_t e = 0; e < ne; ++e) {
+    measu_detail::Resident(inst_ops_set[e]);
     const auto &s = sites_set[e];
     if (inst_ops_set[e].size() + 1 != s.size() || !IsOrderKept(s))
       throw std::invalid_argument("MeasureMultiSiteOp: one insertion operator per gap between ascending sites");
@@ -363,6 +376,7 @@ MeasuRes<TenElemT> MeasureTwoSiteOp(FiniteMPS<TenElemT, QNT> &mps, const std::ve
   if (phys_ops.size() != 2) throw std::invalid_argument("MeasureTwoSiteOp: two physical operators");
   const size_t ne = sites_set.size();
   if (inst_ops_set.size() != ne) throw std::invalid_argument("MeasureTwoSiteOp: one insertion string per event");
+  measu_detail::Resident(phys_ops);
   TenVVV<TenElemT, QNT> strings(ne);
   for (size_t e = 0; e < ne; ++e) {
     if (sites_set[e].size() != 2) throw std::invalid_argument("MeasureTwoSiteOp: two sites per event");
@@ -380,6 +394,8 @@ MeasuRes<TenElemT> MeasureTwoSiteOp(FiniteMPS<TenElemT, QNT> &mps, const std::ve
                                     const std::string &res_file_basename) {
   if (phys_ops.size() != 2) throw std::invalid_argument("MeasureTwoSiteOp: two physical operators");
   const size_t ne = sites_set.size();
+  measu_detail::Resident(phys_ops);
+  inst_op.ToDevice();
   return MeasureMultiSiteOp(mps, TenVV<TenElemT, QNT>(ne, phys_ops), TenVV<TenElemT, QNT>(ne, {inst_op}), sites_set,
                             res_file_basename);
 }

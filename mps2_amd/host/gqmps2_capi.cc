@@ -308,6 +308,12 @@ struct Model : ModelBase {
     require_mps();
     if (!basename) throw std::invalid_argument("measure_multi_site: NULL file basename");
     const auto &id_ops = mps->GetSitesInfo().id_ops;
+    // one device copy per operator, shared by the per-event copies below
+    auto op_at = [&](int32_t id) -> const TenT & {
+      const TenT &t = ops.at(id);
+      t.ToDevice();
+      return t;
+    };
     gqmps2::TenVV<double, QNT> phys_set(nevents);
     gqmps2::TenVVV<double, QNT> inst_set(nevents);
     std::vector<std::vector<size_t>> sites_set(nevents);
@@ -316,14 +322,14 @@ struct Model : ModelBase {
       if (nphys[e] < 2) throw std::invalid_argument("measure_multi_site: an event needs two operators or more");
       for (int k = 0; k < nphys[e]; ++k, ++p) {
         if (st[p] < 0 || (size_t)st[p] >= N) throw std::out_of_range("measure_multi_site: site out of range");
-        phys_set[e].push_back(ops.at(phys_ids[p]));
+        phys_set[e].push_back(op_at(phys_ids[p]));
         sites_set[e].push_back((size_t)st[p]);
       }
       if (!gqmps2::IsOrderKept(sites_set[e])) throw std::invalid_argument("measure_multi_site: sites not ascending");
       for (int k = 0; k + 1 < nphys[e]; ++k, ++g) {
         std::vector<TenT> gap;
         for (size_t s = sites_set[e][k] + 1; s < sites_set[e][k + 1]; ++s)
-          gap.push_back(inst_ids[g] < 0 ? id_ops[s] : ops.at(inst_ids[g]));
+          gap.push_back(inst_ids[g] < 0 ? id_ops[s] : op_at(inst_ids[g]));
         inst_set[e].push_back(std::move(gap));
       }
     }
