@@ -363,3 +363,44 @@ def test_lanczos_mixed_matches_fp64(ctx):
     assert abs(emx - e64) <= 1e-10 * abs(e64)
     ov = abs(float((g64.get().dense() * gmx.get().dense()).sum()))
     assert ov >= 1 - 1e-9
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED])
+def test_long_k_walks_over_several_pairs(ctx, prec):
+    """Output blocks fed by long K walks over two block pairs (K = 800 and 750,
+    two contracted legs, 97 K slabs per tile): values against numpy for
+    leading and trailing contracted legs on both operands, and bit-identical
+    on a second run (owner-computes, fixed order)."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(11)
+    k1 = Leg(OUT, [((0,), 20), ((1,), 30)])
+    k2 = Leg(OUT, [((0,), 25), ((1,), 40)])
+    for (M, N) in [(200, 150), (64, 64), (129, 257)]:
+        m = Leg(IN, [((0,), M)])
+        n = Leg(OUT, [((0,), N)])
+        A0 = BTensor.random([m, k1, k2], (1,), rng)
+        B0 = BTensor.random([k1.inverse(), k2.inverse(), n], (-1,), rng)
+        ref = np.tensordot(A0.dense(), B0.dense(), axes=([1, 2], [0, 1]))
+        bound = np.tensordot(np.abs(A0.dense()), np.abs(B0.dense()), axes=([1, 2], [0, 1]))
+        for case in ("NN", "TN", "NT", "TT"):
+            if case[0] == "N":
+                A, ax = A0, [1, 2]
+            else:
+                A = BTensor([k1, k2, m], 1, {(c[1], c[2], c[0]): b.transpose(1, 2, 0).copy() for c, b in A0.blocks.items()})
+                ax = [0, 1]
+            if case[1] == "N":
+                B, bx = B0, [0, 1]
+            else:
+                B = BTensor([n, k1.inverse(), k2.inverse()], 1,
+                            {(c[2], c[0], c[1]): b.transpose(2, 0, 1).copy() for c, b in B0.blocks.items()})
+                bx = [1, 2]
+            dA, dB = ctx.put(A), ctx.put(B)
+            got = E.contract_ex(dA, dB, [ax, bx], prec).get()
+            again = E.contract_ex(dA, dB, [ax, bx], prec).get()
+            assert sorted(got.blocks) == [(0, 0)]
+            assert np.array_equal(got.blocks[(0, 0)], again.blocks[(0, 0)]), (M, N, case)
+            g = got.dense()
+            if prec == E.PREC_FP64:
+                assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (M, N, case)
+            else:
+                assert (np.abs(g - ref) <= 1e-5 * bound + 1e-30).all(), (M, N, case)
