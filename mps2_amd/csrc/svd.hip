@@ -897,6 +897,16 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     // error theta^2 in the refined values; the tail's own maximum, ~1e-10 top,
     // then resolves a cut at 1e-8 s_max in one round.
     constexpr double kAccept = 1e-7, kTail = 1e-10, kMargin = 1e3;
+    // phase clocks of the refinement (engine counters svd_refine_{vec,eigh}_ms)
+    double vec_ms = 0, eigh_ms = 0;
+    std::chrono::steady_clock::time_point tv0;
+    auto synced_now = [&] {
+      c->sync();
+      return std::chrono::steady_clock::now();
+    };
+    auto ms_since = [](std::chrono::steady_clock::time_point t0) {
+      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    };
     std::vector<double> lam_top(ns, 0.0);  // scale of each sector's current problem
     for (int q = 0; q < ns; ++q)
       for (double x : evals[q]) lam_top[q] = std::max(lam_top[q], x);
@@ -947,6 +957,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
           tot += (int64_t)gn[q] * rn[i];
         }
         Esub = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
+        tv0 = synced_now();
         eigh_vectors_all(c, er, cols, static_cast<double *>(Esub->p), esoff);
         ES0 = Esub;
         es0off = esoff;
@@ -978,6 +989,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         if (totE > 0) {
           auto ELb = c->alloc(totE * sizeof(double));
           eigh_vectors_all(c, er, colsL, static_cast<double *>(ELb->p), eLoff);
+          c->sync();
           EL0 = ELb;
           el0off = eLoff;
           FLb = c->alloc(totF * sizeof(double));
@@ -1021,6 +1033,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
             for (int j = 0; j < nL[q]; ++j) evals[q][gn[q] - nL[q] + j] = std::ldexp(lv[iLoff[q] + j], 2 * dq[q]);
         }
       }
+      if (round == 0) vec_ms += ms_since(tv0);
       const double *Es = static_cast<const double *>(Esub->p);
       // W_S and the Gram of the projected tail
       std::vector<int64_t> woff(nr), g2off(nr);
@@ -1101,7 +1114,9 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       else raw_gemm(c, kTrans, kNormal, W_, W_, G_, gb2, gp2);
       if (!std::isfinite(dev_maxabs(c, G_, gtot2)))
         throw Error(kErrNumeric, "SVD refinement round " + std::to_string(round) + ": non-finite Gram of the projected tail");
+      const auto te0 = synced_now();
       EighResult er2 = eigh_named(c, G_, g2off, rn, "refinement round " + std::to_string(round));
+      eigh_ms += ms_since(te0);
       for (int i = 0; i < nr; ++i) {
         const int q = rs[i], r = rn[i];
         for (double &x : er2.evals[i]) x = std::ldexp(x, 2 * dq[q]);
@@ -1169,6 +1184,8 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     }
     c->count("svd_refine_ms",
              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
+    c->count("svd_refine_vec_ms", vec_ms);
+    c->count("svd_refine_eigh_ms", eigh_ms);
   }
   // 6. new bond index
   Leg knew;

@@ -269,7 +269,7 @@ def contract_ex(a: DTen, b: DTen, axes, precision=PREC_FP64) -> DTen:
 SIGNATURES["mps2_ctx_counter"] = (C.c_int, [_vp, C.c_char_p, _dp])
 COUNTERS = ("svd_calls", "svd_ms", "svd_eigh_ms", "svd_sectors", "svd_max_sector_sum", "syncs", "lanczos_calls",
             "matvec_fp64", "matvec_fp32", "hip_malloc", "alloc_trim", "svd_refine_rounds", "svd_refine_sectors",
-            "svd_refine_ms")
+            "svd_refine_ms", "svd_refine_vec_ms", "svd_refine_eigh_ms")
 
 
 def ctx_counters(ctx_handle, names=COUNTERS):
