@@ -140,6 +140,43 @@ class FiniteMPS : public MPS<TenElemT, QNT> {
   }
 };
 
+// TruncateMPS (finite_mps.h:245-303): Centralize(0), normalise the first
+// tensor, then one truncated device SVD per bond from left to right, each
+// with a normalised S carried into the next tensor.  Leaves every tensor but
+// the last left-canonical; the MPS counts as not centralised (the tensors are
+// replaced through operator()).
+template <typename TenElemT, typename QNT>
+void TruncateMPS(FiniteMPS<TenElemT, QNT> &mps, const GQTEN_Double trunc_err, const size_t Dmin, const size_t Dmax) {
+  using LocalTenT = GQTensor<TenElemT, QNT>;
+  const size_t N = mps.size();
+  if (N < 2) throw std::invalid_argument("TruncateMPS: at least two sites");
+  mps.Centralize(0);
+  mps[0].Normalize();
+  for (size_t i = 0; i + 1 < N; ++i) {
+    const size_t ldims = i == 0 ? 1 : 2;
+    GQTensor<GQTEN_Double, QNT> s;
+    LocalTenT vt;
+    auto *u = new LocalTenT;
+    double err = 0;
+    size_t D = 0;
+    const LocalTenT &A = mps[i];
+    SVD(&A, ldims, Div(A), trunc_err, Dmin, Dmax, u, &s, &vt, &err, &D);
+    if (Verbose())
+      std::cout << "Truncate MPS bond " << std::setw(4) << i << " TruncErr = " << std::setprecision(2)
+                << std::scientific << err << std::fixed << " D = " << std::setw(5) << D << std::scientific
+                << std::endl;
+    s.Normalize();
+    delete mps(i);
+    mps(i) = u;
+    LocalTenT sv;
+    Contract(&s, &vt, {{1}, {0}}, &sv);
+    auto *next = new LocalTenT;
+    Contract(&sv, mps(i + 1), {{1}, {0}}, next);
+    delete mps(i + 1);
+    mps(i + 1) = next;
+  }
+}
+
 // finite_mps_init.h:216-262
 template <typename TenElemT, typename QNT>
 void DirectStateInitMps(FiniteMPS<TenElemT, QNT> &mps, const std::vector<size_t> &stat_labs, const QNT &zero_div) {

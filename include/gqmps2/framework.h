@@ -152,12 +152,20 @@ class DuoVector {
   DuoVector(DuoVector &&o) noexcept : raw_(std::move(o.raw_)) {}
   virtual ~DuoVector() { clear(); }
 
-  ElemT &operator[](const size_t i) { return *raw_[i]; }
+  // mutable access default-constructs a missing element (duovector.h:121-131)
+  ElemT &operator[](const size_t i) {
+    if (!raw_[i]) raw_[i] = new ElemT;
+    return *raw_[i];
+  }
   const ElemT &operator[](const size_t i) const { return *raw_[i]; }
   ElemT *&operator()(const size_t i) { return raw_[i]; }
   const ElemT *operator()(const size_t i) const { return raw_[i]; }
+  ElemT &front() { return *raw_.front(); }
+  const ElemT &front() const { return *raw_.front(); }
   ElemT &back() { return *raw_.back(); }
   const ElemT &back() const { return *raw_.back(); }
+  // element pointers (duovector.h:161-170)
+  std::vector<const ElemT *> cdata() const { return std::vector<const ElemT *>(raw_.begin(), raw_.end()); }
   ElemT *alloc(const size_t i) {
     delete raw_[i];
     raw_[i] = new ElemT;

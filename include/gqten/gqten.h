@@ -220,6 +220,12 @@ class GQTensor {
   double &operator()(size_t c0, Ints... cs) {
     return (*this)(std::vector<size_t>{c0, (size_t)cs...});
   }
+  // read-only element access on a const tensor (as gqten's const operator())
+  double operator()(const std::vector<size_t> &coors) const { return GetElem(coors); }
+  template <typename... Ints>
+  double operator()(size_t c0, Ints... cs) const {
+    return GetElem(std::vector<size_t>{c0, (size_t)cs...});
+  }
   double GetElem(const std::vector<size_t> &coors) const {
     ToHost();
     BlockKey k;

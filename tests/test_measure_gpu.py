@@ -230,3 +230,16 @@ def test_correlation_sweep_reuse_at_d256(tmp_path):
         want = _oracle_pair(mps, phys, Hm.SZ, Hm.SZ, 0, j, None)
         assert abs(vals["asc"][(0, j)] - want) < 1e-12, (j, vals["asc"][(0, j)], want)
     m.close()
+
+
+def test_cpp_finite_mps_cases(tmp_path):
+    """test_finite_mps.cc:31-260 as a C++ program over the drop-in headers:
+    Centralize to every site (canonical tensors contract to the identity to
+    1e-14, flags, centre), copy/move, const versus mutable access, Dump/Load,
+    and TruncateMPS (bond dimensions, left-canonical tensors, normalised
+    result, unchanged state when nothing is cut)."""
+    exe = os.path.join(ROOT, "tests", "cpp", "test_finite_mps")
+    assert os.path.exists(exe), "tests/cpp is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
