@@ -32,6 +32,9 @@ namespace gqten {
 using GQTEN_Double = double;
 
 enum class GQTenIndexDirType { IN = 0, OUT = 1, NDIR = 2 };
+// gqten's short names for the two directions (IndexT({...}, OUT) in user code)
+constexpr GQTenIndexDirType IN = GQTenIndexDirType::IN;
+constexpr GQTenIndexDirType OUT = GQTenIndexDirType::OUT;
 
 // ---------------------------------------------------------------- errors / context
 inline void CheckMps2(int rc, const char *what) {

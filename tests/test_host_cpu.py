@@ -6,6 +6,8 @@ independent numpy Kronecker sum of the same terms; (2) exact diagonalisation
 of that dense Hamiltonian reproduces the reference's golden energies; (3) the
 oracle sweep, started from the product MPS initialisation, reproduces them.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -181,3 +183,24 @@ def test_oracle_config1_heisenberg_l16_matches_ed():
     r = O.dmrg(mps, [m.mpo(i) for i in range(m.N)], 4, 1, 64, 0.0, 1e-10, 200)
     assert abs(r["energy"] - ed["heisenberg_obc_L16_sz0"]) < 1e-8
     assert r["dims"].max() == 64
+
+
+def test_cpp_host_containers(tmp_path):
+    """The reference's DuoVector, TenVec file, SiteVec and MPO unit-test cases
+    (test_duovector.cc, test_ten_vec.cc, test_site_vec.cc, test_mpo.cc) as a
+    C++ program over the drop-in headers, compiled here and run on the host
+    (no device call)."""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "mps2_amd")
+    if not os.path.exists(os.path.join(lib, "libmps2_amd.so")):
+        pytest.fail("mps2_amd/libmps2_amd.so is not built (run __graft_entry__.build())")
+    cxx = shutil.which("g++") or shutil.which("c++")
+    exe = str(tmp_path / "test_containers")
+    subprocess.run([cxx, "-O1", "-std=c++17", "-I", os.path.join(root, "include"), "-o", exe,
+                    os.path.join(root, "tests", "cpp", "test_containers.cc"), "-L", lib, "-lmps2_amd",
+                    "-Wl,-rpath," + lib], check=True, timeout=300)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
