@@ -10,10 +10,8 @@
 // mpogen_impl.h:301-349).
 //
 // Construction (own design, replacing the reference's FSM + symbolic
-// compression): a finite-state automaton whose intermediate states are the
-// distinct unfinished term prefixes (head site, operator sequence so far).
-// Coefficients are applied on the last operator, so terms that share a prefix
-// share a state; the ready (identity) and done states close the automaton.
+// compression): the channels of each bond are the distinct remainders of the
+// sum of terms to the right of that bond, up to a scale (see Gen()).
 #ifndef MPS2_AMD_GQMPS2_MPOGEN_H
 #define MPS2_AMD_GQMPS2_MPOGEN_H
 
@@ -117,80 +115,96 @@ class MPOGenerator {
   }
 
   MPO<GQTensorT> Gen() {
-    // ---- automaton states per bond b (between site b and b+1)
-    // state keys: ready = {-1}, done = {-2}, prefix = {head, op_0, ..., op_b}
-    std::vector<std::map<std::vector<long>, size_t>> sid(N_ > 0 ? N_ - 1 : 0);
-    std::vector<std::vector<std::vector<long>>> order(N_ > 0 ? N_ - 1 : 0);
-    auto state = [&](size_t b, const std::vector<long> &key) {
-      auto it = sid[b].find(key);
-      if (it != sid[b].end()) return it->second;
-      size_t id = order[b].size();
-      sid[b][key] = id;
-      order[b].push_back(key);
-      return id;
-    };
-    const std::vector<long> kReady{-1}, kDone{-2};
-    for (size_t b = 0; b + 1 < N_; ++b) state(b, kReady);
-    // transitions on site i: (from state key at bond i-1, to key at bond i, coef, op)
-    struct Tr {
-      std::vector<long> from, to;
-      TenElemT coef;
-      size_t op;
-    };
-    std::vector<std::vector<Tr>> tr(N_);
-    // An edge into a shared prefix state is added once; edges into the done
-    // state carry one term's coefficient each and are summed.
-    std::vector<std::map<std::pair<std::vector<long>, std::vector<long>>, bool>> seen(N_);
+    if (N_ < 2) throw std::invalid_argument("MPOGenerator: at least two sites are required");
+    // A channel of bond b (between sites b and b+1) stands for a weighted set
+    // of operator strings on sites b+1 .. N-1 (its "future"), normalised so
+    // that its first string has weight 1.  Starting from the whole sum of
+    // terms before site 0, each site splits every future by its first
+    // operator; futures equal up to a scale share one channel (the scale goes
+    // on the edge into it).  Terms with a common prefix therefore share
+    // channels on the left and terms with proportional remainders share them
+    // on the right, which reproduces the compressed FSM dimensions of
+    // test_mpogen.cc:79-207.
+    using Str = std::vector<size_t>;
+    using Fut = std::map<Str, TenElemT>;
+    std::vector<size_t> id_op(N_);
+    for (size_t i = 0; i < N_; ++i) id_op[i] = Intern_(site_vec_.id_ops[i]);
+    Fut root;
     for (auto &t : terms_) {
-      size_t tail = t.head + t.ops.size() - 1;
-      std::vector<long> prev = kReady;
-      for (size_t i = t.head; i <= tail; ++i) {
-        std::vector<long> cur;
-        TenElemT c = 1;
-        if (i == tail) {
-          cur = kDone;
-          c = t.coef;
-          tr[i].push_back({prev, cur, c, t.ops[i - t.head]});
-        } else {
-          cur = {(long)t.head};
-          for (size_t k = 0; k <= i - t.head; ++k) cur.push_back((long)t.ops[k]);
-          state(i, cur);
-          if (!seen[i].count({prev, cur})) {
-            seen[i][{prev, cur}] = true;
-            tr[i].push_back({prev, cur, c, t.ops[i - t.head]});
+      Str str(N_);
+      for (size_t i = 0; i < N_; ++i) str[i] = id_op[i];
+      for (size_t k = 0; k < t.ops.size(); ++k) str[t.head + k] = t.ops[k];
+      root[str] += t.coef;
+    }
+    for (auto it = root.begin(); it != root.end();) it = it->second == TenElemT(0) ? root.erase(it) : std::next(it);
+    struct Edge {
+      size_t from, op, to;
+      TenElemT w;
+    };
+    std::vector<std::vector<Edge>> edges(N_);
+    std::vector<std::vector<QNT>> chan_qn(N_);  // channels of bond i (i < N-1)
+    std::vector<Fut> cur{root};
+    std::vector<QNT> cur_qn{zero_div_};
+    for (size_t i = 0; i < N_; ++i) {
+      std::vector<Fut> next;
+      std::map<Fut, size_t> nidx;
+      std::vector<QNT> next_qn;
+      for (size_t j = 0; j < cur.size(); ++j) {
+        std::map<size_t, Fut> by_op;
+        for (auto &kv : cur[j]) by_op[kv.first[0]][Str(kv.first.begin() + 1, kv.first.end())] += kv.second;
+        for (auto &ob : by_op) {
+          Fut &f = ob.second;
+          for (auto it = f.begin(); it != f.end();) it = it->second == TenElemT(0) ? f.erase(it) : std::next(it);
+          if (f.empty()) continue;
+          if (i + 1 == N_) {
+            edges[i].push_back({j, ob.first, 0, f.begin()->second});  // the empty string's weight
+            continue;
           }
+          const TenElemT scale = f.begin()->second;
+          for (auto &kv : f) kv.second /= scale;
+          const QNT q = zero_div_ - Div(ops_[ob.first]) + cur_qn[j];
+          auto found = nidx.find(f);
+          size_t k;
+          if (found == nidx.end()) {
+            k = next.size();
+            nidx.emplace(f, k);
+            next.push_back(std::move(f));
+            next_qn.push_back(q);
+          } else {
+            k = found->second;
+            if (!(next_qn[k] == q)) throw std::runtime_error("MPOGenerator: terms with different quantum-number divergence");
+          }
+          edges[i].push_back({j, ob.first, k, scale});
         }
-        prev = cur;
+      }
+      if (i + 1 < N_) {
+        if (next.empty()) throw std::runtime_error("MPOGenerator: no terms");
+        chan_qn[i] = next_qn;
+        cur.swap(next);
+        cur_qn.swap(next_qn);
       }
     }
-    for (size_t b = 0; b + 1 < N_; ++b) state(b, kDone);
-    // ---- QN of each state and bond indexes (grouped by QN, first appearance)
-    std::vector<std::map<std::vector<long>, QNT>> sqn(N_ > 0 ? N_ - 1 : 0);
-    for (size_t b = 0; b + 1 < N_; ++b) {
-      sqn[b][kReady] = zero_div_;
-      sqn[b][kDone] = zero_div_;
-    }
-    for (size_t i = 0; i < N_; ++i)
-      for (auto &x : tr[i]) {
-        if (i + 1 >= N_ || x.to == kDone) continue;
-        QNT lq = (i == 0 || x.from == kReady) ? zero_div_ : sqn[i - 1].at(x.from);
-        sqn[i][x.to] = zero_div_ - Div(ops_[x.op]) + lq;
-      }
-    std::vector<IndexT> bonds(N_ > 0 ? N_ - 1 : 0);
-    std::vector<std::map<std::vector<long>, size_t>> coor(N_ > 0 ? N_ - 1 : 0);
+    // ---- bond indexes: channels grouped by QN in order of first appearance (mpogen_impl.h:301-349)
+    std::vector<IndexT> bonds(N_ - 1);
+    std::vector<std::vector<size_t>> coor(N_ - 1);
     for (size_t b = 0; b + 1 < N_; ++b) {
       std::vector<QNT> qorder;
-      std::map<QNT, std::vector<std::vector<long>>> groups;
-      for (auto &key : order[b]) {
-        const QNT &q = sqn[b].at(key);
-        if (!groups.count(q)) qorder.push_back(q);
-        groups[q].push_back(key);
+      std::vector<std::vector<size_t>> groups;
+      for (size_t k = 0; k < chan_qn[b].size(); ++k) {
+        size_t g = 0;
+        while (g < qorder.size() && !(qorder[g] == chan_qn[b][k])) ++g;
+        if (g == qorder.size()) {
+          qorder.push_back(chan_qn[b][k]);
+          groups.emplace_back();
+        }
+        groups[g].push_back(k);
       }
       std::vector<QNSctT> secs;
+      coor[b].assign(chan_qn[b].size(), 0);
       size_t c = 0;
-      for (auto &q : qorder) {
-        secs.emplace_back(q, groups[q].size());
-        for (auto &key : groups[q]) coor[b][key] = c++;
+      for (size_t g = 0; g < qorder.size(); ++g) {
+        secs.emplace_back(qorder[g], groups[g].size());
+        for (auto k : groups[g]) coor[b][k] = c++;
       }
       bonds[b] = IndexT(secs, GQTenIndexDirType::OUT);
       if (Verbose()) std::cout << std::setw(3) << c << std::endl;   // bond dimension, as mpogen_impl.h:254-256
@@ -198,36 +212,20 @@ class MPOGenerator {
     // ---- realise the tensors
     MPO<GQTensorT> mpo(N_);
     for (size_t i = 0; i < N_; ++i) {
-      if (N_ == 1) throw std::invalid_argument("MPOGenerator: at least two sites are required");
       if (i == 0) mpo(i) = new GQTensorT({pb_in_[0], bonds[0], pb_out_[0]});
       else if (i == N_ - 1) mpo(i) = new GQTensorT({pb_in_[i], InverseIndex(bonds[i - 1]), pb_out_[i]});
       else mpo(i) = new GQTensorT({InverseIndex(bonds[i - 1]), pb_in_[i], pb_out_[i], bonds[i]});
-      std::vector<Tr> all = tr[i];
-      const size_t id = Intern_(site_vec_.id_ops[i]);
-      if (i > 0 && i + 1 < N_) {
-        all.push_back({kReady, kReady, TenElemT(1), id});
-        all.push_back({kDone, kDone, TenElemT(1), id});
-      } else if (i == 0) {
-        all.push_back({kReady, kReady, TenElemT(1), id});
-      } else {
-        all.push_back({kDone, kDone, TenElemT(1), id});
-      }
-      for (auto &x : all) {
-        const GQTensorT &op = ops_[x.op];
+      for (auto &e : edges[i]) {
+        const GQTensorT &op = ops_[e.op];
         size_t d = op.GetIndexes()[0].dim();
         for (size_t a = 0; a < d; ++a)
           for (size_t bb = 0; bb < d; ++bb) {
             double v = op.GetElem({a, bb});
             if (v == 0.0) continue;
-            v *= x.coef;
-            if (i == 0) {
-              if (N_ > 1 && !coor[0].count(x.to)) continue;
-              (*mpo(i))({a, coor[0].at(x.to), bb}) += v;
-            } else if (i == N_ - 1) {
-              (*mpo(i))({a, coor[i - 1].at(x.from), bb}) += v;
-            } else {
-              (*mpo(i))({coor[i - 1].at(x.from), a, bb, coor[i].at(x.to)}) += v;
-            }
+            v *= e.w;
+            if (i == 0) (*mpo(i))({a, coor[0][e.to], bb}) += v;
+            else if (i == N_ - 1) (*mpo(i))({a, coor[i - 1][e.from], bb}) += v;
+            else (*mpo(i))({coor[i - 1][e.from], a, bb, coor[i][e.to]}) += v;
           }
       }
     }

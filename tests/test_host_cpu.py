@@ -60,7 +60,9 @@ def heis_terms(pairs, J=1.0):
 def test_heisenberg_mpo_matches_kron_sum():
     N = 6
     m = HM.heisenberg_chain(N)
-    assert list(m.bond_dims) == [5] * (N - 1)
+    # 5 channels inside; the first bond needs no "finished" channel and the
+    # last no "not yet started" one (the edge pattern of test_mpogen.cc:115-140)
+    assert list(m.bond_dims) == [4] + [5] * (N - 3) + [4]
     H = kron_terms_dense(N, [2] * N, heis_terms([(i, i + 1) for i in range(N - 1)]))
     assert np.abs(mpo_dense(m) - H).max() < 1e-13
 
@@ -204,3 +206,33 @@ def test_cpp_host_containers(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0 and "PASSED" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["one_site", "zero_coef", "repeated", "chain4", "long_range5", "two_coupling4"])
+def test_mpogen_fsm_cases_bond_dims_and_operator(case):
+    """The term sets of test_mpogen.cc:79-207 (cases 1-5): the bond dimensions
+    of the generated MPO equal the shapes of the reference's compressed FSM
+    matrices (case 1: 1x1, 1x1; case 3: 1x2, 2x3, 3x2, 2x1; case 4: 1x2, 2x2,
+    2x2, 2x2, 2x1; case 5: 1x2, 2x4, 4x2, 2x1), and the MPO contracts to the
+    Kronecker sum of the same terms (a zero coefficient adds nothing, a
+    repeated term doubles)."""
+    from mps2_amd.gqmps2 import Model
+    sz = np.diag([0.5, -0.5])
+    site = [((0,), 1), ((1,), 1)]
+    spec = {
+        "one_site": (2, [(1.0, [0])], [1]),
+        "zero_coef": (2, [(1.0, [0]), (0.0, [0])], [1]),
+        "repeated": (2, [(1.0, [0, 1]), (1.0, [0, 1])], [1]),
+        "chain4": (4, [(1.0, [0, 1]), (1.0, [1, 2]), (1.0, [2, 3])], [2, 3, 2]),
+        "long_range5": (5, [(1.0, [0, 4]), (1.0, [1, 2, 4]), (1.0, [1, 2, 3])], [2, 2, 2, 2]),
+        "two_coupling4": (4, [(0.5, [0, 1]), (0.5, [0, 2]), (0.5, [1, 3]), (0.5, [2, 3]), (2.0, [0, 3]),
+                              (2.0, [1, 2])], [2, 4, 2]),
+    }[case]
+    N, terms, dims = spec
+    m = Model([site] * N)
+    for coef, sites in terms:
+        m.term(coef, [sz] * len(sites), sites)
+    assert list(m.gen_mpo()) == dims
+    H = kron_terms_dense(N, [2] * N, [(c, [(s, sz) for s in st]) for c, st in terms])
+    assert np.abs(mpo_dense(m) - H).max() < 1e-14
+    m.close()
