@@ -275,3 +275,23 @@ def test_mpogen_random_terms_match_kron_sum(seed):
     H = kron_terms_dense(N, [2] * N, dense_terms)
     assert np.abs(mpo_dense(m) - H).max() < 1e-13
     m.close()
+
+
+def test_mpogen_tail_insertion_string():
+    """AddTerm with as many insertion operators as physical ones
+    (mpogen_impl.h:144-199): the last one is listed from the last physical
+    site to the end of the chain; at the last physical site the physical
+    operator takes precedence (the first listed operator of a site is used,
+    mpogen_impl.h:99-113), so the string covers the sites after it."""
+    from mps2_amd.gqmps2 import Model
+    N = 6
+    F = 2 * HM.SZ
+    m = Model([HM.SPIN_SITE] * N)
+    m.term(0.7, [HM.SP, HM.SM], [1, 3], inst=[F, HM.SZ])
+    m.term(-0.3, [HM.SZ, HM.SZ], [0, 4])
+    m.gen_mpo()
+    # F between 1 and 3 (site 2), then SZ on every site after the last physical one
+    t1 = [(1, HM.SP), (2, F), (3, HM.SM), (4, HM.SZ), (5, HM.SZ)]
+    H = kron_terms_dense(N, [2] * N, [(0.7, t1), (-0.3, [(0, HM.SZ), (4, HM.SZ)])])
+    assert np.abs(mpo_dense(m) - H).max() < 1e-13
+    m.close()
