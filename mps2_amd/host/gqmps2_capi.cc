@@ -68,6 +68,11 @@ struct Model : ModelBase {
   std::unique_ptr<gqmps2::FiniteMPS<double, QNT>> mps;
   std::vector<TenT> saved;
 
+  // Read-only view of the MPS: FiniteMPS's mutable accessors mark a tensor
+  // as no longer canonical (as in the reference), so checks and reads go
+  // through this view and a centred MPS stays centred between calls.
+  const gqmps2::FiniteMPS<double, QNT> &cm() const { return *mps; }
+
   Model(int nsites, const int32_t *nsect, const int32_t *qns, const int32_t *dims) : N(nsites) {
     gqten::IndexVec<QNT> idx;
     int p = 0;
@@ -179,8 +184,8 @@ struct Model : ModelBase {
   }
 
   void export_mps(int site, std::vector<int32_t> &meta, std::vector<double> &data) override {
-    if (!(*mps)(site)) throw std::runtime_error("MPS tensor not resident");
-    (*mps)[site].ToFlat(meta, data);
+    if (!cm()(site)) throw std::runtime_error("MPS tensor not resident");
+    cm()[site].ToFlat(meta, data);
   }
 
   void import_mps(int site, const int32_t *meta, const double *data) override {
@@ -225,7 +230,7 @@ struct Model : ModelBase {
     for (size_t i = 0; i < N; ++i) {
       TenT c;
       mps2_ten h = nullptr;
-      gqten::CheckMps2(mps2_ten_copy((*mps)[i].Handle(), &h), "checkpoint");
+      gqten::CheckMps2(mps2_ten_copy(cm()[i].Handle(), &h), "checkpoint");
       c.AdoptDevice(h);
       saved.push_back(std::move(c));
     }
@@ -245,7 +250,7 @@ struct Model : ModelBase {
   void bench_bond(int64_t site, int warmup, int reps) override {
     if (site < 1 || (size_t)site + 2 >= N) throw std::invalid_argument("bench_bond: site must be an interior bond");
     for (size_t i = 0; i < N; ++i)
-      if (!(*mps)(i)) throw std::runtime_error("MPS not initialised");
+      if (!cm()(i)) throw std::runtime_error("MPS not initialised");
     mps->Centralize((int)site);
     gqmps2::SweepParams sp(1, 1, 1, 0.0, gqmps2::LanczosParams());
     gqmps2::TwoSiteVMPSEngine<double, QNT> eng(*mps, mpo, sp);
@@ -255,14 +260,14 @@ struct Model : ModelBase {
     for (int64_t s = 0; s < site; ++s) {
       if (s == 0) {
         TenT temp;
-        Contract(&(*mps)[0], &mpo[0], {{0}, {0}}, &temp);
-        auto ad = gqten::Dag((*mps)[0]);
+        Contract(&cm()[0], &mpo[0], {{0}, {0}}, &temp);
+        auto ad = gqten::Dag(cm()[0]);
         Contract(&temp, &ad, {{2}, {0}}, &L[1]);
       } else {
         TenT t1, t2;
-        Contract(&L[s], &(*mps)[s], {{0}, {0}}, &t1);
+        Contract(&L[s], &cm()[s], {{0}, {0}}, &t1);
         Contract(&t1, &mpo[s], {{0, 2}, {0, 1}}, &t2);
-        auto ad = gqten::Dag((*mps)[s]);
+        auto ad = gqten::Dag(cm()[s]);
         Contract(&t2, &ad, {{0, 2}, {0, 1}}, &L[s + 1]);
       }
     }
@@ -271,7 +276,7 @@ struct Model : ModelBase {
     eng.InitEnvs(rlen);
     std::vector<TenT *> H{&L[site], &mpo[site], &mpo[site + 1], &R[rlen]};
     TenT init;
-    Contract(&(*mps)[site], &(*mps)[site + 1], {{2}, {0}}, &init);
+    Contract(&cm()[site], &cm()[site + 1], {{2}, {0}}, &init);
     for (int p = 0; p < 2; ++p)
       gqten::CheckMps2(mps2_heff_bench(H[0]->Handle(), H[1]->Handle(), H[2]->Handle(), H[3]->Handle(), init.Handle(),
                                        "cent", p == 0 ? MPS2_PREC_FP64 : MPS2_PREC_MIXED, warmup, reps,
@@ -285,7 +290,7 @@ struct Model : ModelBase {
 
   void require_mps() const {
     for (size_t i = 0; i < N; ++i)
-      if (!(*mps)(i)) throw std::runtime_error("MPS not initialised");
+      if (!cm()(i)) throw std::runtime_error("MPS not initialised");
   }
 
   void measure_one_site(int nops, const int32_t *op_ids, const char *const *basenames, double *avgs) override {
@@ -341,7 +346,7 @@ struct Model : ModelBase {
               int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te, double *ut,
               double *st) override {
     for (size_t i = 0; i < N; ++i)
-      if (!(*mps)(i)) throw std::runtime_error("MPS not initialised");
+      if (!cm()(i)) throw std::runtime_error("MPS not initialised");
     gqmps2::LanczosParams lp(lerr, (size_t)lmax,
                              prec == MPS2_PREC_MIXED ? gqmps2::LanczosPrecision::Mixed : gqmps2::LanczosPrecision::FP64);
     double e0 = 0;
