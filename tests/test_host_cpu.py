@@ -295,3 +295,43 @@ def test_mpogen_tail_insertion_string():
     H = kron_terms_dense(N, [2] * N, [(0.7, t1), (-0.3, [(0, HM.SZ), (4, HM.SZ)])])
     assert np.abs(mpo_dense(m) - H).max() < 1e-13
     m.close()
+
+
+def test_oracle_ground_state_correlations_match_ed():
+    """Oracle pipeline on the host: two-site DMRG of the 10-site Heisenberg
+    chain (exact at D=32), then the oracle's transfer-chain expectation values
+    on its MPS against the exact ground state: <Sz_i>, <Sz_i Sz_j>,
+    <S+_i S-_j> and the energy as the expectation of the MPO-free sum."""
+    n = 10
+    m = HM.heisenberg_chain(n)
+    m.init_mps([[i % 2 for i in range(n)]], centralize=False)
+    raw = [m.mps(i) for i in range(n)]
+    mps = O.centralize0(raw)
+    r = O.dmrg(mps, [m.mpo(i) for i in range(n)], 6, 8, 64, 0.0, 1e-12, 200)
+    H = kron_terms_dense(n, [2] * n, heis_terms([(i, i + 1) for i in range(n - 1)]))
+    w, v = np.linalg.eigh(H)
+    psi = v[:, 0]
+    assert abs(r["energy"] - w[0]) < 1e-10
+    phys = HM.SPIN_SITE
+    from mps2_amd.flat import Leg, OUT
+    pl = Leg(OUT, phys)
+    from tests import helpers as TH
+
+    def mps_val(opmap):
+        return O.expect(r["mps"], [TH.dense_op_tensor(pl, opmap[k]) if k in opmap else None for k in range(n)])
+
+    def ed_val(opmap):
+        return TH.dense_expect(psi, [2] * n, [opmap.get(k) for k in range(n)])
+
+    norm = mps_val({})
+    assert abs(norm - 1.0) < 1e-12
+    worst = 0.0
+    for i in range(n):
+        worst = max(worst, abs(mps_val({i: HM.SZ}) - ed_val({i: HM.SZ})))
+        for j in range(i + 1, n):
+            for a, b in ((HM.SZ, HM.SZ), (HM.SP, HM.SM)):
+                worst = max(worst, abs(mps_val({i: a, j: b}) - ed_val({i: a, j: b})))
+    assert worst < 1e-9, worst
+    e_corr = sum(mps_val({i: HM.SZ, i + 1: HM.SZ}) + 0.5 * mps_val({i: HM.SP, i + 1: HM.SM})
+                 + 0.5 * mps_val({i: HM.SM, i + 1: HM.SP}) for i in range(n - 1))
+    assert abs(e_corr - w[0]) < 1e-9
