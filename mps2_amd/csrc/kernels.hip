@@ -3,6 +3,8 @@
 // reductions, block permutations and strided block copies.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include <algorithm>
 #include <cstring>
 
@@ -150,8 +152,21 @@ TenP ten_copy(const Ten &t) {
 }
 
 TenP ten_dag(const TenP &t) {
-  auto m = std::make_shared<Meta>(*t->meta);
-  for (auto &l : m->legs) l.dir = l.dir == kIn ? kOut : kIn;
+  // The flipped layout is shared by every Dag of tensors with t's layout
+  // (and Dag of it returns t's layout); the data buffer is shared as well.
+  static std::mutex mu;
+  MetaP m;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    m = t->meta->dag.p.lock();
+    if (!m) {
+      auto nm = std::make_shared<Meta>(*t->meta);
+      for (auto &l : nm->legs) l.dir = l.dir == kIn ? kOut : kIn;
+      nm->dag.p = t->meta;
+      t->meta->dag.p = nm;
+      m = nm;
+    }
+  }
   auto r = std::make_shared<Ten>(*t);
   r->meta = m;
   return r;

@@ -77,6 +77,22 @@ struct CoordHash {
 
 // Immutable layout of a block-sparse tensor: legs + sorted block list with
 // element offsets into one contiguous buffer.
+struct Meta;
+
+// The layout with every leg direction flipped (Dag), remembered by the
+// layout it was made from so that repeated Dag of tensors sharing a layout
+// yields one layout object (and the contraction-plan cache, keyed by layout
+// objects, hits).  A copied Meta starts unlinked: a copy may be modified.
+struct DagLink {
+  std::weak_ptr<const Meta> p;
+  DagLink() = default;
+  DagLink(const DagLink &) {}
+  DagLink &operator=(const DagLink &) {
+    p.reset();
+    return *this;
+  }
+};
+
 struct Meta {
   int qw = 1;
   std::vector<Leg> legs;
@@ -85,6 +101,7 @@ struct Meta {
   std::vector<int64_t> bsize;           // elements in each block
   int64_t total = 0;
   std::unordered_map<Coord, int32_t, CoordHash> lookup;
+  mutable DagLink dag;                  // see DagLink
 
   int rank() const { return (int)legs.size(); }
   int64_t dim_of(int blk, int leg) const { return legs[leg].dim[coords[blk][leg]]; }
