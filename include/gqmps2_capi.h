@@ -88,6 +88,10 @@ int gqmps2_phase_times(gqmps2_model m, double *out, int64_t n);
 /* Console output of the host layer on (1, the reference's behaviour) or off. */
 int gqmps2_set_verbose(int verbose);
 
+/* TruncateMPS (finite_mps.h:245-303) of the resident MPS: Centralize(0),
+ * then one truncated SVD per bond, left to right, with normalised S. */
+int gqmps2_mps_truncate(gqmps2_model m, double trunc_err, int64_t dmin, int64_t dmax);
+
 /* MeasureOneSiteOp (finite_mps_measu.h:150-199): each of the nops operators
  * at every site, the MPS centred there; avgs[j * N + i] = <op_j at site i>.
  * Writes basenames[j] + ".json" (DumpMeasuRes, finite_mps_measu.h:556-582).

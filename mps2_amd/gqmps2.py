@@ -44,6 +44,7 @@ SIGNATURES = {
     "gqmps2_bench_bond": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_int]),
     "gqmps2_phase_times": (C.c_int, [_vp, _dp, C.c_int64]),
     "gqmps2_set_verbose": (C.c_int, [C.c_int]),
+    "gqmps2_mps_truncate": (C.c_int, [_vp, C.c_double, C.c_int64, C.c_int64]),
     "gqmps2_measure_one_site": (C.c_int, [_vp, C.c_int, _i32p, C.POINTER(C.c_char_p), _dp]),
     "gqmps2_measure_multi_site": (C.c_int, [_vp, C.c_int64, _i32p, _i64p, _i32p, _i32p, C.c_char_p, _dp]),
     "gqmps2_engine_ctx": (_vp, []),
@@ -186,6 +187,10 @@ class Model:
         arr = (_vp * 5)()
         check(lib().gqmps2_bench_tensors(self.h, arr))
         return [engine.DTen(_vp(arr[k]), None) if arr[k] else None for k in range(5)]
+
+    def truncate_mps(self, trunc_err, dmin, dmax):
+        """TruncateMPS on the resident MPS (device SVDs)."""
+        check(lib().gqmps2_mps_truncate(self.h, float(trunc_err), int(dmin), int(dmax)))
 
     def measure_one_site(self, ops, basenames):
         """MeasureOneSiteOp with a list of dense operators (one result file

@@ -51,6 +51,7 @@ struct ModelBase {
                       int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te,
                       double *ut, double *st) = 0;
   virtual void bench_bond(int64_t site, int warmup, int reps) = 0;
+  virtual void truncate_mps(double trunc_err, int64_t dmin, int64_t dmax) = 0;
   virtual void measure_one_site(int nops, const int32_t *op_ids, const char *const *basenames, double *avgs) = 0;
   virtual void measure_multi_site(int64_t nevents, const int32_t *nphys, const int64_t *sites, const int32_t *phys_ids,
                                   const int32_t *inst_ids, const char *basename, double *avgs) = 0;
@@ -291,6 +292,12 @@ struct Model : ModelBase {
   void require_mps() const {
     for (size_t i = 0; i < N; ++i)
       if (!cm()(i)) throw std::runtime_error("MPS not initialised");
+  }
+
+  void truncate_mps(double trunc_err, int64_t dmin, int64_t dmax) override {
+    require_mps();
+    if (dmin < 1 || dmax < dmin) throw std::invalid_argument("truncate_mps: need 1 <= dmin <= dmax");
+    gqmps2::TruncateMPS(*mps, trunc_err, (size_t)dmin, (size_t)dmax);
   }
 
   void measure_one_site(int nops, const int32_t *op_ids, const char *const *basenames, double *avgs) override {
@@ -580,6 +587,13 @@ int gqmps2_bench_tensors(gqmps2_model m, mps2_ten *out5) {
       out5[k] = nullptr;
       if (m->m->bench_tens[k]) gqten::CheckMps2(mps2_ten_copy(m->m->bench_tens[k], &out5[k]), "bench copy");
     }
+  });
+}
+
+int gqmps2_mps_truncate(gqmps2_model m, double trunc_err, int64_t dmin, int64_t dmax) {
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    m->m->truncate_mps(trunc_err, dmin, dmax);
   });
 }
 

@@ -211,3 +211,31 @@ def dense_op_tensor(phys: Leg, dense):
             if np.any(blk != 0):
                 t.blocks[(a, b)] = np.array(blk, dtype=np.float64)
     return t
+
+
+def oracle_truncate_mps(mps, trunc_err, dmin, dmax):
+    """TruncateMPS (finite_mps.h:245-303) on the oracle: Centralize(0),
+    normalise the first tensor, then per bond a truncated SVD (U carries the
+    tensor's divergence), normalised S, S Vt into the next tensor.  Returns
+    (mps, kept dimension per bond)."""
+    mps = O.centralize0(list(mps))
+    h = mps[0]
+    n = h.norm()
+    mps[0] = BTensor(h.legs, h.qw, {c: b / n for c, b in h.blocks.items()})
+    dims = []
+    for i in range(len(mps) - 1):
+        A = mps[i]
+        ldiv = A.block_div(next(iter(A.blocks)))
+        u, s, vt, _, D = O.svd(A, 1 if i == 0 else 2, ldiv, trunc_err, dmin, dmax)
+        sn = s.norm()
+        s = BTensor(s.legs, s.qw, {c: b / sn for c, b in s.blocks.items()})
+        mps[i] = u
+        mps[i + 1] = O.contract(O.contract(s, vt, [[1], [0]]), mps[i + 1], [[1], [0]])
+        dims.append(D)
+    return mps, dims
+
+
+def schmidt_values(psi, dims, cut):
+    """Schmidt values of a dense state across the cut after site `cut`."""
+    left = int(np.prod(dims[:cut + 1]))
+    return np.linalg.svd(psi.reshape(left, -1), compute_uv=False)

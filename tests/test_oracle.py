@@ -293,3 +293,25 @@ def test_expect_reference_product_states():
     assert [two(m1, ntot, ntot, s) for s in sites_set] == [1.0] * 6
     assert [two(m2, idt, idt, s) for s in sites_set] == [1.0] * 6
     assert [two(m2, ntot, ntot, s) for s in sites_set] == [0, 0, 0, 0, 1, 0]
+
+
+def test_oracle_truncate_mps_against_schmidt_values():
+    """The oracle's TruncateMPS restatement on four spin-1/2 sites, where only
+    the middle bond can exceed Dmax = 2: that cut keeps the two largest
+    Schmidt values of the state across it (normalised), the result is
+    normalised, and nothing changes when nothing is cut."""
+    rng = np.random.default_rng(91)
+    n = 4   # bonds 0 and 2 have rank <= 2, bond 1 up to 4; even: total charge 0
+    mps = H.random_spin_mps(rng, n, maxdim=3)
+    psi = H.mps_dense_state(mps)
+    psi = psi / np.linalg.norm(psi)
+    full, dims_full = H.oracle_truncate_mps(mps, 0.0, 1, 1000)
+    phi = H.mps_dense_state(full)
+    assert abs(abs(phi @ psi) - 1.0) < 1e-12
+    cut, _ = H.oracle_truncate_mps(mps, 0.0, 1, 2)
+    phi = H.mps_dense_state(cut)
+    assert abs(np.linalg.norm(phi) - 1.0) < 1e-12
+    s_psi = H.schmidt_values(psi, [2] * n, 1)
+    s_phi = H.schmidt_values(phi, [2] * n, 1)
+    top = s_psi[:2] / np.linalg.norm(s_psi[:2])
+    assert np.allclose(s_phi[:2], top, atol=1e-12) and np.all(s_phi[2:] < 1e-12)
