@@ -7,7 +7,7 @@ operands, closed forms, and dense eigensolvers.
 import numpy as np
 import pytest
 
-from mps2_amd.flat import BTensor, IN, OUT
+from mps2_amd.flat import BTensor, IN, OUT, Leg
 from oracle import oracle_py as O
 from tests import helpers as H
 
@@ -315,3 +315,27 @@ def test_oracle_truncate_mps_against_schmidt_values():
     s_phi = H.schmidt_values(phi, [2] * n, 1)
     top = s_psi[:2] / np.linalg.norm(s_psi[:2])
     assert np.allclose(s_phi[:2], top, atol=1e-12) and np.all(s_phi[2:] < 1e-12)
+
+
+def test_u1xu1_contract_and_svd_against_dense():
+    """U(1)xU(1) charges (qw = 2, the t-J / Hubbard sites): the oracle's
+    Contract against numpy on the dense arrays, and its SVD against the dense
+    SVD (reconstruction, orthonormal factors, singular values)."""
+    rng = np.random.default_rng(13)
+    site = Leg(OUT, [((0, 0), 1), ((1, 0), 1), ((0, 1), 1), ((1, 1), 1)])   # Hubbard basis
+    l0 = Leg(IN, [((0, 0), 2), ((1, 0), 3), ((0, 1), 3), ((1, 1), 2), ((2, 1), 1)])
+    r = Leg(OUT, [((1, 1), 2), ((2, 1), 3), ((1, 2), 3), ((2, 2), 4), ((3, 2), 1)])
+    psi = BTensor.random([l0, site, site, r], (0, 0), rng, qw=2)
+    assert psi.blocks
+    B = BTensor.random([r.inverse(), site, Leg(OUT, [((1, 1), 2), ((2, 2), 3)])], (0, 0), rng, qw=2)
+    C = O.contract(psi, B, [[3], [0]])
+    ref = np.einsum("abcd,def->abcef", psi.dense(), B.dense())
+    assert np.abs(C.dense() - ref).max() < 1e-13
+    U, S, Vt, err, D = O.svd(psi, 2, (0, 0), 0.0, 1000, 1000)
+    rec = np.einsum("abk,kl,lcd->abcd", U.dense(), S.dense(), Vt.dense())
+    assert np.abs(rec - psi.dense()).max() < 1e-13
+    u = U.dense().reshape(-1, U.dense().shape[-1])
+    assert np.abs(u.T @ u - np.eye(u.shape[1])).max() < 1e-13
+    dense_s = np.linalg.svd(psi.dense().reshape(l0.dim * 4, -1), compute_uv=False)
+    ours = np.sort(np.diag(S.dense()))[::-1]
+    assert np.abs(ours - dense_s[:len(ours)]).max() < 1e-13
