@@ -404,3 +404,29 @@ def test_long_k_walks_over_several_pairs(ctx, prec):
                 assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (M, N, case)
             else:
                 assert (np.abs(g - ref) <= 1e-5 * bound + 1e-30).all(), (M, N, case)
+
+
+def test_u1xu1_contract_and_svd_parity(ctx):
+    """U(1)xU(1) charges (qw = 2): device Contract and SVD against the oracle
+    on the same tensors (identical block sets, 1e-12 values, identical kept
+    dimension and new-bond sectors)."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(13)
+    site = Leg(OUT, [((0, 0), 1), ((1, 0), 1), ((0, 1), 1), ((1, 1), 1)])
+    l0 = Leg(IN, [((0, 0), 12), ((1, 0), 23), ((0, 1), 23), ((1, 1), 12), ((2, 1), 7)])
+    r = Leg(OUT, [((-1, -1), 12), ((0, -1), 23), ((-1, 0), 23), ((0, 0), 24), ((-2, -1), 7), ((1, 0), 9), ((0, 1), 8)])
+    psi = BTensor.random([l0, site, site, r], (0, 0), rng, qw=2)
+    B = BTensor.random([r.inverse(), site, Leg(OUT, [((0, 0), 9), ((1, 1), 13), ((1, 0), 5)])], (0, 0), rng, qw=2)
+    assert len(psi.blocks) > 20 and B.blocks
+    ref = O.contract(psi, B, [[3], [0]])
+    got = E.contract(ctx.put(psi), ctx.put(B), [[3], [0]]).get()
+    assert same_blocks(got, ref)
+    assert H.max_rel_diff(got, ref) < TOL
+    for dmax in (1000, 40):
+        u, s, vt, err, D = O.svd(psi, 2, (0, 0), 0.0, 1, dmax)
+        gu, gs, gvt, gerr, gD = E.svd(ctx.put(psi), 2, (0, 0), 0.0, 1, dmax, E.ORTH_LEFT)
+        gs = gs.get()
+        assert gD == D and gs.legs[0].key() == s.legs[0].key()
+        assert dmax == 1000 or err > 0.0   # the second case cuts
+        assert np.abs(np.sort(np.diag(gs.dense())) - np.sort(np.diag(s.dense()))).max() < 1e-12 * np.abs(s.dense()).max()
+        assert abs(gerr - err) < 1e-12

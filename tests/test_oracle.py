@@ -324,10 +324,10 @@ def test_u1xu1_contract_and_svd_against_dense():
     rng = np.random.default_rng(13)
     site = Leg(OUT, [((0, 0), 1), ((1, 0), 1), ((0, 1), 1), ((1, 1), 1)])   # Hubbard basis
     l0 = Leg(IN, [((0, 0), 2), ((1, 0), 3), ((0, 1), 3), ((1, 1), 2), ((2, 1), 1)])
-    r = Leg(OUT, [((1, 1), 2), ((2, 1), 3), ((1, 2), 3), ((2, 2), 4), ((3, 2), 1)])
+    r = Leg(OUT, [((-1, -1), 2), ((0, -1), 3), ((-1, 0), 3), ((0, 0), 4), ((-2, -1), 1), ((1, 0), 2), ((0, 1), 2)])
     psi = BTensor.random([l0, site, site, r], (0, 0), rng, qw=2)
-    assert psi.blocks
-    B = BTensor.random([r.inverse(), site, Leg(OUT, [((1, 1), 2), ((2, 2), 3)])], (0, 0), rng, qw=2)
+    assert len(psi.blocks) > 20
+    B = BTensor.random([r.inverse(), site, Leg(OUT, [((0, 0), 2), ((1, 1), 3), ((1, 0), 2)])], (0, 0), rng, qw=2)
     C = O.contract(psi, B, [[3], [0]])
     ref = np.einsum("abcd,def->abcef", psi.dense(), B.dense())
     assert np.abs(C.dense() - ref).max() < 1e-13
