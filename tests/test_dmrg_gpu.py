@@ -237,3 +237,19 @@ def test_cpp_lanczos_solver_cases(tmp_path):
                 w -= Q @ (Q.T @ w)
             Q = np.hstack([Q, (w / np.linalg.norm(w))[:, None]])
         assert min(abs(e - r) for r in ritz) < 1e-8 * abs(w0), (e, ritz[-3:])
+
+
+@pytest.mark.gpu
+def test_cpp_tj_cases(tmp_path):
+    """test_two_site_update_finite_vmps.cc:491-640 (real cases) as a C++
+    program over the drop-in headers: U(1)xU(1) t-J model (t = 3, J = 1) with
+    the two-operator AddTerm and a Jordan-Wigner insertion operator; 4-site
+    chain E0 = -6.947478526233 and 2x2 plaquette E0 = -8.868563739680, each to
+    the reference's 1e-10."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp",
+                       "test_tj_sweeps")
+    assert os.path.exists(exe), "tests/cpp is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
