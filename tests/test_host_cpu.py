@@ -35,10 +35,10 @@ def mpo_dense(model):
     return R[0]
 
 
-def kron_terms_dense(N, dims, terms):
+def kron_terms_dense(N, dims, terms, dtype=float):
     """terms: list of (coef, [(site, op_in_out_matrix)]) with identity elsewhere."""
     D = int(np.prod(dims))
-    H = np.zeros((D, D))
+    H = np.zeros((D, D), dtype=dtype)
     for coef, ops in terms:
         mats = [np.eye(d) for d in dims]
         for s, op in ops:
@@ -128,6 +128,64 @@ def test_ed_reproduces_reference_golden_energy(name):
         total = tuple(np.sum([qns[i][labs[i]] for i in range(m.N)], axis=0))
         e = sector_ground_energy(H, qns, total)
     assert abs(e - case["e0"]) < max(case["tol"], 1e-11), (e, case["e0"])
+
+
+def test_ed_reproduces_reference_complex_rashba_energy():
+    """Target of the complex (GQTEN_Complex) path, which this build does not
+    provide yet: the reference's Rashba t-J case
+    (test_two_site_update_finite_vmps.cc:664-789) on a 3x2 lattice, periodic in
+    x, with imaginary couplings i*lambda on the y bonds.  Its sweep keeps every
+    state (D = 30 >= 27 at the centre bond), so its golden energy is the exact
+    ground energy of the Hamiltonian its AddTerm calls define: operators as
+    op[in, out], the insertion operator f on the sites strictly between the two
+    operators of a term, N = 5 from the start state {0, 1, 0, 2, 0, 1}."""
+    import itertools
+
+    def op(entries):
+        m = np.zeros((3, 3), complex)
+        for (i, j), v in entries.items():
+            m[i, j] = v
+        return m
+
+    f = op({(0, 0): -1, (1, 1): -1, (2, 2): 1})
+    sz, sp, sm = op({(0, 0): 0.5, (1, 1): -0.5}), op({(0, 1): 1}), op({(1, 0): 1})
+    cup, cdagup, cdn, cdagdn = op({(2, 0): 1}), op({(0, 2): 1}), op({(2, 1): 1}), op({(1, 2): 1})
+    ntot = op({(0, 0): 1, (1, 1): 1})
+    t, J, lamb = 3.0, 1.0, 0.03
+    Nx, Ny = 3, 2
+    N = Nx * Ny
+    spec = []
+
+    def bond(s0, s1, so):
+        s0, s1 = min(s0, s1), max(s0, s1)
+        spec.extend([(-t, cdagup, s0, cup, s1), (-t, cdagdn, s0, cdn, s1), (-t, cup, s0, cdagup, s1),
+                     (-t, cdn, s0, cdagdn, s1)] + [(c, A, s0, B, s1) for c, A, B in so])
+        for c, A, B in [(J, sz, sz), (J / 2, sp, sm), (J / 2, sm, sp), (-J / 4, ntot, ntot)]:
+            terms.append((c, [(s0, A), (s1, B)]))
+
+    terms = []
+    for x in range(Nx):
+        for y in range(Ny):
+            s0 = x * Ny + y
+            if x != Nx - 1:
+                so = [(lamb, cdagup, cdn), (lamb, cup, cdagdn), (-lamb, cdagdn, cup), (-lamb, cdn, cdagup)]
+            else:
+                so = [(lamb, cdn, cdagup), (lamb, cdagdn, cup), (-lamb, cup, cdagdn), (-lamb, cdagup, cdn)]
+            bond(s0, ((x + 1) % Nx) * Ny + y, so)
+            if y != Ny - 1:
+                il = 1j * lamb
+                bond(s0, x * Ny + y + 1, [(il, cdagup, cdn), (il, cdagdn, cup), (-il, cup, cdagdn), (-il, cdn, cdagup)])
+    terms += fermion_terms_with_strings(spec, N, f)
+    H = kron_terms_dense(N, [3] * N, terms, dtype=complex)
+    assert np.abs(H - H.conj().T).max() < 1e-14
+    assert np.abs(H.imag).max() > 0.02          # the imaginary couplings are present
+    qn = [1, 1, 0]
+    keep = [k for k, s in enumerate(itertools.product(range(3), repeat=N)) if sum(qn[v] for v in s) == 5]
+    e = np.linalg.eigvalsh(H[np.ix_(keep, keep)])[0]
+    assert abs(e - (-11.018692166942165)) < 1e-12, e
+    # without the imaginary y-bond couplings the energy changes: they are not a no-op
+    Hr = H.real
+    assert abs(np.linalg.eigvalsh(Hr[np.ix_(keep, keep)])[0] - e) > 1e-6
 
 
 @pytest.mark.parametrize("name", ["heisenberg_1d_N6", "heisenberg_2d_6", "ising_1d_N6", "tj_1d_N4", "hubbard_2x2",
