@@ -78,10 +78,12 @@ struct Model : ModelBase {
     gqten::IndexVec<QNT> idx;
     int p = 0;
     for (int i = 0; i < nsites; ++i) {
+      if (nsect[i] < 1) throw std::invalid_argument("every site needs at least one sector");
       gqten::QNSectorVec<QNT> secs;
       for (int s = 0; s < nsect[i]; ++s) {
         QNT q;
         for (int k = 0; k < QNT::kWidth; ++k) q[k] = qns[p * QNT::kWidth + k];
+        if (dims[p] < 1) throw std::invalid_argument("sector dimensions must be positive");
         secs.emplace_back(q, (size_t)dims[p]);
         ++p;
       }
@@ -425,15 +427,13 @@ int gqmps2_model_create(int qn_width, int nsites, const int32_t *nsect, const in
   return guard([&] {
     if (!out || !nsect || !qns || !dims) throw std::invalid_argument("NULL argument");
     if (nsites < 2) throw std::invalid_argument("at least two sites are required");
-    auto h = new gqmps2_model_s;
+    auto h = std::make_unique<gqmps2_model_s>();
     if (qn_width == 1) h->m = std::make_unique<Model<gqten::QN<gqten::U1QNVal>>>(nsites, nsect, qns, dims);
     else if (qn_width == 2)
       h->m = std::make_unique<Model<gqten::QN<gqten::U1QNVal, gqten::U1QNVal>>>(nsites, nsect, qns, dims);
-    else {
-      delete h;
+    else
       throw std::invalid_argument("qn_width must be 1 or 2");
-    }
-    *out = h;
+    *out = h.release();
   });
 }
 

@@ -393,3 +393,20 @@ def test_oracle_ground_state_correlations_match_ed():
     e_corr = sum(mps_val({i: HM.SZ, i + 1: HM.SZ}) + 0.5 * mps_val({i: HM.SP, i + 1: HM.SM})
                  + 0.5 * mps_val({i: HM.SM, i + 1: HM.SP}) for i in range(n - 1))
     assert abs(e_corr - w[0]) < 1e-9
+
+
+def test_model_create_rejects_bad_site_tables():
+    """gqmps2_model_create returns an error status (raised here) for a
+    zero-dimensional sector, a site without sectors, an unsupported QN width
+    and a one-site chain, and a good table still builds afterwards."""
+    from mps2_amd import gqmps2 as G
+    up_dn = [((1,), 1), ((-1,), 1)]
+    bad = [([up_dn, [((1,), 0), ((-1,), 1)]], 1, "positive"),
+           ([up_dn, []], 1, "at least one sector"),
+           ([[((1, 0, 0), 1)], [((1, 0, 0), 1)]], 3, "qn_width"),
+           ([up_dn], 1, "two sites")]
+    for sites, qw, msg in bad:
+        with pytest.raises(RuntimeError, match=msg):
+            G.Model(sites, qw=qw)
+    m = G.Model([up_dn, up_dn])
+    m.close()
