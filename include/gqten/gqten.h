@@ -6,8 +6,10 @@
 // include/mps2_amd.h) so that code written against the reference compiles
 // unchanged.  Small tensors built element by element (operators, MPO tensors,
 // initial MPS) live on the host until an operation needs them on the device;
-// every operation runs on the GPU through the C ABI.  Real (GQTEN_Double)
-// element type only.
+// contractions, SVD, linear combinations, Dag and norms run on the GPU through
+// the C ABI, while the set-up edits of user code (element writes, Random,
+// Transpose, scalar multiples and sums) act on the host copy.  Real
+// (GQTEN_Double) element type only.
 #ifndef MPS2_AMD_GQTEN_H
 #define MPS2_AMD_GQTEN_H
 
@@ -256,6 +258,44 @@ class GQTensor {
   void Random(const QNT &div);
   void Transpose(const std::vector<size_t> &perm);
 
+  // gqten's tensor arithmetic: scalar multiples and sums of tensors with equal
+  // indexes (used to build site operators, e.g. -t * c_up in
+  // test_two_site_update_finite_vmps.cc:898-901).  Computed on the host copy.
+  GQTensor &operator*=(const double s) {
+    ToHost();
+    dev_.reset();
+    for (auto &kv : hblk_)
+      for (auto &x : kv.second) x *= s;
+    return *this;
+  }
+  GQTensor operator*(const double s) const {
+    GQTensor r(*this);
+    r *= s;
+    return r;
+  }
+  GQTensor operator-() const { return (*this) * -1.0; }
+  GQTensor &operator+=(const GQTensor &o) {
+    if (is_default_ || o.is_default_ || indexes_ != o.indexes_)
+      throw std::invalid_argument("GQTensor sum: the tensors must have the same indexes");
+    ToHost();
+    o.ToHost();
+    dev_.reset();
+    for (auto &kv : o.hblk_) {
+      auto it = hblk_.find(kv.first);
+      if (it == hblk_.end()) {
+        hblk_.emplace(kv.first, kv.second);
+        continue;
+      }
+      for (size_t e = 0; e < kv.second.size(); ++e) it->second[e] += kv.second[e];
+    }
+    return *this;
+  }
+  GQTensor operator+(const GQTensor &o) const {
+    GQTensor r(*this);
+    r += o;
+    return r;
+  }
+
   bool operator==(const GQTensor &o) const {
     if (is_default_ || o.is_default_) return is_default_ == o.is_default_;
     if (indexes_ != o.indexes_) return false;
@@ -496,6 +536,11 @@ void GQTensor<ElemT, QNT>::FromFlat(const std::vector<int32_t> &meta, const std:
 }
 
 // ---------------------------------------------------------------- free functions
+template <typename ElemT, typename QNT>
+GQTensor<ElemT, QNT> operator*(const double s, const GQTensor<ElemT, QNT> &t) {
+  return t * s;
+}
+
 template <typename T>
 void Contract(const T *a, const T *b, const std::vector<std::vector<size_t>> &axes, T *out) {
   std::vector<int32_t> xa(axes[0].begin(), axes[0].end()), xb(axes[1].begin(), axes[1].end());

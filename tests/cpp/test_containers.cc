@@ -6,6 +6,7 @@
 // Exit status 0 = all cases pass.
 #include <cstdio>
 #include <filesystem>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -160,6 +161,35 @@ int main() {
     Check(mpo[0] == a && mpo[1] == b && mpo[2] == c, "MPO element assignment");
     mpo[1] = c;
     Check(mpo[1] == c, "MPO element reassignment");
+  }
+  // ---- tensor arithmetic (scalar multiples and sums, as site operators are built)
+  {
+    U1QN q0({QNCard("N", U1QNVal(0))}), q1({QNCard("N", U1QNVal(1))});
+    IndexT o({QNSctT(q1, 2), QNSctT(q0, 1)}, OUT);
+    IndexT i = InverseIndex(o);
+    Tensor a({i, o}), b({i, o});
+    a({0, 0}) = 1.5;
+    a({1, 1}) = -2.0;
+    b({1, 1}) = 0.5;
+    b({2, 2}) = 4.0;   // a block that a does not hold
+    Tensor m = -3.0 * a, n = a * 2.0, neg = -a, s = a + b;
+    Check(m(0, 0) == -4.5 && m(1, 1) == 6.0 && m(2, 2) == 0.0, "scalar * tensor");
+    Check(n(0, 0) == 3.0 && n(1, 1) == -4.0, "tensor * scalar");
+    Check(neg(0, 0) == -1.5 && neg(1, 1) == 2.0, "unary minus");
+    Check(s(0, 0) == 1.5 && s(1, 1) == -1.5 && s(2, 2) == 4.0 && s(0, 1) == 0.0, "sum, including a new block");
+    Check(a(0, 0) == 1.5 && a(2, 2) == 0.0 && b(0, 0) == 0.0, "operands unchanged");
+    Tensor c = a;
+    c += c;
+    c *= 0.5;
+    Check(c == a, "c += c; c *= 0.5 gives c back");
+    Tensor w({o, o});
+    bool threw = false;
+    try {
+      a += w;
+    } catch (const std::invalid_argument &) {
+      threw = true;
+    }
+    Check(threw, "sum of tensors with different indexes is rejected");
   }
   std::printf("%s: %d checks, %d failures\n", g_fail ? "FAILED" : "PASSED", g_checks, g_fail);
   return g_fail ? 1 : 0;

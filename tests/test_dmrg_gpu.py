@@ -253,3 +253,19 @@ def test_cpp_tj_cases(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0 and "PASSED" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_hubbard_kondo_cases(tmp_path):
+    """test_two_site_update_finite_vmps.cc:790-951 and :1013-1113 (real cases)
+    as a C++ program over the drop-in headers: the Hubbard 2x2 cluster with
+    hopping operators built as scalar multiples of tensors, E0 =
+    -2.828427124746, and the Kondo double chain on a SiteVec of two different
+    local spaces, E0 = -3.180025784229132, each to the reference's 1e-10."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp",
+                       "test_hubbard_kondo_sweeps")
+    assert os.path.exists(exe), "tests/cpp is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
