@@ -14,6 +14,7 @@
 #define MPS2_AMD_GQMPS2_ALGORITHM_H
 
 #include <cmath>
+#include <cstdio>
 #include <functional>
 #include <iomanip>
 
@@ -94,6 +95,18 @@ struct UpdateRecord {
   size_t D, iters;
   double svd_time, env_time;
 };
+
+// Environment file names and removal (two_site_update_finite_vmps_impl.h:55-69):
+// "<temp_path>/<dir>env<blk_len>.gqten", dir "l" or "r".  The sweep here keeps
+// its environments in device memory and writes none of these files; the helpers
+// remain for callers that look for or clean up such files under temp_path.
+inline std::string GenEnvTenName(const std::string &dir, const long blk_len, const std::string temp_path) {
+  return temp_path + "/" + dir + kEnvFileBaseName + std::to_string(blk_len) + "." + kGQTenFileSuffix;
+}
+
+inline void RemoveFile(const std::string &file) {
+  if (std::remove(file.c_str())) perror(("Unable to delete " + file).c_str());
+}
 
 // two_site_update_finite_vmps_impl.h:43-52
 template <typename DTenT>

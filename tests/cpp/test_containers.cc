@@ -6,6 +6,7 @@
 // Exit status 0 = all cases pass.
 #include <cstdio>
 #include <filesystem>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -140,6 +141,14 @@ int main() {
         Check(IsIdOp(sv.id_ops[i]), "SiteVec identity operator (list)");
       }
     }
+  }
+  // ---- environment file names (two_site_update_finite_vmps_impl.h:55-69)
+  {
+    Check(GenEnvTenName("r", 7, ".temp") == ".temp/renv7.gqten", "GenEnvTenName right");
+    Check(GenEnvTenName("l", 12, "tmp/x") == "tmp/x/lenv12.gqten", "GenEnvTenName left");
+    std::ofstream("env_probe.gqten") << "x";
+    RemoveFile("env_probe.gqten");
+    Check(!IsPathExist("env_probe.gqten"), "RemoveFile deletes the file");
   }
   // ---- MPO
   {
