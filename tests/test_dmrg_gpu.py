@@ -269,3 +269,19 @@ def test_cpp_hubbard_kondo_cases(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0 and "PASSED" in r.stdout
+
+
+@pytest.mark.gpu
+def test_cpp_spin_2d_cases(tmp_path):
+    """test_two_site_update_finite_vmps.cc:249-345 (real cases) as a C++
+    program over the drop-in headers: 2D Heisenberg on six sites with the
+    vector form of AddTerm, E0 = -3.129385241572, and the Kitaev simple case
+    started from ExtendDirectRandomInitMps, E0 = -1.0, each to the reference's
+    1e-12."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp",
+                       "test_spin_2d_sweeps")
+    assert os.path.exists(exe), "tests/cpp is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
