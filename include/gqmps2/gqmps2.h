@@ -7,6 +7,7 @@
 #define GQMPS2_VERSION_MINOR 2
 #define GQMPS2_VERSION_PATCH 0
 
+#include "gqmps2/case_params_parser.h"
 #include "gqmps2/framework.h"
 #include "gqmps2/finite_mps.h"
 #include "gqmps2/finite_mps_measu.h"

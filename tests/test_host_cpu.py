@@ -410,3 +410,26 @@ def test_model_create_rejects_bad_site_tables():
             G.Model(sites, qw=qw)
     m = G.Model([up_dn, up_dn])
     m.close()
+
+
+def test_cpp_case_params_parser(tmp_path):
+    """test_case_params_parser.cc:17-53 as a C++ program over the headers,
+    compiled here and run on the CPU with the reference's fixture
+    (tests/golden/test-params.json, a copy of its testdata file), plus edge
+    cases of the JSON reader; a file without a CaseParams object exits with
+    status 1 and the reference's message."""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    lib = os.path.join(root, "mps2_amd")
+    cxx = shutil.which("g++") or shutil.which("c++")
+    exe = str(tmp_path / "test_case_params_parser")
+    subprocess.run([cxx, "-O1", "-std=c++17", "-Wall", "-I", os.path.join(root, "include"), "-o", exe,
+                    os.path.join(root, "tests", "cpp", "test_case_params_parser.cc"), "-L", lib, "-lmps2_amd",
+                    "-Wl,-rpath," + lib], check=True, timeout=300)
+    fixture = os.path.join(root, "tests", "golden", "test-params.json")
+    r = subprocess.run([exe, fixture], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout
+    r = subprocess.run([exe, fixture, "missing"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "CaseParams object not found, exit!" in r.stdout
