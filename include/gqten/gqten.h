@@ -23,6 +23,7 @@
 #include <map>
 #include <memory>
 #include <random>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -526,8 +527,53 @@ void GQTensor<ElemT, QNT>::ToFlat(std::vector<int32_t> &meta, std::vector<double
   for (auto &kv : hblk_) data.insert(data.end(), kv.second.begin(), kv.second.end());
 }
 
+// Structure check of flat input from outside the engine (tensor files, FFI
+// callers) before it is parsed: header, legs, sectors, block coordinates in
+// range and not repeated, and exactly as many elements as the blocks hold.
+template <typename QNT>
+inline void CheckFlat(const std::vector<int32_t> &m, const size_t ndata) {
+  auto bad = [](const char *what) { throw std::runtime_error(std::string("malformed tensor data: ") + what); };
+  if (m.size() < 3) bad("header");
+  const int64_t rank = m[0], qw = m[1], nblk = m[2];
+  if (rank < 0 || rank > 64) bad("rank");
+  if (qw != QNT::kWidth) bad("QN width");
+  if (nblk < 0) bad("block count");
+  size_t p = 3;
+  std::vector<std::vector<int64_t>> dims((size_t)rank);
+  for (int64_t l = 0; l < rank; ++l) {
+    if (p + 2 > m.size()) bad("legs");
+    if (m[p] != 0 && m[p] != 1) bad("direction");
+    const int64_t ns = m[p + 1];
+    p += 2;
+    if (ns < 0 || (int64_t)((m.size() - p) / (size_t)(qw + 1)) < ns) bad("sectors");
+    for (int64_t s = 0; s < ns; ++s) {
+      p += (size_t)qw;
+      if (m[p] < 0) bad("sector dimension");
+      dims[l].push_back(m[p++]);
+    }
+  }
+  if ((int64_t)(m.size() - p) != nblk * rank) bad("block list");
+  std::set<std::vector<int32_t>> seen;
+  size_t total = 0;
+  for (int64_t b = 0; b < nblk; ++b) {
+    std::vector<int32_t> k(m.begin() + p, m.begin() + p + rank);
+    p += (size_t)rank;
+    size_t sz = 1;
+    for (int64_t l = 0; l < rank; ++l) {
+      if (k[l] < 0 || k[l] >= (int64_t)dims[l].size()) bad("block coordinate");
+      sz *= (size_t)dims[l][k[l]];
+      if (sz > ndata) bad("data size");
+    }
+    if (!seen.insert(k).second) bad("repeated block");
+    total += sz;
+    if (total > ndata) bad("data size");
+  }
+  if (total != ndata) bad("data size");
+}
+
 template <typename ElemT, typename QNT>
 void GQTensor<ElemT, QNT>::FromFlat(const std::vector<int32_t> &meta, const std::vector<double> &data) {
+  CheckFlat<QNT>(meta, data.size());
   is_default_ = false;
   dev_.reset();
   ParseLegs(meta);

@@ -195,9 +195,11 @@ struct Model : ModelBase {
     std::vector<int32_t> m(meta, meta + 3);
     // size the meta: rank, qw, nblk then legs then coords
     int rank = meta[0], qw = meta[1], nb = meta[2];
+    if (rank < 0 || rank > 64 || qw != QNT::kWidth || nb < 0) throw std::invalid_argument("malformed tensor meta");
     int p = 3;
     for (int l = 0; l < rank; ++l) {
       int ns = meta[p + 1];
+      if (ns < 0) throw std::invalid_argument("malformed tensor meta");
       p += 2 + ns * (qw + 1);
     }
     p += nb * rank;
@@ -217,7 +219,12 @@ struct Model : ModelBase {
       }
       for (int b = 0; b < nb; ++b) {
         int64_t sz = 1;
-        for (int l = 0; l < rank; ++l) sz *= dims[l][meta[q++]];
+        for (int l = 0; l < rank; ++l) {
+          const int32_t c = meta[q++];
+          if (c < 0 || c >= (int32_t)dims[l].size()) throw std::invalid_argument("tensor meta: block coordinate out of range");
+          if (dims[l][c] < 0) throw std::invalid_argument("tensor meta: negative sector dimension");
+          sz *= dims[l][c];
+        }
         total += sz;
       }
     }

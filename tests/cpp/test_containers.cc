@@ -150,6 +150,55 @@ int main() {
     RemoveFile("env_probe.gqten");
     Check(!IsPathExist("env_probe.gqten"), "RemoveFile deletes the file");
   }
+  // ---- corrupt flat data (tensor files, FFI input) is rejected before parsing
+  {
+    U1QN q0({QNCard("N", U1QNVal(0))}), q1({QNCard("N", U1QNVal(1))});
+    IndexT o({QNSctT(q0, 2), QNSctT(q1, 3)}, OUT);
+    Tensor t({InverseIndex(o), o});
+    t.Random(q0);
+    std::vector<int32_t> meta;
+    std::vector<double> data;
+    t.ToFlat(meta, data);
+    Tensor back;
+    back.FromFlat(meta, data);
+    Check(back == t, "flat round trip");
+    const size_t nblk = (size_t)meta[2], coords = meta.size() - 2 * nblk;
+    Check(nblk == 2 && data.size() == 13, "flat layout of the probe tensor");
+    auto rejects = [&](std::vector<int32_t> m, std::vector<double> d) {
+      Tensor r;
+      try {
+        r.FromFlat(m, d);
+      } catch (const std::runtime_error &) {
+        return true;
+      }
+      return false;
+    };
+    auto d_short = data;
+    d_short.pop_back();
+    auto d_long = data;
+    d_long.push_back(0.0);
+    Check(rejects(meta, d_short) && rejects(meta, d_long), "element count differs from the blocks");
+    auto m = meta;
+    m[coords] = 2;  // sector 2 of a two-sector leg
+    Check(rejects(m, data), "block coordinate out of range");
+    m = meta;
+    m[coords + 2] = m[coords];
+    m[coords + 3] = m[coords + 1];
+    Check(rejects(m, data), "repeated block");
+    m = meta;
+    m.pop_back();
+    Check(rejects(m, data), "truncated block list");
+    m = meta;
+    m[3] = 7;
+    Check(rejects(m, data), "bad direction");
+    m = meta;
+    m[1] = 2;
+    Check(rejects(m, data), "QN width");
+    m = meta;
+    m[0] = 3;
+    Check(rejects(m, data), "rank beyond the legs present");
+    Check(rejects(std::vector<int32_t>{1, 1}, data), "header too short");
+  }
   // ---- MPO
   {
     U1QN qn({QNCard("N", U1QNVal(0))});
