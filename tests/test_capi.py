@@ -50,3 +50,40 @@ def test_host_layer_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) <= set(gqmps2.SIGNATURES)
+
+
+def test_product_does_not_use_the_oracle():
+    """The oracle is test infrastructure: nothing in the package or the public
+    headers refers to it, the built libraries do not link it, and the only
+    uses outside tests/ are bench.py's cpu_baseline and the entry module's
+    smoke check."""
+    import ast
+    import subprocess
+    hits = []
+    for d in ("mps2_amd", "include"):
+        for dirpath, _, files in os.walk(os.path.join(ROOT, d)):
+            for f in files:
+                if f.endswith((".py", ".cc", ".h", ".hip", ".cpp")) or f == "Makefile":
+                    if "oracle" in open(os.path.join(dirpath, f), errors="replace").read():
+                        hits.append(os.path.join(dirpath, f))
+    assert not hits, hits
+    for so in ("libmps2_amd.so", "libgqmps2_amd.so"):
+        path = os.path.join(ROOT, "mps2_amd", so)
+        if not os.path.exists(path):
+            pytest.fail(f"mps2_amd/{so} is not built (run __graft_entry__.build())")
+        dyn = subprocess.run(["readelf", "-d", path], capture_output=True, text=True, check=True).stdout
+        assert "oracle" not in dyn, (so, [ln for ln in dyn.splitlines() if "oracle" in ln])
+    for script, allowed in (("bench.py", {"cpu_baseline"}), ("__graft_entry__.py", {"smoke"})):
+        tree = ast.parse(open(os.path.join(ROOT, script)).read())
+        users = set()
+        for fn in ast.walk(tree):
+            if isinstance(fn, ast.FunctionDef):
+                for node in ast.walk(fn):
+                    if isinstance(node, ast.ImportFrom) and (node.module or "").startswith("oracle"):
+                        users.add(fn.name)
+                    if isinstance(node, ast.Import) and any(a.name.startswith("oracle") for a in node.names):
+                        users.add(fn.name)
+        top = [n for n in tree.body if isinstance(n, (ast.Import, ast.ImportFrom))
+               and ("oracle" in (getattr(n, "module", None) or "") or any("oracle" in a.name for a in n.names))]
+        assert not top, (script, "imports the oracle at module level")
+        assert users <= allowed, (script, users)
