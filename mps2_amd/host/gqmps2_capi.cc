@@ -109,6 +109,24 @@ struct Model : ModelBase {
   }
 
   void add_term(double coef, int nops, const int32_t *oid, const int64_t *st, int ninst, const int32_t *inst) override {
+    // The generator asserts these; a C caller gets an error status instead.
+    if (nops < 1) throw std::invalid_argument("a term needs at least one operator");
+    if (ninst != 0 && (nops < 2 || (ninst != nops - 1 && ninst != nops)))
+      throw std::invalid_argument("insertion operators: one per gap, or one per gap plus one for the tail");
+    if (ninst != 0 && !inst) throw std::invalid_argument("NULL insertion operator list");
+    auto space_of = [&](int32_t id) -> const gqten::Index<QNT> & { return sites->sites[op_site.at(id)]; };
+    for (int k = 0; k < nops; ++k) {
+      if (st[k] < 0 || (size_t)st[k] >= N) throw std::out_of_range("term site out of range");
+      if (k > 0 && st[k] < st[k - 1]) throw std::invalid_argument("term sites must be in ascending order");
+      if (space_of(oid[k]) != sites->sites[st[k]])
+        throw std::invalid_argument("operator built for another local space than its site's");
+    }
+    for (int k = 0; k < ninst; ++k) {
+      const int64_t from = st[k] + 1, to = k + 1 < nops ? st[k + 1] : (int64_t)N;
+      for (int64_t j = from; j < to; ++j)
+        if (space_of(inst[k]) != sites->sites[j])
+          throw std::invalid_argument("insertion operator built for another local space than its site's");
+    }
     std::vector<TenT> o;
     std::vector<size_t> s;
     for (int k = 0; k < nops; ++k) {

@@ -433,3 +433,35 @@ def test_cpp_case_params_parser(tmp_path):
     assert r.returncode == 0 and "PASSED" in r.stdout
     r = subprocess.run([exe, fixture, "missing"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "CaseParams object not found, exit!" in r.stdout
+
+
+def test_model_add_term_rejects_bad_terms():
+    """gqmps2_model_add_term returns an error status (raised here) where the
+    generator would assert: sites out of order or out of range, an insertion
+    count that is neither one per gap nor one per gap plus the tail, and an
+    operator or insertion operator built for another local space than the
+    sites it lands on; good terms are still accepted afterwards."""
+    import ctypes as C
+    from mps2_amd import gqmps2 as G
+    spin = [((1,), 1), ((-1,), 1)]
+    m = G.Model([spin, spin, [((0,), 4)], spin])
+    SZ, F = np.diag([0.5, -0.5]), np.diag([-1.0, -1.0])
+    bad = [(lambda: m.term(1.0, [SZ, SZ], [1, 0]), "ascending"),
+           (lambda: m.term(1.0, [SZ, SZ], [0, 7]), "out of range"),
+           (lambda: m.term(1.0, [SZ, SZ, SZ], [0, 1, 3], inst=[F]), "one per gap"),
+           (lambda: m.term(1.0, [SZ, SZ], [0, 3], inst=[F]), "insertion operator built")]
+    for f, msg in bad:
+        with pytest.raises(RuntimeError, match=msg):
+            f()
+    big = m.op(2, np.eye(4))                       # registered for the 4-dimensional site
+    ids = np.asarray([m.op(0, SZ), big], dtype=np.int32)
+    st = np.asarray([0, 1], dtype=np.int64)
+    with pytest.raises(RuntimeError, match="local space"):
+        G.check(G.lib().gqmps2_model_add_term(m.h, 1.0, 2, ids.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              st.ctypes.data_as(C.POINTER(C.c_int64)), 0, None))
+    m.term(1.0, [SZ, SZ], [0, 1])
+    m.term(1.0, [SZ, np.eye(4)], [1, 2])
+    with pytest.raises(RuntimeError, match="insertion operator built"):
+        m.term(1.0, [SZ, SZ], [0, 1], inst=[F, F])     # tail string over the 4-dimensional site 2
+    m.term(1.0, [SZ, np.eye(4)], [0, 2], inst=[F, F])  # gap site 1 and tail site 3 are spins
+    m.close()
