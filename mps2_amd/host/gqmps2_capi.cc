@@ -153,6 +153,8 @@ struct Model : ModelBase {
   }
 
   void export_mpo(int site, std::vector<int32_t> &meta, std::vector<double> &data) override {
+    if (mpo.size() == 0) throw std::runtime_error("no MPO yet (call gqmps2_model_gen_mpo)");
+    if (site < 0 || (size_t)site >= mpo.size()) throw std::out_of_range("site out of range");
     mpo[site].ToFlat(meta, data);
   }
 
@@ -205,12 +207,13 @@ struct Model : ModelBase {
   }
 
   void export_mps(int site, std::vector<int32_t> &meta, std::vector<double> &data) override {
+    if (site < 0 || (size_t)site >= N) throw std::out_of_range("site out of range");
     if (!cm()(site)) throw std::runtime_error("MPS tensor not resident");
     cm()[site].ToFlat(meta, data);
   }
 
   void import_mps(int site, const int32_t *meta, const double *data) override {
-    std::vector<int32_t> m(meta, meta + 3);
+    if (site < 0 || (size_t)site >= N) throw std::out_of_range("site out of range");
     // size the meta: rank, qw, nblk then legs then coords
     int rank = meta[0], qw = meta[1], nb = meta[2];
     if (rank < 0 || rank > 64 || qw != QNT::kWidth || nb < 0) throw std::invalid_argument("malformed tensor meta");
@@ -246,6 +249,7 @@ struct Model : ModelBase {
         total += sz;
       }
     }
+    if (total > 0 && !data) throw std::invalid_argument("NULL data for a tensor with elements");
     std::vector<double> d(data, data + total);
     TenT t;
     t.FromFlat(mm, d);
@@ -495,6 +499,7 @@ int gqmps2_model_gen_mpo(gqmps2_model m, int64_t *bond_dims) {
 
 int gqmps2_model_mpo_export_sizes(gqmps2_model m, int site, int64_t *nmeta, int64_t *ndata) {
   return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
     std::vector<int32_t> meta;
     std::vector<double> data;
     m->m->export_mpo(site, meta, data);
@@ -505,6 +510,7 @@ int gqmps2_model_mpo_export_sizes(gqmps2_model m, int site, int64_t *nmeta, int6
 
 int gqmps2_model_mpo_export(gqmps2_model m, int site, int32_t *meta, double *data) {
   return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
     std::vector<int32_t> mt;
     std::vector<double> dt;
     m->m->export_mpo(site, mt, dt);
@@ -516,13 +522,14 @@ int gqmps2_model_mpo_export(gqmps2_model m, int site, int32_t *meta, double *dat
 int gqmps2_mps_init(gqmps2_model m, int nsets, const int64_t *stat_labs, int64_t enlarged_dim, uint64_t seed,
                     int centralize) {
   return guard([&] {
-    if (!m || !stat_labs || nsets < 1) throw std::invalid_argument("bad arguments");
+    if (!m || !stat_labs || nsets < 1 || enlarged_dim < 0) throw std::invalid_argument("bad arguments");
     m->m->mps_init(nsets, stat_labs, enlarged_dim, seed, centralize);
   });
 }
 
 int gqmps2_mps_export_sizes(gqmps2_model m, int site, int64_t *nmeta, int64_t *ndata) {
   return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
     std::vector<int32_t> meta;
     std::vector<double> data;
     m->m->export_mps(site, meta, data);
@@ -533,6 +540,7 @@ int gqmps2_mps_export_sizes(gqmps2_model m, int site, int64_t *nmeta, int64_t *n
 
 int gqmps2_mps_export(gqmps2_model m, int site, int32_t *meta, double *data) {
   return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
     std::vector<int32_t> mt;
     std::vector<double> dt;
     m->m->export_mps(site, mt, dt);
@@ -549,11 +557,17 @@ int gqmps2_mps_import(gqmps2_model m, int site, const int32_t *meta, const doubl
 }
 
 int gqmps2_mps_checkpoint(gqmps2_model m) {
-  return guard([&] { m->m->checkpoint(); });
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    m->m->checkpoint();
+  });
 }
 
 int gqmps2_mps_restore(gqmps2_model m) {
-  return guard([&] { m->m->restore(); });
+  return guard([&] {
+    if (!m) throw std::invalid_argument("NULL model");
+    m->m->restore();
+  });
 }
 
 int gqmps2_dmrg(gqmps2_model m, int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *dmax_schedule,

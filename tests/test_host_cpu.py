@@ -465,3 +465,40 @@ def test_model_add_term_rejects_bad_terms():
         m.term(1.0, [SZ, SZ], [0, 1], inst=[F, F])     # tail string over the 4-dimensional site 2
     m.term(1.0, [SZ, np.eye(4)], [0, 2], inst=[F, F])  # gap site 1 and tail site 3 are spins
     m.close()
+
+
+def test_model_entry_points_check_handles_and_sites():
+    """NULL model handles, MPO export before gen_mpo, and site numbers outside
+    the chain give error statuses from the model-level C ABI."""
+    import ctypes as C
+    from mps2_amd import gqmps2 as G
+    L = G.lib()
+    i32p, i64p, dp = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_double)
+    n = C.c_int64()
+    for rc in (L.gqmps2_mps_checkpoint(None), L.gqmps2_mps_restore(None),
+               L.gqmps2_mps_export_sizes(None, 0, C.byref(n), C.byref(n)),
+               L.gqmps2_model_mpo_export_sizes(None, 0, C.byref(n), C.byref(n)),
+               L.gqmps2_mps_export(None, 0, None, None), L.gqmps2_model_mpo_export(None, 0, None, None)):
+        assert rc != 0
+        assert "NULL" in L.gqmps2_last_error().decode()
+    spin = [((1,), 1), ((-1,), 1)]
+    m = G.Model([spin] * 4)
+    rc = L.gqmps2_model_mpo_export_sizes(m.h, 0, C.byref(n), C.byref(n))
+    assert rc != 0 and "gen_mpo" in L.gqmps2_last_error().decode()
+    m.term(1.0, [np.diag([0.5, -0.5])] * 2, [0, 1])
+    m.gen_mpo()
+    for site in (-1, 4):
+        assert L.gqmps2_model_mpo_export_sizes(m.h, site, C.byref(n), C.byref(n)) != 0
+        assert L.gqmps2_mps_export_sizes(m.h, site, C.byref(n), C.byref(n)) != 0
+    m.init_mps([[0, 1, 0, 1]], centralize=False)
+    meta = np.zeros(64, dtype=np.int32)
+    data = np.zeros(64)
+    assert L.gqmps2_mps_export(m.h, 4, meta.ctypes.data_as(i32p), data.ctypes.data_as(dp)) != 0
+    assert L.gqmps2_mps_import(m.h, 9, meta.ctypes.data_as(i32p), data.ctypes.data_as(dp)) != 0
+    labs = np.zeros(4, dtype=np.int64)
+    assert L.gqmps2_mps_init(m.h, 1, labs.ctypes.data_as(i64p), -1, 0, 0) != 0
+    # a good export/import round trip still works
+    t = m.mps(1)
+    m.set_mps(1, t)
+    assert np.array_equal(m.mps(1).dense(), t.dense())
+    m.close()
