@@ -108,6 +108,10 @@ class Model:
 
     def op(self, site, dense):
         dense = np.ascontiguousarray(dense, dtype=np.float64)
+        # the C side reads d x d elements for the site's local dimension d
+        if 0 <= site < self.N and dense.shape != (self.site_dims[site],) * 2:
+            d = self.site_dims[site]
+            raise ValueError(f"operator for site {site} must be {d}x{d}, got shape {dense.shape}")
         key = (site, dense.tobytes())
         if key in self._ops:
             return self._ops[key]

@@ -459,6 +459,8 @@ def test_model_add_term_rejects_bad_terms():
     with pytest.raises(RuntimeError, match="local space"):
         G.check(G.lib().gqmps2_model_add_term(m.h, 1.0, 2, ids.ctypes.data_as(C.POINTER(C.c_int32)),
                                               st.ctypes.data_as(C.POINTER(C.c_int64)), 0, None))
+    with pytest.raises(ValueError, match="must be 4x4"):
+        m.op(2, np.eye(2))                         # would be read as 16 elements
     m.term(1.0, [SZ, SZ], [0, 1])
     m.term(1.0, [SZ, np.eye(4)], [1, 2])
     with pytest.raises(RuntimeError, match="insertion operator built"):
