@@ -5,19 +5,30 @@ Workload: J1-J2 Heisenberg model on a width-4 cylinder (4x24 = 96 sites,
 periodic in y, snake ordering, J1 = 1, J2 = 0.5), U(1) Sz = 0, MPO from the
 MPOGenerator, seeded random initial MPS (extended direct-product state).
 
-Flow (one GPU):
-  1. ramp sweeps (untimed, FP64) with Dmax 256 -> 512 -> 1024 -> 2048,
-  2. `--warmup` untimed FP64 sweeps at Dmax = 4096, then a device checkpoint,
-  3. `--steps` timed FP64 sweeps from the checkpoint, then the same number
-     of timed mixed-precision sweeps from the same checkpoint,
-  4. the centre-bond matvec benchmarked with HIP events (FP64 and mixed),
-  5. the CPU FP64 oracle timed on that same centre-bond matvec (host cores).
-A step is one full sweep (2(N-1) two-site updates) at Dmax = 4096.
-Prints one JSON line.
+A step is one centre-bond effective-Hamiltonian matvec H*psi at D = 4096 in
+the mixed form (FP32 MFMA operands and accumulation): the operation whose
+TFLOP/s the metric quotes.  Flow (one GPU):
+  1. ramp sweeps (untimed, FP64) with Dmax 256 -> 512 -> 1024 -> 2048 -> 4096,
+     then a device checkpoint of the D = 4096 state;
+  2. one full FP64 sweep and one full mixed sweep from that checkpoint
+     (sweep wall-time, phases, energies and kept dimensions of both);
+  3. the centre-bond operator bound once per precision; `--warmup` untimed
+     steps, then exactly `--steps` timed steps (host clock, the stream
+     synchronised on both sides), FP64 and mixed;
+  4. per-stage HIP-event timing of the matvec; the dominant grouped-GEMM
+     launch as the roofline kernel;
+  5. FP64 vs mixed Lanczos from one random start at the centre bond, and one
+     GPU's share of an 8-way owner-computes split of the dominant stage;
+  6. the CPU FP64 oracle on the host cores: the same centre-bond matvec (all
+     threads and one thread), the SVD of the same state, and the full
+     L=16 Heisenberg run (D=64, 4 sweeps).
+Optional parts (5, 6) are trimmed when the wall-clock budget runs short; the
+JSON line is always printed.
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -28,34 +39,95 @@ sys.path.insert(0, ROOT)
 
 METRIC = "sweep wall-time + H·ψ TFLOP/s at D=4096: FP64 vs mixed, |ΔE|/|E|≤1e-10"
 PEAK = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense matrix-core TFLOP/s
+T_START = time.perf_counter()
 
 
 def log(*a):
-    print("[bench]", *a, file=sys.stderr, flush=True)
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(tensors, flops, budget_s):
-    """Time the CPU FP64 oracle on the same centre-bond matvec."""
+def elapsed():
+    return time.perf_counter() - T_START
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(tensors, flops, budget_s, gpu_fp64_matvecs_per_update, n_updates):
+    """The CPU FP64 oracle on the host cores, bounded by budget_s seconds."""
     from oracle import oracle_py as O
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 64))
-    O.set_threads(cores)
+    from mps2_amd import hamiltonians as HM
+    from mps2_amd.flat import BTensor
+    t_begin = time.perf_counter()
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(aff, 64))
     L, W1, W2, R, psi = [t.get() if t is not None else None for t in tensors]
     hs = [O.OTen.from_bt(x) for x in (L, W1, W2, R, psi)]
     lib = O.lib()
-    reps, t_total = 0, 0.0
-    while True:
+
+    def matvec_s(threads, reps_max, budget):
+        O.set_threads(threads)
+        reps, tot = 0, 0.0
+        while True:
+            t0 = time.perf_counter()
+            r = lib.oracle_heff_mul(hs[0].h, hs[1].h, hs[2].h, hs[3].h, hs[4].h, b"cent")
+            tot += time.perf_counter() - t0
+            O.OTen(r)
+            reps += 1
+            if tot >= budget or reps >= reps_max:
+                return tot / reps, reps
+
+    t_mv, reps = matvec_s(cores, 3, 0.25 * budget_s)
+    log(f"cpu: centre-bond matvec {t_mv:.2f} s on {cores} threads ({reps} reps)")
+    out = dict(value=flops / t_mv / 1e12, unit="TFLOP/s", cores=cores, kind="port",
+               sample=f"{reps} centre-bond H*psi matvec(s) of the D=4096 state, same tensors as the GPU steps, "
+                      f"FP64, OpenMP over {cores} host threads; {t_mv:.2f} s each",
+               cpu_model=cpu_model(), nproc=os.cpu_count(), affinity_cores=aff)
+    left = budget_s - (time.perf_counter() - t_begin)
+    if left > 3 * t_mv * cores ** 0.5:
+        t1, _ = matvec_s(1, 1, 0)
+        out["matvec_1thread"] = dict(value=flops / t1 / 1e12, unit="TFLOP/s", seconds=round(t1, 2))
+        log(f"cpu: centre-bond matvec {t1:.2f} s on 1 thread")
+    O.set_threads(cores)
+    left = budget_s - (time.perf_counter() - t_begin)
+    t_svd = None
+    if left > 40:
         t0 = time.perf_counter()
-        r = lib.oracle_heff_mul(hs[0].h, hs[1].h, hs[2].h, hs[3].h, hs[4].h, b"cent")
-        t_total += time.perf_counter() - t0
-        O.OTen(r)
-        reps += 1
-        if t_total >= budget_s or reps >= 5:
-            break
-    t = t_total / reps
-    return dict(value=flops / t / 1e12, unit="TFLOP/s", cores=cores, kind="port",
-                sample=f"{reps} centre-bond H*psi matvec(s) of the D=4096 state (same tensors as the GPU "
-                       f"measurement), {t:.2f} s each, OpenMP over {cores} host threads")
+        O.svd(psi, 2, (0,), 0.0, 1, 4096)
+        t_svd = time.perf_counter() - t0
+        out["svd_centre_s"] = round(t_svd, 2)
+        log(f"cpu: SVD of the centre-bond state {t_svd:.1f} s")
+    if t_svd is not None and gpu_fp64_matvecs_per_update:
+        per_update = gpu_fp64_matvecs_per_update * t_mv + t_svd
+        out["sweep_extrapolated_s"] = dict(
+            value=round(per_update * n_updates, 1), per_update_s=round(per_update, 2), updates=n_updates,
+            label="extrapolated: the centre bond's update (the FP64 sweep's mean matvecs per update x the CPU "
+                  "matvec, plus the CPU SVD of that state; environment growth left out) x the sweep's "
+                  "update count; an upper bound for the matvec part since edge bonds are smaller")
+    left = budget_s - (time.perf_counter() - t_begin)
+    if left > 10:
+        m = HM.heisenberg_chain(16)
+        m.init_mps([[i % 2 for i in range(16)], [(i + 1) % 2 for i in range(16)]], enlarged_dim=4, seed=0x5EED,
+                   centralize=False)
+        raw = [m.mps(i) for i in range(m.N)]
+        t0 = time.perf_counter()
+        mps = O.centralize0(raw)
+        h = mps[0]
+        n = h.norm()
+        mps[0] = BTensor(h.legs, h.qw, {c: b / n for c, b in h.blocks.items()})
+        r = O.dmrg(mps, [m.mpo(i) for i in range(m.N)], 4, 1, 64, 0.0, 1e-10, 200)
+        out["heisenberg_l16_full_run"] = dict(seconds=round(time.perf_counter() - t0, 2), energy=r["energy"],
+                                              config="Heisenberg OBC L=16, Sz=0, Dmax=64, 4 sweeps, Lanczos 1e-10")
+        log(f"cpu: L=16 Heisenberg full run {out['heisenberg_l16_full_run']['seconds']} s, E={r['energy']:.12f}")
+    return out
 
 
 def measured_traffic(args):
@@ -70,11 +142,22 @@ def measured_traffic(args):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def timed_steps(engine, ctxh, op, warmup, steps):
+    """`warmup` untimed then exactly `steps` timed applications of op; the
+    context stream is synchronised on both sides of the timed region."""
+    op.run(warmup)
+    engine.check(engine.lib().mps2_ctx_sync(ctxh))
+    t0 = time.perf_counter()
+    op.run(steps)
+    engine.check(engine.lib().mps2_ctx_sync(ctxh))
+    return time.perf_counter() - t0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1, help="timed sweeps per precision")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed sweeps at Dmax before timing")
+    ap.add_argument("--steps", type=int, default=20, help="timed centre-bond matvecs per precision")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed centre-bond matvecs before timing")
     ap.add_argument("--lx", type=int, default=24)
     ap.add_argument("--width", type=int, default=4)
     ap.add_argument("--j2", type=float, default=0.5)
@@ -82,19 +165,23 @@ def main():
     ap.add_argument("--ramp", default="256,512,1024,2048")
     ap.add_argument("--lanczos-error", type=float, default=1e-10)
     ap.add_argument("--lanczos-max-iter", type=int, default=200)
-    ap.add_argument("--matvec-reps", type=int, default=10)
-    ap.add_argument("--cpu-budget-s", type=float, default=20.0)
+    ap.add_argument("--matvec-reps", type=int, default=10, help="matvecs of the per-stage timing pass")
+    ap.add_argument("--budget-s", type=float, default=420.0,
+                    help="wall-clock target: optional parts are trimmed to print within it")
+    ap.add_argument("--cpu-budget-s", type=float, default=60.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
     ap.add_argument("--profile", action="store_true",
-                    help="profiler runs: skip the timed sweeps and the CPU baseline (state, matvec bench and the "
-                         "calibration copy as in the default run)")
+                    help="profiler runs: state, steps, per-stage timing and the calibration copy only")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the DMRG sweep of this build runs on a single MI355X "
               "(multi-GPU sharding is a design note, not an implemented path). Run with --gpus 1.",
               file=sys.stderr)
+        sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
         sys.exit(2)
 
     from mps2_amd import engine
@@ -106,68 +193,61 @@ def main():
         sys.exit(1)
     Lx, W = args.lx, args.width
     N = Lx * W
-    t0 = time.perf_counter()
     model = HM.j1j2_cylinder(Lx, W, 1.0, args.j2)
     w = int(max(model.bond_dims))
-    log(f"model {W}x{Lx} J1-J2 (J2={args.j2}): N={N}, MPO bond dim max {w}, built in {time.perf_counter() - t0:.1f}s")
+    log(f"model {W}x{Lx} J1-J2 (J2={args.j2}): N={N}, MPO bond dim max {w}")
     labs = [[i % 2 for i in range(N)], [(i + 1) % 2 for i in range(N)]]
     model.init_mps(labs, enlarged_dim=8, seed=args.seed)
     ramp = [int(x) for x in args.ramp.split(",") if x]
-    ramp = [d for d in ramp if d < args.dmax]
+    ramp = [d for d in ramp if d < args.dmax] + [args.dmax]
     le, lm = args.lanczos_error, args.lanczos_max_iter
     for d in ramp:
         t0 = time.perf_counter()
         r = model.dmrg(1, 1, d, 0.0, le, lm)
         log(f"ramp sweep Dmax={d}: E={r['energy']:.12f} maxD={r['dims'].max()} in {time.perf_counter() - t0:.1f}s")
-    for k in range(args.warmup):
-        t0 = time.perf_counter()
-        r = model.dmrg(1, 1, args.dmax, 0.0, le, lm)
-        log(f"warmup sweep {k + 1} at Dmax={args.dmax}: E={r['energy']:.12f} maxD={r['dims'].max()} "
-            f"in {time.perf_counter() - t0:.1f}s")
     model.checkpoint()
+    ctxh = G.lib().gqmps2_engine_ctx()
 
     res = {}
     for name, prec in (() if args.profile else (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED))):
         model.restore()
-        engine.check(engine.lib().mps2_ctx_sync(G.lib().gqmps2_engine_ctx()))
+        engine.check(engine.lib().mps2_ctx_sync(ctxh))
         G.engine_reset_stats()
         t0 = time.perf_counter()
-        r = model.dmrg(args.steps, 1, args.dmax, 0.0, le, lm, precision=prec)
+        r = model.dmrg(1, 1, args.dmax, 0.0, le, lm, precision=prec)
         wall = time.perf_counter() - t0
         st = G.engine_stats()
-        cnt = engine.ctx_counters(G.lib().gqmps2_engine_ctx())
-        log(f"{name}: counters {cnt}")
+        cnt = engine.ctx_counters(ctxh)
         res[name] = dict(r=r, wall=wall, stats=st, counters=cnt)
-        ph = r["phase_times"].sum(axis=0) / args.steps
+        ph = r["phase_times"].sum(axis=0)
         res[name]["phase"] = dict(lanczos=round(float(ph[0]), 3), svd=round(float(ph[1]), 3), env=round(float(ph[2]), 3))
-        log(f"{name}: phase seconds per sweep {res[name]['phase']}")
-        log(f"{name}: {args.steps} sweep(s) {wall:.2f}s, E={r['energy']:.14f}, maxD={r['dims'].max()}, "
-            f"mean Lanczos iters {r['iters'].mean():.1f}, gemm flops {st['gemm_flops']:.3e}, "
+        log(f"{name} sweep {wall:.2f}s: E={r['energy']:.14f}, maxD={r['dims'].max()}, phases {res[name]['phase']}, "
+            f"mean Lanczos iters {r['iters'].mean():.2f}, matvecs fp64 {cnt['matvec_fp64']:.0f} "
+            f"fp32 {cnt['matvec_fp32']:.0f}, syncs {cnt['syncs']:.0f}, gemm flops {st['gemm_flops']:.3e}, "
             f"peak HBM {st['peak_bytes'] / 2**30:.1f} GiB")
-
-    if res:
-        e64, emx = res["fp64"]["r"]["energy"], res["mixed"]["r"]["energy"]
-        de_rel = abs(emx - e64) / abs(e64)
-        upd_rel = float(max(abs(a - b) / abs(b) for a, b in zip(res["mixed"]["r"]["energies"],
-                                                               res["fp64"]["r"]["energies"])))
-        dims_equal = bool((res["mixed"]["r"]["dims"] == res["fp64"]["r"]["dims"]).all())
 
     centre = N // 2 - 1
     t0 = time.perf_counter()
     mb = model.bench_bond(centre, warmup=2, reps=args.matvec_reps)
-    log(f"centre-bond matvec bench ({time.perf_counter() - t0:.1f}s): fp64 {mb['fp64']['ms']:.3f} ms "
-        f"({mb['fp64']['flops'] / mb['fp64']['ms'] / 1e9:.2f} TFLOP/s), mixed {mb['mixed']['ms']:.3f} ms "
-        f"({mb['mixed']['flops'] / mb['mixed']['ms'] / 1e9:.2f} TFLOP/s), stage ms fp64 {mb['fp64']['stage_ms']} "
+    log(f"centre-bond per-stage timing ({time.perf_counter() - t0:.1f}s): stage ms fp64 {mb['fp64']['stage_ms']} "
         f"mixed {mb['mixed']['stage_ms']}")
+    L_, W1_, W2_, R_, psi_ = model.bench_tensors()
 
-    # attainable copy bandwidth; the same kernel scales the profiler's byte counters
+    # the steps: exactly --steps centre-bond matvecs per precision
+    steps = {}
+    for name, prec in (("mixed", engine.PREC_MIXED), ("fp64", engine.PREC_FP64)):
+        op = engine.HeffOp(L_, W1_, W2_, R_, psi_, "cent", prec)
+        dt = timed_steps(engine, ctxh, op, args.warmup, args.steps)
+        steps[name] = dict(seconds=dt, ms=dt / args.steps * 1e3, flops=op.flops,
+                           tflops=op.flops * args.steps / dt / 1e12, bytes=op.bytes)
+        op.close()
+        log(f"{name}: {args.steps} steps in {dt * 1e3:.1f} ms = {steps[name]['ms']:.3f} ms per H*psi, "
+            f"{steps[name]['tflops']:.2f} TFLOP/s")
+
     copy_bytes = 1 << 30
-    copy_ms = engine.copy_bench(G.lib().gqmps2_engine_ctx(), copy_bytes, reps=5)
+    copy_ms = engine.copy_bench(ctxh, copy_bytes, reps=5)
     copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
     log(f"calibration copy: {copy_bytes >> 20} MiB in {copy_ms:.3f} ms = {copy_gbs:.0f} GB/s (read + write)")
-
-    def tflops(b):
-        return b["flops"] / (b["ms"] * 1e-3) / 1e12
 
     dom = mb["mixed"]
     dom_ach = dom["dom_flops"] / (dom["dom_ms"] * 1e-3) / 1e12 if dom["dom_ms"] > 0 else 0.0
@@ -175,107 +255,121 @@ def main():
                     frac=round(dom_ach / PEAK["fp32"], 4), traffic=measured_traffic(args),
                     kernel=f"grouped_gemm_kernel<float, ..., 1> (stage {dom['dom_stage'] + 1} of the centre-bond "
                            f"matvec, run as the tagged instance)",
-                    flops_per_launch=dom["dom_flops"], ms_per_launch=dom["dom_ms"])
+                    flops_per_launch=dom["dom_flops"], ms_per_launch=dom["dom_ms"],
+                    algorithmic_bytes_per_launch=dom.get("dom_bytes"))
     d64 = mb["fp64"]
     d64_ach = d64["dom_flops"] / (d64["dom_ms"] * 1e-3) / 1e12 if d64["dom_ms"] > 0 else 0.0
 
     if args.profile:
-        print(json.dumps({"metric": METRIC, "profile": True, "matvec_ms": {k: mb[k]["ms"] for k in mb},
+        print(json.dumps({"metric": METRIC, "profile": True, "steps": args.steps,
+                          "ms_per_step": {k: round(v["ms"], 4) for k, v in steps.items()},
                           "roofline": roofline, "copy_gbs": round(copy_gbs, 1)}), flush=True)
         return
 
-    L_, W1_, W2_, R_, psi_ = model.bench_tensors()
-    ctxh = G.lib().gqmps2_engine_ctx()
-    # one GPU's share: the dominant stage split over 8 owners of its output blocks
-    st = dom["dom_stage"]
-    rows = [engine.heff_bench_shard(L_, W1_, W2_, R_, psi_, "cent", engine.PREC_MIXED, st, 8, r, reps=3)
-            for r in range(8)]
-    sh_ms = [x[0] for x in rows]
-    shard = dict(owners=8, stage=st + 1, precision="mixed", assignment="output blocks, longest-processing-time on flops",
-                 ms_per_owner=[round(m, 4) for m in sh_ms], flops_per_owner=[x[1] for x in rows],
-                 stage_flops=rows[0][2], full_stage_ms=round(dom["dom_ms"], 4), max_owner_ms=round(max(sh_ms), 4),
-                 balance=round(sum(x[1] for x in rows) / 8 / max(x[1] for x in rows), 4),
-                 owner_tflops=[round(x[1] / (x[0] * 1e-3) / 1e12, 2) for x in rows],
-                 speedup_vs_one_gpu=round(dom["dom_ms"] / max(sh_ms), 3))
-    log(f"8-owner share of stage {st + 1}: max {max(sh_ms):.3f} ms per owner vs {dom['dom_ms']:.3f} ms whole "
-        f"(speedup {shard['speedup_vs_one_gpu']}, flop balance {shard['balance']})")
-    # FP64 vs mixed Lanczos on the centre bond from one random start (seed 42)
-    rng = np.random.default_rng(42)
-    bt = psi_.get()
-    for k in bt.blocks:
-        bt.blocks[k] = rng.standard_normal(bt.blocks[k].shape)
-    lz = {}
-    for name, prec in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED)):
-        init = engine.put_on(ctxh, bt)
-        engine.check(engine.lib().mps2_ctx_sync(ctxh))
-        t0 = time.perf_counter()
-        it, e, gs = engine.lanczos(L_, W1_, W2_, R_, init, "cent", le, lm, prec)
-        engine.check(engine.lib().mps2_ctx_sync(ctxh))
-        lz[name] = dict(iters=it, energy=e, s=time.perf_counter() - t0, gs=gs)
-    ov = abs(engine.dot(lz["fp64"]["gs"], lz["mixed"]["gs"])) / np.sqrt(
-        engine.dot(lz["fp64"]["gs"], lz["fp64"]["gs"]) * engine.dot(lz["mixed"]["gs"], lz["mixed"]["gs"]))
-    lanczos_cmp = dict(start="random normal (seed 42) in the centre-bond state's blocks",
-                       iters={k: lz[k]["iters"] for k in lz}, energy={k: lz[k]["energy"] for k in lz},
-                       seconds={k: round(lz[k]["s"], 3) for k in lz},
-                       speedup_mixed_vs_fp64=round(lz["fp64"]["s"] / lz["mixed"]["s"], 3),
-                       dE_rel=abs(lz["mixed"]["energy"] - lz["fp64"]["energy"]) / abs(lz["fp64"]["energy"]),
-                       overlap=float(ov))
-    log(f"centre-bond Lanczos from a random start: fp64 {lz['fp64']['iters']} iters {lz['fp64']['s']:.2f}s, "
-        f"mixed {lz['mixed']['iters']} iters {lz['mixed']['s']:.2f}s, dE_rel {lanczos_cmp['dE_rel']:.2e}, "
-        f"overlap {ov:.15f}")
-    for v in lz.values():
-        v["gs"].free()
+    shard = None
+    lanczos_cmp = None
+    if elapsed() < args.budget_s - 120:
+        st = dom["dom_stage"]
+        rows = [engine.heff_bench_shard(L_, W1_, W2_, R_, psi_, "cent", engine.PREC_MIXED, st, 8, r, reps=3)
+                for r in range(8)]
+        sh_ms = [x[0] for x in rows]
+        shard = dict(owners=8, stage=st + 1, precision="mixed",
+                     assignment="output blocks, longest-processing-time on flops",
+                     ms_per_owner=[round(m, 4) for m in sh_ms], flops_per_owner=[x[1] for x in rows],
+                     stage_flops=rows[0][2], full_stage_ms=round(dom["dom_ms"], 4), max_owner_ms=round(max(sh_ms), 4),
+                     balance=round(sum(x[1] for x in rows) / 8 / max(x[1] for x in rows), 4),
+                     speedup_vs_one_gpu=round(dom["dom_ms"] / max(sh_ms), 3))
+        log(f"8-owner share of stage {st + 1}: max {max(sh_ms):.3f} ms per owner vs {dom['dom_ms']:.3f} ms whole")
+        rng = np.random.default_rng(42)
+        bt = psi_.get()
+        for k in bt.blocks:
+            bt.blocks[k] = rng.standard_normal(bt.blocks[k].shape)
+        lz = {}
+        for name, prec in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED)):
+            init = engine.put_on(ctxh, bt)
+            engine.check(engine.lib().mps2_ctx_sync(ctxh))
+            t0 = time.perf_counter()
+            it, e, gs = engine.lanczos(L_, W1_, W2_, R_, init, "cent", le, lm, prec)
+            engine.check(engine.lib().mps2_ctx_sync(ctxh))
+            lz[name] = dict(iters=it, energy=e, s=time.perf_counter() - t0, gs=gs)
+            init.free()
+        ov = abs(engine.dot(lz["fp64"]["gs"], lz["mixed"]["gs"])) / np.sqrt(
+            engine.dot(lz["fp64"]["gs"], lz["fp64"]["gs"]) * engine.dot(lz["mixed"]["gs"], lz["mixed"]["gs"]))
+        lanczos_cmp = dict(start="random normal (seed 42) in the centre-bond state's blocks",
+                           iters={k: lz[k]["iters"] for k in lz}, energy={k: lz[k]["energy"] for k in lz},
+                           seconds={k: round(lz[k]["s"], 3) for k in lz},
+                           speedup_mixed_vs_fp64=round(lz["fp64"]["s"] / lz["mixed"]["s"], 3),
+                           dE_rel=abs(lz["mixed"]["energy"] - lz["fp64"]["energy"]) / abs(lz["fp64"]["energy"]),
+                           overlap=float(ov))
+        log(f"centre-bond Lanczos from a random start: fp64 {lz['fp64']['iters']} iters {lz['fp64']['s']:.2f}s, "
+            f"mixed {lz['mixed']['iters']} iters {lz['mixed']['s']:.2f}s, dE_rel {lanczos_cmp['dE_rel']:.2e}, "
+            f"overlap {ov:.15f}")
+        for v in lz.values():
+            v["gs"].free()
+    else:
+        log("budget: owner-share and random-start Lanczos comparison skipped")
 
     cpu = None
-    if not args.no_cpu_baseline:
-        t0 = time.perf_counter()
+    cpu_budget = min(args.cpu_budget_s, args.budget_s - elapsed() - 15)
+    if not args.no_cpu_baseline and cpu_budget > 5:
+        n_upd = len(res["fp64"]["r"]["energies"])
+        mv_per_update = res["fp64"]["counters"]["matvec_fp64"] / max(1, n_upd)
         try:
-            cpu = cpu_baseline(model.bench_tensors(), mb["fp64"]["flops"], args.cpu_budget_s)
-            log(f"cpu baseline {cpu['value']:.4f} TFLOP/s on {cpu['cores']} cores ({time.perf_counter() - t0:.1f}s)")
+            cpu = cpu_baseline((L_, W1_, W2_, R_, psi_), steps["fp64"]["flops"], cpu_budget, mv_per_update, n_upd)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline failed: {e}")
+    elif not args.no_cpu_baseline:
+        log("budget: CPU baseline skipped")
 
-    step_ms = res["mixed"]["wall"] / args.steps * 1e3
+    e64, emx = res["fp64"]["r"]["energy"], res["mixed"]["r"]["energy"]
+    de_rel = abs(emx - e64) / abs(e64)
+    upd_rel = float(max(abs(a - b) / abs(b) for a, b in zip(res["mixed"]["r"]["energies"],
+                                                           res["fp64"]["r"]["energies"])))
+    dims_equal = bool((res["mixed"]["r"]["dims"] == res["fp64"]["r"]["dims"]).all())
     out = {
         "metric": METRIC,
-        "value": round(tflops(mb["mixed"]), 4),
+        "value": round(steps["mixed"]["tflops"], 4),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(step_ms, 1),
+        "ms_per_step": round(steps["mixed"]["ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (FP32-MFMA matvec) + f64 (residual, Lanczos, SVD)",
         "data": "synthetic (seeded random initial MPS, D ramped to the target by sweeping)",
         "config": {"workload": f"J1-J2 Heisenberg {W}x{Lx} cylinder, snake order, J2={args.j2}, U(1) Sz=0, "
-                               f"D={args.dmax}, two-site DMRG sweep",
+                               f"D={args.dmax}, two-site DMRG",
                    "sites": N, "mpo_bond_dim": w, "dmax": args.dmax, "lanczos_error": le,
-                   "lanczos_max_iter": lm, "step": "one full sweep (2(N-1) two-site updates)"},
-        "sweep_s": {"fp64": round(res["fp64"]["wall"] / args.steps, 3),
-                    "mixed": round(res["mixed"]["wall"] / args.steps, 3)},
-        "sweep_phase_s": {"fp64": res["fp64"]["phase"], "mixed": res["mixed"]["phase"]},
+                   "lanczos_max_iter": lm,
+                   "step": "one centre-bond effective-Hamiltonian matvec H*psi (mixed: FP32 MFMA)"},
+        "matvec_tflops": {k: round(v["tflops"], 4) for k, v in steps.items()},
+        "matvec_ms": {k: round(v["ms"], 4) for k, v in steps.items()},
+        "matvec_flops": steps["fp64"]["flops"],
+        "matvec_frac_of_peak": {"fp64": round(steps["fp64"]["tflops"] / PEAK["fp64"], 4),
+                                "mixed": round(steps["mixed"]["tflops"] / PEAK["fp32"], 4)},
+        "sweep_s": {k: round(res[k]["wall"], 3) for k in res},
+        "sweep_phase_s": {k: res[k]["phase"] for k in res},
         "sweep_speedup_mixed_vs_fp64": round(res["fp64"]["wall"] / res["mixed"]["wall"], 3),
-        "matvec_tflops": {"fp64": round(tflops(mb["fp64"]), 4), "mixed": round(tflops(mb["mixed"]), 4)},
-        "matvec_ms": {"fp64": round(mb["fp64"]["ms"], 4), "mixed": round(mb["mixed"]["ms"], 4)},
-        "matvec_flops": mb["fp64"]["flops"],
-        "matvec_frac_of_peak": {"fp64": round(tflops(mb["fp64"]) / PEAK["fp64"], 4),
-                                "mixed": round(tflops(mb["mixed"]) / PEAK["fp32"], 4)},
+        "sweep_counters": {k: {c: res[k]["counters"][c] for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls",
+                                                                  "hip_malloc")} for k in res},
+        "stage_ms": {"fp64": mb["fp64"]["stage_ms"], "mixed": mb["mixed"]["stage_ms"]},
         "roofline_fp64": dict(bound="mfma", achieved=round(d64_ach, 3), peak=PEAK["fp64"], unit="TFLOP/s",
-                              frac=round(d64_ach / PEAK["fp64"], 4)),
+                              frac=round(d64_ach / PEAK["fp64"], 4), flops_per_launch=d64["dom_flops"],
+                              ms_per_launch=d64["dom_ms"]),
         "energy": {"fp64": e64, "mixed": emx, "dE_rel_final": de_rel, "dE_rel_max_update": upd_rel,
                    "kept_dims_identical": dims_equal},
         "truncation_error": {k: {"max": float(res[k]["r"]["trunc_errs"].max()),
                                  "mean": float(res[k]["r"]["trunc_errs"].mean())} for k in res},
         "centre_bond_lanczos": lanczos_cmp,
         "one_gpu_share": shard,
-        "lanczos_iters_mean": {"fp64": float(res["fp64"]["r"]["iters"].mean()),
-                               "mixed": float(res["mixed"]["r"]["iters"].mean())},
+        "lanczos_iters_mean": {k: float(res[k]["r"]["iters"].mean()) for k in res},
         "peak_hbm_gib": round(max(res[k]["stats"]["peak_bytes"] for k in res) / 2**30, 2),
         "roofline": roofline,
         "hbm_copy_gbs": round(copy_gbs, 1),
         "cpu_baseline": cpu,
+        "wall_s": round(elapsed(), 1),
     }
     print(json.dumps(out), flush=True)
 

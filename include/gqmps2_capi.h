@@ -72,7 +72,7 @@ int gqmps2_dmrg(gqmps2_model m, int64_t sweeps, int64_t dmin, int64_t dmax, cons
  * Hamiltonian for FP64 and mixed precision and keep copies of L, W1, W2, R
  * and the initial state.  site < 0 disables it. */
 int gqmps2_set_bench(gqmps2_model m, int64_t site, int warmup, int reps);
-/* Results: out[0..10] FP64, out[11..21] mixed (layout of mps2_heff_bench). */
+/* Results: out[0..11] FP64, out[12..23] mixed (layout of mps2_heff_bench). */
 int gqmps2_get_bench(gqmps2_model m, double *out);
 /* New handles on the kept tensors (L, W1, W2, R, psi); NULL where absent. */
 int gqmps2_bench_tensors(gqmps2_model m, mps2_ten *out5);

@@ -62,6 +62,8 @@ int mps2_device_count(int *n);
 
 /* Context: one device, one stream, caching allocator, plan cache. */
 int mps2_ctx_create(int device, mps2_ctx *out);
+/* Invalidates the handle.  Tensors created from it may still be exported and
+ * must still be freed; the device context goes away with the last of them. */
 int mps2_ctx_destroy(mps2_ctx ctx);
 int mps2_ctx_sync(mps2_ctx ctx);
 /* Counters: algorithmic GEMM flops issued, grouped-GEMM launches, live and
@@ -128,6 +130,25 @@ int mps2_svd(mps2_ten t, int ldims, const int32_t *ldiv, double trunc_err, int64
  * NULL, as in the reference where they are unused. */
 int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where, mps2_ten *out);
 
+/* The effective Hamiltonian bound once to its operands: L, W1, W2, R and the
+ * block layout of `psi`, at one precision (MPS2_PREC_MIXED makes the FP32
+ * operand copies once).  It is the operator that eff_ham_mul_state_{lend,
+ * cent,rend} (lanczos_solver_impl.h:228-256) applies at every Lanczos step
+ * (:149,171), with its setup hoisted out of the loop.  FP64 operands are
+ * shared with the caller's tensors (kept alive by the operator; they must not
+ * be modified while it is in use).  mps2_heff_run applies it n times to the
+ * bound state (psi embedded into the full two-site layout, at the
+ * operator's precision), stream-ordered, without a host sync: the matvec
+ * alone, as a benchmark step or a warm-up. */
+typedef struct mps2_heff_s *mps2_heff;
+int mps2_heff_create(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                     int precision, mps2_heff *out);
+int mps2_heff_run(mps2_heff h, int n);
+/* Algorithmic flops of one application (sum over QN-matched block pairs of
+ * 2*M*N*K) and the FP64 bytes of its operands plus two states. */
+int mps2_heff_info(mps2_heff h, double *flops, double *bytes);
+int mps2_heff_destroy(mps2_heff h);
+
 /* LanczosSolver({L, W1, W2, R}, init, {error, max_iterations}, where)
  * (lanczos_solver_impl.h:79-225).  init is not consumed (the caller frees it).
  * Returns iterations, ground energy and a new ground-state tensor. */
@@ -145,7 +166,9 @@ int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init
  * separately named kernel instance (grouped_gemm_kernel<..., 1>), so a
  * profiler's per-kernel statistics of that instance cover these launches,
  * out[6] FP64 bytes of L, W1, W2, R and two states (the fused-ideal traffic),
- * out[7..10] ms per launch of stages 1..4. */
+ * out[7..10] ms per launch of stages 1..4, out[11] the dominant stage's
+ * algorithmic bytes per launch (its operands read once and its result
+ * written once, at the working precision). */
 int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
                     int precision, int warmup, int reps, double *out);
 

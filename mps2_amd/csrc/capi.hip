@@ -14,10 +14,6 @@ struct mps2_ten_s {
   mps2::TenP t;
 };
 
-namespace mps2 {
-void plan_cache_clear(Ctx *c);
-}
-
 namespace {
 thread_local std::string g_err;
 
@@ -87,8 +83,11 @@ int mps2_ctx_create(int device, mps2_ctx *out) {
 int mps2_ctx_destroy(mps2_ctx ctx) {
   return guard([&] {
     if (!ctx) return;
-    mps2::plan_cache_clear(ctx->c.get());
+    // Tensors of this context may outlive the handle: the context itself is
+    // deleted when the last of them is freed.
+    mps2::Ctx *c = ctx->c.release();
     delete ctx;
+    c->close();
   });
 }
 
@@ -296,6 +295,78 @@ int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi
   });
 }
 
+}  // extern "C"
+
+struct mps2_heff_s {
+  mps2::TenP L, W1, W2, R, x, y;
+  std::string where;
+  mps2::Prec prec = mps2::kF64;
+  double flops = 0, bytes = 0;
+};
+
+extern "C" {
+
+int mps2_heff_create(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where,
+                     int precision, mps2_heff *out) {
+  return guard([&] {
+    need(W1, "W1");
+    need(W2, "W2");
+    need(psi, "psi");
+    need(where, "where");
+    need(out, "out");
+    std::string w(where);
+    if (w != "cent" && w != "lend" && w != "rend") throw mps2::Error(mps2::kErrArg, "where must be lend, cent or rend");
+    if (w == "cent" || w == "rend") need(L, "L");
+    if (w == "cent" || w == "lend") need(R, "R");
+    mps2::Ctx *c = psi->t->ctx;
+    auto h = std::make_unique<mps2_heff_s>();
+    h->where = w;
+    h->prec = precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64;
+    auto cast = [&](const mps2::TenP &t) -> mps2::TenP {
+      if (!t || h->prec == mps2::kF64) return t;
+      auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
+      mps2::k_cast_d2f(c, t->n(), t->d(), r->f());
+      return r;
+    };
+    auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
+    h->L = cast(opt(L));
+    h->W1 = cast(W1->t);
+    h->W2 = cast(W2->t);
+    h->R = cast(opt(R));
+    h->x = cast(mps2::embed(psi->t, full));
+    h->y = mps2::ten_alloc(c, full, h->prec);
+    // one application: builds (and caches) the plans and counts the flops
+    const double f0 = c->gemm_flops;
+    mps2::heff_mul(h->L, h->W1, h->W2, h->R, h->x, w, h->y, h->prec);
+    h->flops = c->gemm_flops - f0;
+    for (auto &t : {opt(L), W1->t, W2->t, opt(R)})
+      if (t) h->bytes += 8.0 * t->n();
+    h->bytes += 2 * 8.0 * full->total;
+    c->sync();
+    *out = h.release();
+  });
+}
+
+int mps2_heff_run(mps2_heff h, int n) {
+  return guard([&] {
+    need(h, "h");
+    if (n < 0) throw mps2::Error(mps2::kErrArg, "heff_run: n must be >= 0");
+    for (int i = 0; i < n; ++i) mps2::heff_mul(h->L, h->W1, h->W2, h->R, h->x, h->where, h->y, h->prec);
+  });
+}
+
+int mps2_heff_info(mps2_heff h, double *flops, double *bytes) {
+  return guard([&] {
+    need(h, "h");
+    if (flops) *flops = h->flops;
+    if (bytes) *bytes = h->bytes;
+  });
+}
+
+int mps2_heff_destroy(mps2_heff h) {
+  return guard([&] { delete h; });
+}
+
 int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init, const char *where, double error,
                  int64_t max_iter, int precision, int64_t *iters, double *energy, mps2_ten *gs) {
   return guard([&] {
@@ -365,14 +436,17 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     size_t per = c->gemm_timings.size() / reps;
-    std::vector<double> st_ms(per, 0.0), st_fl(per, 0.0);
+    std::vector<double> st_ms(per, 0.0), st_fl(per, 0.0), st_by(per, 0.0);
     double gemm_ms = 0, flops = 0;
     for (size_t k = 0; k < c->gemm_timings.size(); ++k) {
       float t = 0;
       MPS2_HIP(hipEventElapsedTime(&t, c->gemm_timings[k].start, c->gemm_timings[k].stop));
       gemm_ms += t;
       st_ms[k % per] += t;
-      if (k < per) st_fl[k] = c->gemm_timings[k].flops;
+      if (k < per) {
+        st_fl[k] = c->gemm_timings[k].flops;
+        st_by[k] = c->gemm_timings[k].bytes;
+      }
       (void)hipEventDestroy(c->gemm_timings[k].start);
       (void)hipEventDestroy(c->gemm_timings[k].stop);
     }
@@ -417,7 +491,7 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     for (auto &t : {opt(L), W1->t, W2->t, opt(R)})
       if (t) bytes += 8.0 * t->n();
     bytes += 2 * 8.0 * full->total;
-    for (int i = 0; i < 11; ++i) out[i] = 0;
+    for (int i = 0; i < 12; ++i) out[i] = 0;
     out[0] = ms / reps;
     out[1] = flops;
     out[2] = gemm_ms / reps;
@@ -426,6 +500,7 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     out[5] = dom_ms;
     out[6] = bytes;
     for (size_t k = 0; k < per && k < 4; ++k) out[7 + k] = st_ms[k] / reps;
+    out[11] = per ? st_by[dom] : 0;
   });
 }
 

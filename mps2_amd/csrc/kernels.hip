@@ -84,6 +84,7 @@ BufP Ctx::alloc(size_t bytes) {
   }
   live_bytes += rb;
   peak_bytes = std::max(peak_bytes, live_bytes);
+  ++nbufs;
   auto b = std::make_shared<Buf>();
   b->ctx = this;
   b->p = p;
@@ -92,9 +93,30 @@ BufP Ctx::alloc(size_t bytes) {
 }
 
 void Ctx::release(void *p, size_t bytes) {
+  --nbufs;
+  live_bytes -= bytes;
+  if (closing) {
+    // The owner has destroyed this context: hand the block back to HIP and
+    // delete the context with its last buffer (nothing touches it after).
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(p);
+    if (nbufs == 0) delete this;
+    return;
+  }
   free_.emplace(bytes, p);
   cached_bytes += bytes;
-  live_bytes -= bytes;
+}
+
+void Ctx::close() {
+  if (stream) (void)hipStreamSynchronize(stream);
+  plan_cache_clear(this);
+  red_partial.reset();
+  dev_scalars.reset();
+  for (auto &kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+  cached_bytes = 0;
+  closing = true;
+  if (nbufs == 0) delete this;
 }
 
 void Ctx::trim() {

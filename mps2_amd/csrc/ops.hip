@@ -6,7 +6,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "runtime.h"
@@ -31,8 +33,11 @@ struct PlanEntry {
   MetaP keep_out;   // retarget layout kept alive while cached
 };
 
+// Process-wide plan cache (keys carry the context), FIFO eviction; every
+// access holds g_plan_mu, so contexts may be driven from different threads.
+std::mutex g_plan_mu;
 std::map<PlanKey, PlanEntry> g_plans;
-std::vector<PlanKey> g_plan_order;
+std::deque<PlanKey> g_plan_order;
 constexpr size_t kPlanCacheCap = 512;
 
 void perm_descs(const Meta &m, const std::vector<int> &perm, std::vector<PermDesc> &out, std::vector<int64_t> &starts) {
@@ -156,13 +161,19 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
 std::shared_ptr<DevPlan> get_plan_t(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
                                     const std::vector<int> &bx, const MetaP &target) {
   PlanKey key{c, a.get(), b.get(), ax, bx, target.get()};
+  std::lock_guard<std::mutex> lock(g_plan_mu);
   auto it = g_plans.find(key);
   if (it != g_plans.end() && it->second.plan->hp->a == a && it->second.plan->hp->b == b) return it->second.plan;
   auto hp = plan_contract(a, b, ax, bx);
   auto dp = build_dev_plan(c, hp, target.get());
+  if (it != g_plans.end()) {
+    // same addresses, new layouts (the old ones were freed): replace in place
+    it->second = PlanEntry{dp, target};
+    return dp;
+  }
   if (g_plans.size() >= kPlanCacheCap) {
     g_plans.erase(g_plan_order.front());
-    g_plan_order.erase(g_plan_order.begin());
+    g_plan_order.pop_front();
   }
   g_plans[key] = PlanEntry{dp, target};
   g_plan_order.push_back(key);
@@ -255,6 +266,7 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   if (c->time_gemms) {
     MPS2_HIP(hipEventRecord(tm.stop, c->stream));
     tm.flops = dp.hp->flops;
+    tm.bytes = (double)sizeof(T) * ((double)a->n() + (double)b->n() + (double)dp.hp->out->total);
     c->gemm_timings.push_back(tm);
   }
   c->gemm_flops += dp.hp->flops;
@@ -263,11 +275,17 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
 }  // namespace
 
 void plan_cache_clear(Ctx *c) {
+  std::vector<PlanEntry> dropped;   // released after the lock (their buffers go back to c)
+  std::lock_guard<std::mutex> lock(g_plan_mu);
   for (auto it = g_plans.begin(); it != g_plans.end();) {
-    if (std::get<0>(it->first) == c) it = g_plans.erase(it);
-    else ++it;
+    if (std::get<0>(it->first) == c) {
+      dropped.push_back(std::move(it->second));
+      it = g_plans.erase(it);
+    } else {
+      ++it;
+    }
   }
-  std::vector<PlanKey> keep;
+  std::deque<PlanKey> keep;
   for (auto &k : g_plan_order)
     if (std::get<0>(k) != c) keep.push_back(k);
   g_plan_order.swap(keep);

@@ -79,6 +79,7 @@ struct Ctx {
   struct GemmTiming {
     hipEvent_t start, stop;
     double flops;
+    double bytes;   // operand and result bytes of the launch (algorithmic)
   };
   std::vector<GemmTiming> gemm_timings;
   // Profiling aid: within each effective-Hamiltonian matvec, the grouped GEMM
@@ -95,9 +96,18 @@ struct Ctx {
   // named event counters (svd calls, Jacobi sweeps, host syncs, ...)
   std::map<std::string, double> counters;
   void count(const char *name, double v = 1.0) { counters[name] += v; }
+  // Buffers handed out and not yet released.  A closed context (close())
+  // stays alive until the last of them is released: every Buf keeps a raw
+  // pointer to its context.
+  int64_t nbufs = 0;
+  bool closing = false;
 
   explicit Ctx(int dev);
   ~Ctx();
+  // Destroy on behalf of the owner: frees the plans and the cache now, and
+  // deletes the context at once or, while tensors allocated from it are
+  // alive, when the last of them is released.
+  void close();
   BufP alloc(size_t bytes);
   void release(void *p, size_t bytes);
   void trim();
@@ -200,6 +210,8 @@ struct DevPlan {
 
 std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
                                   const std::vector<int> &bx);
+// Drops every cached plan of context c (thread-safe).
+void plan_cache_clear(Ctx *c);
 TenP contract(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx);
 // Contract writing into `out` whose meta must contain every output block of the
 // plan; blocks of `out` not produced are zero-filled.

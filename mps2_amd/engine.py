@@ -55,6 +55,10 @@ SIGNATURES = {
     "mps2_heff_bench_shard": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, _dp]),
     "mps2_heff_bench": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, _dp]),
+    "mps2_heff_create": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.POINTER(_vp)]),
+    "mps2_heff_run": (C.c_int, [_vp, C.c_int]),
+    "mps2_heff_info": (C.c_int, [_vp, _dp, _dp]),
+    "mps2_heff_destroy": (C.c_int, [_vp]),
 }
 
 
@@ -247,11 +251,11 @@ def copy_bench(ctx_handle, nbytes, reps=5):
 
 
 def heff_bench(L, W1, W2, R, psi, where, precision=PREC_FP64, warmup=2, reps=10):
-    out = np.zeros(11)
+    out = np.zeros(12)
     check(lib().mps2_heff_bench(_h(L), W1.h, W2.h, _h(R), psi.h, where.encode(), precision, warmup, reps,
                                 out.ctypes.data_as(_dp)))
     return dict(ms=out[0], flops=out[1], gemm_ms=out[2], dom_stage=int(out[3]), dom_flops=out[4],
-                dom_ms=out[5], bytes_min=out[6], stage_ms=list(out[7:11]))
+                dom_ms=out[5], bytes_min=out[6], stage_ms=list(out[7:11]), dom_bytes=out[11])
 
 
 SIGNATURES["mps2_contract_ex"] = (C.c_int, [_vp, _vp, C.c_int, _i32p, _i32p, C.c_int, C.POINTER(_vp)])
@@ -280,3 +284,31 @@ def ctx_counters(ctx_handle, names=COUNTERS):
         check(lib().mps2_ctx_counter(ctx_handle, n.encode(), C.byref(v)))
         out[n] = v.value
     return out
+
+
+class HeffOp:
+    """The effective Hamiltonian bound once to (L, W1, W2, R) and a state
+    layout at one precision (mps2_heff_create); run(n) applies it n times,
+    stream-ordered, with no host sync."""
+
+    def __init__(self, L, W1, W2, R, psi, where, precision=PREC_FP64):
+        self.h = _vp()
+        check(lib().mps2_heff_create(_h(L), _h(W1), _h(W2), _h(R), _h(psi), where.encode(), precision,
+                                     C.byref(self.h)))
+        f, b = C.c_double(), C.c_double()
+        check(lib().mps2_heff_info(self.h, C.byref(f), C.byref(b)))
+        self.flops, self.bytes = f.value, b.value
+
+    def run(self, n):
+        check(lib().mps2_heff_run(self.h, int(n)))
+
+    def close(self):
+        if self.h:
+            check(lib().mps2_heff_destroy(self.h))
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -178,13 +178,13 @@ class Model:
         return self.get_bench()
 
     def get_bench(self):
-        out = np.zeros(22)
+        out = np.zeros(24)
         check(lib().gqmps2_get_bench(self.h, out.ctypes.data_as(_dp)))
 
         def unpack(o):
             return dict(ms=o[0], flops=o[1], gemm_ms=o[2], dom_stage=int(o[3]), dom_flops=o[4], dom_ms=o[5],
-                        bytes_min=o[6], stage_ms=list(o[7:11]))
-        return dict(fp64=unpack(out[:11]), mixed=unpack(out[11:]))
+                        bytes_min=o[6], stage_ms=list(o[7:11]), dom_bytes=o[11])
+        return dict(fp64=unpack(out[:12]), mixed=unpack(out[12:]))
 
     def bench_tensors(self):
         """(L, W1, W2, R, psi) of the benchmarked bond as engine DTen handles."""

@@ -38,7 +38,7 @@ struct ModelBase {
   int64_t bench_site = -1;
   int bench_warmup = 2, bench_reps = 10;
   bool bench_done = false;
-  double bench_out[22] = {0};
+  double bench_out[24] = {0};
   std::vector<double> phase_times;   // (lanczos, svd, env) seconds per update of the last dmrg call
   std::vector<mps2_ten> bench_tens = std::vector<mps2_ten>(5, nullptr);
   void clear_bench_tens() {
@@ -258,6 +258,7 @@ struct Model : ModelBase {
   }
 
   void checkpoint() override {
+    require_mps();
     saved.clear();
     for (size_t i = 0; i < N; ++i) {
       TenT c;
@@ -281,8 +282,8 @@ struct Model : ModelBase {
 
   void bench_bond(int64_t site, int warmup, int reps) override {
     if (site < 1 || (size_t)site + 2 >= N) throw std::invalid_argument("bench_bond: site must be an interior bond");
-    for (size_t i = 0; i < N; ++i)
-      if (!cm()(i)) throw std::runtime_error("MPS not initialised");
+    require_mps();
+    require_mpo();
     mps->Centralize((int)site);
     gqmps2::SweepParams sp(1, 1, 1, 0.0, gqmps2::LanczosParams());
     gqmps2::TwoSiteVMPSEngine<double, QNT> eng(*mps, mpo, sp);
@@ -312,7 +313,7 @@ struct Model : ModelBase {
     for (int p = 0; p < 2; ++p)
       gqten::CheckMps2(mps2_heff_bench(H[0]->Handle(), H[1]->Handle(), H[2]->Handle(), H[3]->Handle(), init.Handle(),
                                        "cent", p == 0 ? MPS2_PREC_FP64 : MPS2_PREC_MIXED, warmup, reps,
-                                       bench_out + 11 * p),
+                                       bench_out + 12 * p),
                        "heff_bench");
     clear_bench_tens();
     mps2_ten src[5] = {H[0]->Handle(), H[1]->Handle(), H[2]->Handle(), H[3]->Handle(), init.Handle()};
@@ -323,6 +324,12 @@ struct Model : ModelBase {
   void require_mps() const {
     for (size_t i = 0; i < N; ++i)
       if (!cm()(i)) throw std::runtime_error("MPS not initialised");
+  }
+
+  // The sweep indexes one MPO tensor per site (the reference asserts
+  // mps.size() == mpo.size(), two_site_update_finite_vmps_impl.h:83).
+  void require_mpo() const {
+    if (mpo.size() != N) throw std::runtime_error("no MPO yet (call gqmps2_model_gen_mpo)");
   }
 
   void truncate_mps(double trunc_err, int64_t dmin, int64_t dmax) override {
@@ -383,8 +390,9 @@ struct Model : ModelBase {
   double dmrg(int64_t sweeps, int64_t dmin, int64_t dmax, const int64_t *sched, double trunc_err, double lerr,
               int64_t lmax, int prec, int verbose, double *en, int64_t *dims, int64_t *its, double *te, double *ut,
               double *st) override {
-    for (size_t i = 0; i < N; ++i)
-      if (!cm()(i)) throw std::runtime_error("MPS not initialised");
+    require_mps();
+    require_mpo();
+    if (sweeps < 0 || dmin < 1 || dmax < dmin) throw std::invalid_argument("dmrg: need sweeps >= 0 and 1 <= dmin <= dmax");
     gqmps2::LanczosParams lp(lerr, (size_t)lmax,
                              prec == MPS2_PREC_MIXED ? gqmps2::LanczosPrecision::Mixed : gqmps2::LanczosPrecision::FP64);
     double e0 = 0;
@@ -424,7 +432,7 @@ struct Model : ModelBase {
           for (int p = 0; p < 2; ++p)
             gqten::CheckMps2(mps2_heff_bench(h(H[0]), H[1]->Handle(), H[2]->Handle(), h(H[3]), init.Handle(),
                                              where.c_str(), p == 0 ? MPS2_PREC_FP64 : MPS2_PREC_MIXED, bench_warmup,
-                                             bench_reps, bench_out + 11 * p),
+                                             bench_reps, bench_out + 12 * p),
                              "heff_bench");
           clear_bench_tens();
           mps2_ten src[5] = {h(H[0]), H[1]->Handle(), H[2]->Handle(), h(H[3]), init.Handle()};
@@ -615,7 +623,7 @@ int gqmps2_get_bench(gqmps2_model m, double *out) {
   return guard([&] {
     if (!m || !out) throw std::invalid_argument("NULL argument");
     if (!m->m->bench_done) throw std::runtime_error("no benchmark recorded");
-    for (int i = 0; i < 22; ++i) out[i] = m->m->bench_out[i];
+    for (int i = 0; i < 24; ++i) out[i] = m->m->bench_out[i];
   });
 }
 

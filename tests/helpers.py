@@ -239,3 +239,16 @@ def schmidt_values(psi, dims, cut):
     """Schmidt values of a dense state across the cut after site `cut`."""
     left = int(np.prod(dims[:cut + 1]))
     return np.linalg.svd(psi.reshape(left, -1), compute_uv=False)
+
+
+def run_pair(model, sweeps, dmin, dmax, trunc, lerr, lmax=200, labs=None, enlarged=8, seed=11):
+    """The GPU sweep (gqmps2 host layer, FP64) and the oracle sweep from the
+    same right-canonical start: per-update records of both."""
+    N = model.N
+    labs = labs or [[i % 2 for i in range(N)], [(i + 1) % 2 for i in range(N)]]
+    model.init_mps(labs, enlarged_dim=enlarged, seed=seed, centralize=True)
+    mps0 = [model.mps(i) for i in range(N)]
+    mpo = [model.mpo(i) for i in range(N)]
+    g = model.dmrg(sweeps, dmin, dmax, trunc, lerr, lmax)
+    o = O.dmrg(mps0, mpo, sweeps, dmin, dmax, trunc, lerr, lmax)
+    return g, o

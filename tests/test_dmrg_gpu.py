@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from mps2_amd import hamiltonians as HM
-from oracle import oracle_py as O
+from tests.helpers import run_pair
 
 pytestmark = pytest.mark.gpu
 
@@ -25,17 +25,6 @@ def test_golden_energy_on_gpu(name):
     m.init_mps(init["labs"], enlarged_dim=init.get("enlarged", 0), seed=7, centralize=True)
     r = m.dmrg(case["sweeps"], case["dmin"], case["dmax"], case["trunc"], case["lerr"], case["lmax"])
     assert abs(r["energy"] - case["e0"]) < case["tol"], (r["energy"], case["e0"])
-
-
-def run_pair(model, sweeps, dmin, dmax, trunc, lerr, lmax=200, labs=None, enlarged=8, seed=11):
-    N = model.N
-    labs = labs or [[i % 2 for i in range(N)], [(i + 1) % 2 for i in range(N)]]
-    model.init_mps(labs, enlarged_dim=enlarged, seed=seed, centralize=True)
-    mps0 = [model.mps(i) for i in range(N)]
-    mpo = [model.mpo(i) for i in range(N)]
-    g = model.dmrg(sweeps, dmin, dmax, trunc, lerr, lmax)
-    o = O.dmrg(mps0, mpo, sweeps, dmin, dmax, trunc, lerr, lmax)
-    return g, o
 
 
 def test_heisenberg_chain_per_update_parity_exact_bond():

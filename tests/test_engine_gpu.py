@@ -430,3 +430,24 @@ def test_u1xu1_contract_and_svd_parity(ctx):
         assert dmax == 1000 or err > 0.0   # the second case cuts
         assert np.abs(np.sort(np.diag(gs.dense())) - np.sort(np.diag(s.dense()))).max() < 1e-12 * np.abs(s.dense()).max()
         assert abs(gerr - err) < 1e-12
+
+
+def test_tensor_outlives_its_context():
+    """A tensor freed after its context was destroyed returns its memory to
+    HIP; the context is deleted with its last tensor (no use after free)."""
+    rng = np.random.default_rng(5)
+    A = BTensor.random([H.bond(IN, (5, 7), (0, 2)), H.bond(OUT, (5, 7), (0, 2))], (0,), rng)
+    c = E.Context(0)
+    d1, d2 = c.put(A), c.put(A)
+    s = E.dot(d1, d2)
+    c.close()
+    assert abs(s - float((A.dense() ** 2).sum())) < 1e-12 * abs(s)
+    got = d1.get()   # still readable: the context lives until its last tensor goes
+    assert H.max_rel_diff(got, A) == 0.0
+    d1.free()
+    d2.free()
+    c2 = E.Context(0)   # a fresh context works afterwards
+    d3 = c2.put(A)
+    assert H.max_rel_diff(d3.get(), A) == 0.0
+    d3.free()
+    c2.close()

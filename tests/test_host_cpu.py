@@ -504,3 +504,22 @@ def test_model_entry_points_check_handles_and_sites():
     m.set_mps(1, t)
     assert np.array_equal(m.mps(1).dense(), t.dense())
     m.close()
+
+
+def test_model_checkpoint_and_sweeps_need_mps_and_mpo():
+    """checkpoint before init_mps, and dmrg / bench_bond before gen_mpo, return
+    error statuses (the reference asserts mps.size() == mpo.size(),
+    two_site_update_finite_vmps_impl.h:83) instead of reading missing tensors."""
+    from mps2_amd import gqmps2 as G
+    L = G.lib()
+    spin = [((1,), 1), ((-1,), 1)]
+    m = G.Model([spin] * 4)
+    assert L.gqmps2_mps_checkpoint(m.h) != 0
+    assert "MPS not initialised" in L.gqmps2_last_error().decode()
+    m.term(1.0, [np.diag([0.5, -0.5])] * 2, [0, 1])
+    m.init_mps([[0, 1, 0, 1]], centralize=False)
+    with pytest.raises(RuntimeError, match="gen_mpo"):
+        m.dmrg(1, 1, 4, 0.0, 1e-10)
+    assert L.gqmps2_bench_bond(m.h, 1, 1, 1) != 0
+    assert "gen_mpo" in L.gqmps2_last_error().decode()
+    m.close()

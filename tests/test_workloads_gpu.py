@@ -1,0 +1,169 @@
+"""The benchmark workloads on the GPU: the J1-J2 width-4 cylinder of the
+sweep-time metric and the two Fermi-Hubbard cylinders, each at an
+oracle-sized bond dimension against the CPU FP64 oracle update by update,
+and at a production bond dimension through the properties the physics and
+the precision contract fix (mixed against FP64 from one checkpoint).
+
+Long tests append one line per finished sweep to gpurun_out/test_progress.log
+(a file a runner can watch), since pytest holds their console output.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from mps2_amd import hamiltonians as HM
+from tests.helpers import run_pair
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def progress(msg):
+    d = os.path.join(ROOT, "gpurun_out")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "test_progress.log"), "a") as f:
+        f.write(f"{time.strftime('%H:%M:%S')} {msg}\n")
+    print(msg, flush=True)
+
+
+def spin_labs(N):
+    return [[i % 2 for i in range(N)], [(i + 1) % 2 for i in range(N)]]
+
+
+def hubbard_labs(N, ne):
+    """Two product states of ne electrons (alternating up/down, holes at one
+    end), basis 0 = empty, 1 = up, 2 = down, 3 = double."""
+    up_dn = [1 if i % 2 == 0 else 2 for i in range(ne)]
+    return [up_dn + [0] * (N - ne), [0] * (N - ne) + up_dn[::-1]]
+
+
+def assert_same_path(g, o, rel=1e-10):
+    """Update by update: energies to `rel` relative, identical kept dimensions
+    and identical Lanczos iteration counts."""
+    e_g, e_o = g["energies"], o["energies"]
+    d = np.abs(e_g - e_o) / np.abs(e_o)
+    assert d.max() < rel, (d.max(), int(d.argmax()))
+    assert np.array_equal(g["dims"], o["dims"]), np.nonzero(g["dims"] != o["dims"])[0]
+    assert np.array_equal(g["iters"], o["iters"]), np.nonzero(g["iters"] != o["iters"])[0]
+
+
+def mixed_vs_fp64(m, dmax, lerr, tag):
+    """One FP64 and one mixed sweep from the model's checkpoint."""
+    m.restore()
+    t0 = time.time()
+    g64 = m.dmrg(1, 1, dmax, 0.0, lerr, 200)
+    progress(f"{tag}: FP64 sweep D={dmax} {time.time() - t0:.1f}s E={g64['energy']:.14f}")
+    m.restore()
+    t0 = time.time()
+    gmx = m.dmrg(1, 1, dmax, 0.0, lerr, 200, precision=1)
+    progress(f"{tag}: mixed sweep D={dmax} {time.time() - t0:.1f}s E={gmx['energy']:.14f}")
+    rel = np.abs(gmx["energies"] - g64["energies"]) / np.abs(g64["energies"])
+    return g64, gmx, rel
+
+
+# ------------------------------------------------------------------ J1-J2 cylinder (the metric's workload)
+
+def test_j1j2_4x6_cylinder_per_update_parity():
+    """J1-J2 (J2 = 0.5) on a width-4 cylinder, periodic in y, snake order, as
+    the 4x24 benchmark but 6 columns long: three sweeps to Dmax = 256
+    (binding from the second sweep on), GPU FP64 against the oracle from the
+    same right-canonical start: every update's energy to 1e-10 relative, kept
+    dimensions and Lanczos iteration counts identical.  Then mixed against
+    FP64, one sweep each from the converged state: every update to 1e-10
+    relative and identical kept dimensions."""
+    m = HM.j1j2_cylinder(6, 4, 1.0, 0.5)
+    g, o = run_pair(m, 3, 1, 256, 1e-12, 1e-10, labs=spin_labs(m.N), enlarged=8, seed=11)
+    assert g["dims"].max() == 256
+    assert_same_path(g, o)
+    m.checkpoint()
+    g64, gmx, rel = mixed_vs_fp64(m, 256, 1e-12, "j1j2 4x6")
+    assert rel.max() < 1e-10, rel.max()
+    assert np.array_equal(gmx["dims"], g64["dims"])
+
+
+def test_j1j2_4x24_cylinder_D4096_mixed_vs_fp64():
+    """The metric's workload at full size: J1-J2 4x24 cylinder (96 sites,
+    J2 = 0.5), U(1) Sz = 0, FP64 ramp 256 -> 512 -> 1024 -> 2048 -> 4096 from a
+    seeded random start, then one FP64 and one mixed (FP32-MFMA matvec, FP64
+    residual restarts) sweep at D = 4096 from the same checkpoint.  No
+    reference output exists at this size; the checks are the precision
+    contract and what the variational method fixes: every update's energy of
+    the mixed sweep within 1e-10 relative of the FP64 sweep, identical kept
+    dimensions, D = 4096 reached, and the energy at the end of each sweep not
+    above the previous sweep's (within the Lanczos tolerance)."""
+    N = 96
+    m = HM.j1j2_cylinder(24, 4, 1.0, 0.5)
+    m.init_mps(spin_labs(N), enlarged_dim=8, seed=1234)
+    e_prev = None
+    for d in (256, 512, 1024, 2048, 4096):
+        t0 = time.time()
+        r = m.dmrg(1, 1, d, 0.0, 1e-10, 200)
+        progress(f"j1j2 4x24 ramp D={d} {time.time() - t0:.1f}s E={r['energy']:.12f} maxD={r['dims'].max()}")
+        if e_prev is not None:
+            assert r["energy"] <= e_prev + 1e-9, (d, r["energy"], e_prev)
+        e_prev = r["energy"]
+    assert r["dims"].max() == 4096
+    m.checkpoint()
+    g64, gmx, rel = mixed_vs_fp64(m, 4096, 1e-10, "j1j2 4x24")
+    assert rel.max() <= 1e-10, rel.max()
+    assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
+    assert np.array_equal(gmx["dims"], g64["dims"]), np.nonzero(gmx["dims"] != g64["dims"])[0]
+    assert g64["energy"] <= e_prev + 1e-9 and g64["dims"].max() == 4096
+
+
+# ------------------------------------------------------------------ Fermi-Hubbard cylinders (U(1) x U(1))
+
+def test_hubbard_4x16_per_update_parity():
+    """Fermi-Hubbard 4x16 cylinder (t = 1, U = 8, 56 electrons: filling
+    0.875), U(1)xU(1) (N, Sz) blocks, local dimension 4: two sweeps at
+    Dmax = 32 (binding), GPU FP64 against the oracle from the same start,
+    every update to 1e-10 relative with identical kept dimensions and
+    Lanczos iteration counts."""
+    m = HM.hubbard_cylinder(16, 4, t=1.0, tp=0.0, U=8.0)
+    g, o = run_pair(m, 2, 1, 32, 1e-12, 1e-10, labs=hubbard_labs(m.N, 56), enlarged=4, seed=11)
+    assert g["dims"].max() == 32
+    assert_same_path(g, o)
+
+
+def test_hubbard_6x12_tprime_per_update_parity():
+    """Fermi-Hubbard 6x12 cylinder with next-nearest hopping (t = 1,
+    t' = -0.2, U = 10, 64 electrons: filling 0.89), U(1)xU(1), local
+    dimension 4, MPO bond dimension 38: two sweeps at Dmax = 24 against the
+    oracle, every update to 1e-10 relative, identical kept dimensions and
+    iteration counts."""
+    m = HM.hubbard_cylinder(12, 6, t=1.0, tp=-0.2, U=10.0)
+    g, o = run_pair(m, 2, 1, 24, 1e-12, 1e-10, labs=hubbard_labs(m.N, 64), enlarged=4, seed=11)
+    assert g["dims"].max() == 24
+    assert_same_path(g, o)
+
+
+def test_hubbard_4x16_D1024_mixed_vs_fp64():
+    """Hubbard 4x16 cylinder (t = 1, U = 8, filling 0.875) at D = 1024: an
+    FP64 ramp 128 -> 256 -> 512 -> 1024 lowers the energy, then one FP64 and
+    one mixed sweep from the same checkpoint (Lanczos tolerance 1e-12).  This
+    state is strongly truncated (discarded weight ~1e-5 per cut), where any
+    two eigensolvers' eigenvector differences at the 1e-12 energy tolerance
+    reach the energies at first order through the truncation: every update
+    agrees to 1e-9 relative, the final energy to 1e-10, kept dimensions are
+    identical, and the truncation errors are reported."""
+    N = 64
+    m = HM.hubbard_cylinder(16, 4, t=1.0, tp=0.0, U=8.0)
+    m.init_mps(hubbard_labs(N, 56), enlarged_dim=4, seed=11)
+    es = []
+    for d in (128, 256, 512, 1024):
+        t0 = time.time()
+        r = m.dmrg(1, 1, d, 0.0, 1e-10, 200)
+        es.append(r["energy"])
+        progress(f"hubbard 4x16 ramp D={d} {time.time() - t0:.1f}s E={r['energy']:.12f} "
+                 f"max trunc err {r['trunc_errs'].max():.2e}")
+    assert es[-1] < es[0] and r["dims"].max() == 1024
+    m.checkpoint()
+    g64, gmx, rel = mixed_vs_fp64(m, 1024, 1e-12, "hubbard 4x16")
+    progress(f"hubbard 4x16 D=1024: max rel dE per update {rel.max():.2e}, "
+             f"trunc err max {g64['trunc_errs'].max():.2e}")
+    assert rel.max() < 1e-9, rel.max()
+    assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
+    assert np.array_equal(gmx["dims"], g64["dims"])
