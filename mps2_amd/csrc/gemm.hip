@@ -17,6 +17,10 @@
 // scalar read per fragment and step, which measured faster there.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
+
 #include "runtime.h"
 
 namespace mps2 {
@@ -274,6 +278,65 @@ void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const Ti
 }
 
 }  // namespace
+
+// XCD-aware tile order.  Workgroup i of a launch runs on XCD i % 8, and each
+// XCD has its own L2.  The tiles of one output block that share operand
+// panels (the same rows of A or columns of B) are gathered into groups of up
+// to kGroup x kGroup tiles, each group is assigned whole to one XCD
+// (longest-processing-time on the groups' MFMA work, so the XCDs finish
+// together), and each XCD's queue runs heaviest group first with a group's
+// tiles consecutive, so they are resident together and read each panel
+// through that XCD's L2 once per group rather than once per tile.  The list
+// interleaves the queues (entry 8k + x is the k-th tile of XCD x); shorter
+// queues are padded with blk = -1 entries, which exit at once.
+// MPS2_XCDS=8 with MPS2_TILE_GROUP (1..8) selects this order.  The default
+// (MPS2_XCDS=1, group 1) is one queue of single tiles, heaviest first: on the
+// D=4096 J1-J2 matvec the grouped order (groups of 2, 4 or 8) made the FP32
+// stage-4 launch 15-19% slower (4.67 -> 5.38..5.58 ms; FP64 unchanged).
+std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts) {
+  static const int kXcds = [] {
+    const char *e = std::getenv("MPS2_XCDS");
+    int x = e ? std::atoi(e) : 1;
+    return x == 8 ? 8 : 1;
+  }();
+  static const int kGroup = [] {
+    const char *e = std::getenv("MPS2_TILE_GROUP");
+    int g = e ? std::atoi(e) : (kXcds == 8 ? 4 : 1);
+    return g < 1 ? 1 : (g > 8 ? 8 : g);
+  }();
+  struct Group {
+    int blk, tm0, tm1, tn0, tn1;
+    double cost;
+  };
+  std::vector<Group> groups;
+  for (size_t b = 0; b < blks.size(); ++b) {
+    const int tm = (blks[b].M + ts - 1) / ts, tn = (blks[b].N + ts - 1) / ts;
+    for (int i = 0; i < tm; i += kGroup)
+      for (int j = 0; j < tn; j += kGroup) {
+        Group g{(int)b, i, std::min(tm, i + kGroup), j, std::min(tn, j + kGroup), 0.0};
+        g.cost = (double)(g.tm1 - g.tm0) * (g.tn1 - g.tn0) * (double)std::max<int64_t>(kc[b], 1);
+        groups.push_back(g);
+      }
+  }
+  std::stable_sort(groups.begin(), groups.end(), [](const Group &x, const Group &y) { return x.cost > y.cost; });
+  std::vector<std::vector<TileD>> q(kXcds);
+  std::vector<double> load(kXcds, 0.0);
+  for (const Group &g : groups) {
+    int x = 0;
+    for (int k = 1; k < kXcds; ++k)
+      if (load[k] < load[x]) x = k;
+    load[x] += g.cost;
+    for (int i = g.tm0; i < g.tm1; ++i)
+      for (int j = g.tn0; j < g.tn1; ++j) q[x].push_back(TileD{g.blk, i * ts, j * ts, 0});
+  }
+  size_t len = 0;
+  for (auto &v : q) len = std::max(len, v.size());
+  std::vector<TileD> out;
+  out.reserve(len * kXcds);
+  for (size_t k = 0; k < len; ++k)
+    for (int x = 0; x < kXcds; ++x) out.push_back(k < q[x].size() ? q[x][k] : TileD{-1, 0, 0, 0});
+  return out;
+}
 
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,

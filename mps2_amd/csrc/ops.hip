@@ -96,16 +96,7 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
     pairs[i].K = hp->pairs[i].K;
     pairs[i].pad = 0;
   }
-  // Tiles in descending order of their K extent (heaviest first, so the
-  // light ones fill the tail of the launch); consecutive launches of equal
-  // cost keep the round-robin dispatch over the XCDs in step.  (Assigning
-  // whole output blocks to XCDs cut the fabric traffic of the D=4096 stage-4
-  // launch 2.4x but made it 12% slower.)
-  struct TC {
-    TileD t;
-    int64_t cost;
-  };
-  std::vector<TC> tl, tl64;
+  std::vector<int64_t> kcs(hp->blks.size(), 0);
   for (size_t i = 0; i < hp->blks.size(); ++i) {
     const BlkH &b = hp->blks[i];
     blks[i].c_off = b.c_off;
@@ -119,23 +110,16 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
     blks[i].N = b.N;
     blks[i].pb = b.pair_begin;
     blks[i].pe = b.pair_end;
-    int64_t kc = 0;
-    for (int p = b.pair_begin; p < b.pair_end; ++p) kc += hp->pairs[p].K;
-    for (int m0 = 0; m0 < b.M; m0 += ts)
-      for (int n0 = 0; n0 < b.N; n0 += ts) tl.push_back({TileD{(int32_t)i, m0, n0, 0}, kc});
-    if (ts != 64)
-      for (int m0 = 0; m0 < b.M; m0 += 64)
-        for (int n0 = 0; n0 < b.N; n0 += 64) tl64.push_back({TileD{(int32_t)i, m0, n0, 0}, kc});
+    for (int p = b.pair_begin; p < b.pair_end; ++p) kcs[i] += hp->pairs[p].K;
   }
-  auto upload_sorted = [&](std::vector<TC> &v, int &count) {
-    std::stable_sort(v.begin(), v.end(), [](const TC &x, const TC &y) { return x.cost > y.cost; });
-    std::vector<TileD> t(v.size());
-    for (size_t i = 0; i < v.size(); ++i) t[i] = v[i].t;
+  // Tile lists ordered for the XCDs (schedule_tiles, gemm.hip).
+  auto upload_tiles = [&](int edge, int &count) {
+    std::vector<TileD> t = schedule_tiles(blks, kcs, edge);
     count = (int)t.size();
     return upload_bytes(c, t.data(), t.size() * sizeof(TileD));
   };
-  dp->tiles = upload_sorted(tl, dp->ntiles);
-  if (ts != 64) dp->tiles64 = upload_sorted(tl64, dp->ntiles64);
+  dp->tiles = upload_tiles(ts, dp->ntiles);
+  if (ts != 64) dp->tiles64 = upload_tiles(64, dp->ntiles64);
   else dp->tiles64 = dp->tiles, dp->ntiles64 = dp->ntiles;
   dp->blks = upload_bytes(c, blks.data(), blks.size() * sizeof(BlkD));
   dp->pairs = upload_bytes(c, pairs.data(), pairs.size() * sizeof(PairD));

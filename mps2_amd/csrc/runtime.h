@@ -187,6 +187,9 @@ struct PairD {
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0);
+// Tile list of one grouped-GEMM launch (tile edge ts) ordered for the XCDs.
+// kc[i] is block i's summed K extent (its per-tile cost).  See gemm.hip.
+std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts);
 
 // ------------------------------------------------------------------ ops (ops.hip)
 // Device-side contraction plan (descriptors uploaded once, cached by layout).

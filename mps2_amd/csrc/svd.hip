@@ -98,10 +98,10 @@ BufP upload(Ctx *c, const void *h, size_t bytes) {
 // Grouped GEMM from host-side block / pair lists (not cached).
 void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *C, const std::vector<BlkD> &blks,
               const std::vector<PairD> &pairs, int sub = 0) {
-  std::vector<TileD> tiles;
+  std::vector<int64_t> kc(blks.size(), 0);
   for (size_t i = 0; i < blks.size(); ++i)
-    for (int m0 = 0; m0 < blks[i].M; m0 += 64)
-      for (int n0 = 0; n0 < blks[i].N; n0 += 64) tiles.push_back(TileD{(int32_t)i, m0, n0, 0});
+    for (int p = blks[i].pb; p < blks[i].pe; ++p) kc[i] += pairs[p].K;
+  std::vector<TileD> tiles = schedule_tiles(blks, kc, 64);
   if (tiles.empty()) return;
   auto dt = upload(c, tiles.data(), tiles.size() * sizeof(TileD));
   auto db = upload(c, blks.data(), blks.size() * sizeof(BlkD));
