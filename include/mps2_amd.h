@@ -193,6 +193,22 @@ int mps2_copy_bench(mps2_ctx ctx, int64_t bytes, int reps, double *ms_per_copy);
  * be NULL (jobz 'N'). */
 int mps2_tridiag_lowest(mps2_ctx ctx, int n, const double *a, const double *b, double *eval, double *vec);
 
+/* Allocation-free form of mps2_tridiag_lowest.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  d_a (n values), d_b (n - 1 values) and d_out are device
+ * pointers the caller owns; d_out[0] receives the lowest eigenvalue and,
+ * when want_vec is nonzero, d_out[1..n] its unit eigenvector.  One kernel
+ * launch on the context's stream, with no allocation and no host sync (the
+ * caller synchronises, e.g. mps2_ctx_sync).  The kernel's work space is a
+ * static array per GPU, held by the library until the process ends, sized
+ * for n <= MPS2_NOALLOC_MAX_N; larger n returns MPS2_ERR_ARG.  Every call
+ * on one GPU uses that GPU's array, so calls that overlap on one GPU (other
+ * streams or threads) race. */
+#define MPS2_NOALLOC_MAX_N 4096
+int mps2_tridiag_lowest_noalloc(mps2_ctx ctx, int n, const double *d_a, const double *d_b, int want_vec,
+                                double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -594,6 +594,14 @@ int mps2_copy_bench(mps2_ctx ctx, int64_t bytes, int reps, double *ms_per_copy) 
   });
 }
 
+int mps2_tridiag_lowest_noalloc(mps2_ctx ctx, int n, const double *d_a, const double *d_b, int want_vec,
+                                double *d_out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    mps2::tridiag_lowest_noalloc(ctx->c.get(), n, d_a, d_b, want_vec, d_out);
+  });
+}
+
 int mps2_tridiag_lowest(mps2_ctx ctx, int n, const double *a, const double *b, double *eval, double *vec) {
   return guard([&] {
     need(ctx, "ctx");

@@ -26,6 +26,7 @@ Ctx::Ctx(int dev) : device(dev) {
   if (n <= 0) throw Error(kErrNoDevice, "no HIP device visible");
   if (dev < 0 || dev >= n) throw Error(kErrArg, "device index out of range");
   MPS2_HIP(hipSetDevice(dev));
+  tridiag_scratch_init(dev);
   MPS2_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   hipDeviceProp_t prop;
   MPS2_HIP(hipGetDeviceProperties(&prop, dev));

@@ -248,6 +248,12 @@ SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t d
 
 // Symmetric tridiagonal lowest eigenpair on the device (tridiag.hip).
 int tridiag_lowest_dev(Ctx *c, int n, const double *a, const double *b, double *eval, double *vec);
+// Allocation-free form on device data: d_out[0] = eigenvalue, d_out[1..n] =
+// eigenvector (want_vec); work space in a static per-device array.
+void tridiag_lowest_noalloc(Ctx *c, int n, const double *d_a, const double *d_b, int want_vec, double *d_out);
+// Takes the device address of that array; called once per device, with the
+// device current (context construction).
+void tridiag_scratch_init(int device);
 
 // Lanczos (lanczos.hip).
 struct LanczosOut {

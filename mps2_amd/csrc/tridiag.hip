@@ -9,10 +9,38 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "runtime.h"
 
 namespace mps2 {
+
+// Work space of the allocation-free entry point (mps2_tridiag_lowest_noalloc):
+// one array per GPU, held until the process ends, sized for matrices of up
+// to kNoAllocMaxN rows (the inverse iteration's 5n + 8 doubles).  Host code
+// never uses the name: set-up (tridiag_scratch_init, from the context
+// constructor with the device current) takes each device's address once and
+// kernels receive that address.  Calls that overlap on one GPU race on it.
+constexpr int kNoAllocMaxN = 4096;
+static __device__ __align__(256) double scratch[5 * kNoAllocMaxN + 8];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void tridiag_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) throw Error(kErrArg, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  MPS2_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error(kErrHip, "hipGetSymbolAddress returned no address for the tridiagonal scratch array");
+  g_scratch_ptr[device] = static_cast<double *>(p);
+}
+
 namespace {
 
 __device__ int sturm_count(int n, const double *a, const double *b, double x) {
@@ -33,6 +61,13 @@ __global__ __launch_bounds__(64) void tridiag_lowest_kernel(int n, const double 
   __shared__ double lo_s, hi_s;
   __shared__ int cnt_s[64];
   const int lane = threadIdx.x;
+  if (n == 1) {
+    if (lane == 0) {
+      out[0] = a[0];
+      if (want_vec) out[1] = 1.0;
+    }
+    return;
+  }
   if (lane == 0) {
     double lo = a[0], hi = a[0];
     for (int i = 0; i < n; ++i) {
@@ -157,6 +192,17 @@ int tridiag_lowest_dev(Ctx *c, int n, const double *a, const double *b, double *
   if (vec)
     for (int i = 0; i < n; ++i) vec[i] = o[1 + i];
   return 0;
+}
+
+void tridiag_lowest_noalloc(Ctx *c, int n, const double *d_a, const double *d_b, int want_vec, double *d_out) {
+  if (n < 1 || n > kNoAllocMaxN)
+    throw Error(kErrArg, "tridiag_lowest_noalloc: n = " + std::to_string(n) + " outside 1.." +
+                             std::to_string(kNoAllocMaxN) + " (the size of its static work space)");
+  if (!d_a || !d_out || (n > 1 && !d_b)) throw Error(kErrArg, "tridiag_lowest_noalloc: NULL device pointer");
+  double *work = g_scratch_ptr[c->device];
+  if (!work) throw Error(kErrState, "tridiag_lowest_noalloc: scratch not set up on this device");
+  tridiag_lowest_kernel<<<1, 64, 0, c->stream>>>(n, d_a, n > 1 ? d_b : d_a, want_vec ? 1 : 0, d_out, work);
+  MPS2_HIP(hipGetLastError());
 }
 
 }  // namespace mps2

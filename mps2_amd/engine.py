@@ -51,6 +51,7 @@ SIGNATURES = {
     "mps2_lanczos": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_double, C.c_int64, C.c_int, _i64p, _dp,
                                C.POINTER(_vp)]),
     "mps2_tridiag_lowest": (C.c_int, [_vp, C.c_int, _dp, _dp, _dp, _dp]),
+    "mps2_tridiag_lowest_noalloc": (C.c_int, [_vp, C.c_int, _vp, _vp, C.c_int, _vp]),
     "mps2_copy_bench": (C.c_int, [_vp, C.c_int64, C.c_int, _dp]),
     "mps2_heff_bench_shard": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, _dp]),
