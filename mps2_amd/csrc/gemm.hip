@@ -52,14 +52,13 @@ struct Mfma<float> {
 };
 
 constexpr int kThreads = 256;
-constexpr int kBK = 16;
 
 // Per-thread share of one BK-deep slab of A (BM x BK) and B (BK x BN),
 // loaded from global memory into registers, then stored to LDS: FP32 as
 // As[m][k], Bs[n][k] with row stride SK (k contiguous, rows 16-byte
 // aligned), FP64 as As[k][m], Bs[k][n] with strides PA, PB.  Out-of-range
 // elements read as zero.
-template <typename T, int BM, int BN, bool TA, bool TB>
+template <typename T, int BM, int BN, bool TA, bool TB, int kBK>
 struct Slab {
   static constexpr bool kQ = sizeof(T) == 4;
   static constexpr int NA = BM * kBK / kThreads;
@@ -148,20 +147,24 @@ struct Quad {
 
 // The (block pair, K slab) sequence of a tile is walked as one flat loop so
 // the global loads of the next slab (possibly of the next pair) overlap the
-// matrix-core work on the current one: registers -> LDS double buffer, one
-// barrier per slab.
+// matrix-core work on the current one.  NBUF = 2: registers -> LDS double
+// buffer, one barrier per slab; NBUF = 1: one LDS buffer (half the LDS, so
+// more workgroups per compute unit), two barriers per slab.  BK is the slab
+// depth; for FP32 lane group g covers k in [g BK/4, (g+1) BK/4) and the
+// MFMA steps walk that range four k at a time (one 16-byte LDS read per
+// fragment per four steps).
 // TAG only names the instance (profiling: isolates one stage's launches).
-template <typename T, int BM, int BN, bool TA, bool TB, int TAG>
+template <typename T, int BM, int BN, bool TA, bool TB, int TAG, int BK, int NBUF>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
                                                                  T *__restrict__ C, const TileD *__restrict__ tiles,
                                                                  const BlkD *__restrict__ blks,
                                                                  const PairD *__restrict__ pairs, int ntiles, int sub) {
   using V4 = typename Mfma<T>::v4;
-  using SlabT = Slab<T, BM, BN, TA, TB>;
+  using SlabT = Slab<T, BM, BN, TA, TB, BK>;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
-  __shared__ __align__(16) T As[2][SlabT::LA];
-  __shared__ __align__(16) T Bs[2][SlabT::LB];
+  __shared__ __align__(16) T As[NBUF][SlabT::LA];
+  __shared__ __align__(16) T Bs[NBUF][SlabT::LB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -191,7 +194,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   __syncthreads();
   int cur = 0;
   while (true) {
-    int np = p, nk0 = k0 + kBK;
+    int np = p, nk0 = k0 + BK;
     PairD npr = pr;
     if (nk0 >= pr.K) {
       np = p + 1;
@@ -202,25 +205,29 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
     if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
     if constexpr (SlabT::kQ) {
       constexpr int SK = SlabT::SK;
-      const float *as = As[cur] + (wm * WM + (lane & 15)) * SK + 4 * (lane >> 4);
-      const float *bs = Bs[cur] + (wn * WN + (lane & 15)) * SK + 4 * (lane >> 4);
-      Quad af[FM], bf[FN];
+      constexpr int KPL = BK / 4;   // k values per lane group and slab
+      const float *as = As[cur] + (wm * WM + (lane & 15)) * SK + KPL * (lane >> 4);
+      const float *bs = Bs[cur] + (wn * WN + (lane & 15)) * SK + KPL * (lane >> 4);
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i].load(as + i * 16 * SK);
+      for (int h = 0; h < KPL / 4; ++h) {
+        Quad af[FM], bf[FN];
 #pragma unroll
-      for (int j = 0; j < FN; ++j) bf[j].load(bs + j * 16 * SK);
+        for (int i = 0; i < FM; ++i) af[i].load(as + i * 16 * SK + 4 * h);
 #pragma unroll
-      for (int st = 0; st < 4; ++st)
+        for (int j = 0; j < FN; ++j) bf[j].load(bs + j * 16 * SK + 4 * h);
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int st = 0; st < 4; ++st)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i].v[st], bf[j].v[st], acc[i][j]);
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i].v[st], bf[j].v[st], acc[i][j]);
+      }
     } else {
       constexpr int PA = SlabT::PA, PB = SlabT::PB;
       const T *as = As[cur];
       const T *bs = Bs[cur];
 #pragma unroll
-      for (int ks = 0; ks < kBK; ks += 4) {
+      for (int ks = 0; ks < BK; ks += 4) {
         const int kr = ks + (lane >> 4);
         T af[FM], bf[FN];
 #pragma unroll
@@ -233,13 +240,20 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
           for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
       }
     }
-    if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
-    __syncthreads();
-    if (!has_next) break;
+    if constexpr (NBUF == 2) {
+      if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
+      __syncthreads();
+      if (!has_next) break;
+      cur ^= 1;
+    } else {
+      if (!has_next) break;
+      __syncthreads();
+      slab.store(As[0], Bs[0], tid);
+      __syncthreads();
+    }
     p = np;
     k0 = nk0;
     pr = npr;
-    cur ^= 1;
   }
   // ---- epilogue: element r of each 16x16 fragment is row Mfma<T>::row(lane, r), column lane%16
   T *Cb = C + blk.c_off;
@@ -259,22 +273,44 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
     }
 }
 
+template <typename T, int BM, int BN, int TAG, int BK, int NBUF>
+void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                    const BlkD *blks, const PairD *pairs, int sub) {
+  dim3 grid(ntiles), block(kThreads);
+  if (la == 0 && lb == 0)
+    grouped_gemm_kernel<T, BM, BN, false, false, TAG, BK, NBUF>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else if (la == 0 && lb == 1)
+    grouped_gemm_kernel<T, BM, BN, false, true, TAG, BK, NBUF>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else if (la == 1 && lb == 0)
+    grouped_gemm_kernel<T, BM, BN, true, false, TAG, BK, NBUF>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else
+    grouped_gemm_kernel<T, BM, BN, true, true, TAG, BK, NBUF>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+}
+
+// FP32 128x128 slab variant (MPS2_GEMM_VARIANT): 0 = BK 16, double LDS
+// buffer; 1 = BK 32, double buffer; 2 = BK 32, one buffer.
+int f32_variant() {
+  static const int v = [] {
+    const char *e = std::getenv("MPS2_GEMM_VARIANT");
+    int x = e ? std::atoi(e) : 0;
+    return x < 0 || x > 2 ? 0 : x;
+  }();
+  return v;
+}
+
 template <typename T, int BM, int BN, int TAG>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                   const BlkD *blks, const PairD *pairs, int sub) {
-  dim3 grid(ntiles), block(kThreads);
-  if (la == 0 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, false, false, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
-                                                                                     ntiles, sub);
-  else if (la == 0 && lb == 1)
-    grouped_gemm_kernel<T, BM, BN, false, true, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
-                                                                                    ntiles, sub);
-  else if (la == 1 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, true, false, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
-                                                                                    ntiles, sub);
-  else
-    grouped_gemm_kernel<T, BM, BN, true, true, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs,
-                                                                                   ntiles, sub);
+  if constexpr (sizeof(T) == 4 && BM == 128) {
+    const int v = f32_variant();
+    if (v == 1) return launch_tiles_v<T, BM, BN, TAG, 32, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    if (v == 2) return launch_tiles_v<T, BM, BN, TAG, 32, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  }
+  launch_tiles_v<T, BM, BN, TAG, 16, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
 }
 
 }  // namespace
