@@ -273,6 +273,216 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
     }
 }
 
+// ---------------------------------------------------------------- FP32, direct-to-LDS loads
+// FP32 grouped GEMM for row-major A (M x K) and row-major B (K x N) whose
+// slabs go from global memory straight into LDS (global_load_lds_dword: lane
+// i of a wavefront writes the dword at LDS base + 4 i), so no registers stage
+// them and NST - 1 slabs stay in flight while one is multiplied.  Slab depth
+// 16.  LDS per stage:
+//   A: 128 rows x 16 k, row m at 16 m, k stored at (k ^ ((m >> 2 & 3) << 2))
+//      (quads permuted per row group: the 16-byte fragment reads of 16
+//      consecutive rows fall on distinct banks);
+//   B: 16 k rows x 128 n, row k at kLdB * k (kLdB = 132: the four lane groups'
+//      rows of a fragment read fall on distinct banks).
+// Out-of-range rows and columns read clamped, valid addresses (their products
+// land only in outputs the epilogue does not store); k beyond the pair's K
+// reads B from a zero array, so those terms vanish.
+__device__ float g_dma_zero[64];
+
+// s_waitcnt vmcnt(N) lgkmcnt(0) (expcnt not waited for), gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8].
+template <int N>
+__device__ __forceinline__ void dma_wait() {
+  static_assert(N >= 0 && N < 64, "vmcnt holds 6 bits");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4));
+}
+
+// Workgroup barrier without the release/acquire fences of __syncthreads (a
+// workgroup-scope release would wait for every load-to-LDS still in flight,
+// the ones the pipeline keeps in flight on purpose).  Each thread first
+// waits for its own loads of the slab about to be read (dma_wait) and for its
+// LDS reads (lgkmcnt 0), so after the barrier that slab is complete in LDS
+// and no wavefront still reads the stage the next loads overwrite; the signal
+// fences keep the compiler from moving memory accesses across it.
+__device__ __forceinline__ void dma_barrier() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+constexpr int kDmaBK = 16;
+constexpr int kLdB = 132;
+constexpr int kDmaA = 128 * kDmaBK;   // floats per A stage
+constexpr int kDmaB = kDmaBK * kLdB;  // floats per B stage
+
+// One step's MFMA operands from LDS: four A quads at a + 1024 i bytes and
+// sixteen B values at b + 4 kLdB s + 64 j bytes, in one asm block that waits
+// for its own reads (lgkmcnt 0) before it ends, so every output is complete
+// when the block returns.  The compiler does not see these reads as LDS
+// accesses and so adds no wait for the loads-to-LDS still in flight; the
+// explicit waits and barriers of the loop order them.
+struct Frags {
+  float4 a[4];
+  float b[4][4];  // [s][j]
+};
+__device__ __forceinline__ void read_frags(Frags &f, uint32_t a, uint32_t b) {
+  asm volatile(
+      "ds_read_b128 %0, %20\n\t"
+      "ds_read_b128 %1, %20 offset:1024\n\t"
+      "ds_read_b128 %2, %20 offset:2048\n\t"
+      "ds_read_b128 %3, %20 offset:3072\n\t"
+      "ds_read_b32 %4, %21\n\t"
+      "ds_read_b32 %5, %21 offset:64\n\t"
+      "ds_read_b32 %6, %21 offset:128\n\t"
+      "ds_read_b32 %7, %21 offset:192\n\t"
+      "ds_read_b32 %8, %21 offset:528\n\t"
+      "ds_read_b32 %9, %21 offset:592\n\t"
+      "ds_read_b32 %10, %21 offset:656\n\t"
+      "ds_read_b32 %11, %21 offset:720\n\t"
+      "ds_read_b32 %12, %21 offset:1056\n\t"
+      "ds_read_b32 %13, %21 offset:1120\n\t"
+      "ds_read_b32 %14, %21 offset:1184\n\t"
+      "ds_read_b32 %15, %21 offset:1248\n\t"
+      "ds_read_b32 %16, %21 offset:1584\n\t"
+      "ds_read_b32 %17, %21 offset:1648\n\t"
+      "ds_read_b32 %18, %21 offset:1712\n\t"
+      "ds_read_b32 %19, %21 offset:1776\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(f.a[0]), "=&v"(f.a[1]), "=&v"(f.a[2]), "=&v"(f.a[3]), "=&v"(f.b[0][0]), "=&v"(f.b[0][1]),
+        "=&v"(f.b[0][2]), "=&v"(f.b[0][3]), "=&v"(f.b[1][0]), "=&v"(f.b[1][1]), "=&v"(f.b[1][2]),
+        "=&v"(f.b[1][3]), "=&v"(f.b[2][0]), "=&v"(f.b[2][1]), "=&v"(f.b[2][2]), "=&v"(f.b[2][3]),
+        "=&v"(f.b[3][0]), "=&v"(f.b[3][1]), "=&v"(f.b[3][2]), "=&v"(f.b[3][3])
+      : "v"(a), "v"(b)
+      : "memory");
+}
+static_assert(kLdB * 4 == 528, "B row offsets in read_frags assume kLdB = 132");
+
+template <int NST, int TAG>
+__global__ __launch_bounds__(kThreads) void grouped_gemm_dma_kernel(const float *__restrict__ A,
+                                                                     const float *__restrict__ B, float *__restrict__ C,
+                                                                     const TileD *__restrict__ tiles,
+                                                                     const BlkD *__restrict__ blks,
+                                                                     const PairD *__restrict__ pairs, int ntiles,
+                                                                     int sub) {
+  static_assert(NST == 3 || NST == 4, "three or four stages");
+  using V4 = Mfma<float>::v4;
+  constexpr int WM = 64, WN = 64, FM = 4, FN = 4;
+  __shared__ __align__(16) float As[NST][kDmaA];
+  __shared__ __align__(16) float Bs[NST][kDmaB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tix = blockIdx.x;
+  if (tix >= ntiles) return;
+  const TileD tile = tiles[tix];
+  if (tile.blk < 0) return;
+  const BlkD blk = blks[tile.blk];
+  const int M = blk.M, N = blk.N, m0 = tile.m0, n0 = tile.n0;
+  if (m0 >= M || n0 >= N || blk.pb >= blk.pe) return;
+  int nslab = 0;
+  for (int p = blk.pb; p < blk.pe; ++p) nslab += (max(pairs[p].K, 0) + kDmaBK - 1) / kDmaBK;
+
+  // Issue state: the slab (pair ip, depth ik0) whose loads go out next.
+  // A: 8 loads per thread and slab; load j of wave w covers rows
+  // 32 w + 4 j .. +3 (lane i: row + i / 16, LDS column i % 16, holding k
+  // column (i % 16) ^ 4 (j % 4): the quad order of row group j).
+  // B: 8 loads per thread; load j covers k row 4 w + j / 2, columns
+  // 64 (j % 2) .. +63 (lane i: column + i).
+  int ip = blk.pb, ik0 = 0;
+  while (ip < blk.pe && pairs[ip].K <= 0) ++ip;  // pairs without K contribute nothing
+  if (ip >= blk.pe) ip = blk.pb;                 // (then nslab == 0 and nothing is issued)
+  PairD ipr = pairs[ip];
+  auto issue = [&](int stage) {
+    const float *Ab = A + ipr.a_off;
+    const float *Bb = B + ipr.b_off;
+    const int K = ipr.K;
+    float *as = As[stage];
+    float *bs = Bs[stage];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int r = 32 * wave + 4 * j + (lane >> 4);
+      const int k = ik0 + ((lane & 15) ^ ((j & 3) << 2));
+      const uint32_t off = (uint32_t)min(m0 + r, M - 1) * (uint32_t)K + (uint32_t)min(k, K - 1);
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Ab + off),
+                                       (__attribute__((address_space(3))) void *)(as + (32 * wave + 4 * j) * 16), 4,
+                                       0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int kr = 4 * wave + (j >> 1), c0 = 64 * (j & 1), k = ik0 + kr;
+      const uint32_t off = (uint32_t)k * (uint32_t)N + (uint32_t)min(n0 + c0 + lane, N - 1);
+      const float *src = k < K ? Bb + off : g_dma_zero + lane;
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)src,
+                                       (__attribute__((address_space(3))) void *)(bs + kr * kLdB + c0), 4, 0, 0);
+    }
+    ik0 += kDmaBK;
+    if (ik0 >= K) {
+      ik0 = 0;
+      do {
+        ++ip;
+      } while (ip < blk.pe && pairs[ip].K <= 0);
+      if (ip < blk.pe) ipr = pairs[ip];
+    }
+  };
+
+  V4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+  // LDS byte addresses of this lane's fragments within a stage: A row
+  // m = 64 wm + lane % 16 (+16 i), quad lane / 16 permuted by the row group
+  // ((m >> 2) & 3 = (lane % 16) >> 2); B row k = 4 (lane / 16) (+ s), column
+  // 64 wn + lane % 16 (+16 j).
+  const int ma = wm * WM + (lane & 15);
+  const uint32_t a_lane = (uint32_t)(ma * 16 + ((4 * (lane >> 4)) ^ (((lane & 15) >> 2) << 2))) * 4u;
+  const uint32_t b_lane = (uint32_t)((4 * (lane >> 4)) * kLdB + wn * WN + (lane & 15)) * 4u;
+  const uint32_t a_base = (uint32_t)(uintptr_t)&As[0][0], b_base = (uint32_t)(uintptr_t)&Bs[0][0];
+
+  int issued = 0;
+  for (; issued < NST - 1 && issued < nslab; ++issued) issue(issued);
+  int st = 0;  // stage of slab t
+  for (int t = 0; t < nslab; ++t) {
+    // loads still allowed in flight: those of slabs t+1 .. t+NST-2
+    const int ahead = min(NST - 2, nslab - 1 - t);
+    if (ahead >= 2) dma_wait<32>();
+    else if (ahead == 1) dma_wait<16>();
+    else dma_wait<0>();
+    dma_barrier();
+    if (issued < nslab) {
+      issue(issued % NST);
+      ++issued;
+    }
+    Frags f;
+    read_frags(f, a_base + (uint32_t)st * (kDmaA * 4u) + a_lane, b_base + (uint32_t)st * (kDmaB * 4u) + b_lane);
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4)
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const float av = s4 == 0 ? f.a[i].x : s4 == 1 ? f.a[i].y : s4 == 2 ? f.a[i].z : f.a[i].w;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<float>::op(av, f.b[s4][j], acc[i][j]);
+      }
+    st = st + 1 == NST ? 0 : st + 1;
+  }
+
+  float *Cb = C + blk.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm * WM + i * 16 + Mfma<float>::row(lane, r);
+        if (gm < M && gn < N) {
+          if (sub) Cb[(int64_t)gm * N + gn] -= acc[i][j][r];
+          else Cb[(int64_t)gm * N + gn] = acc[i][j][r];
+        }
+      }
+    }
+}
+
 template <typename T, int BM, int BN, int TAG, int BK, int NBUF>
 void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                     const BlkD *blks, const PairD *pairs, int sub) {
@@ -292,12 +502,13 @@ void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const 
 }
 
 // FP32 128x128 slab variant (MPS2_GEMM_VARIANT): 0 = BK 16, double LDS
-// buffer; 1 = BK 32, double buffer; 2 = BK 32, one buffer.
+// buffer; 1 = BK 32, double buffer; 2 = BK 32, one buffer; 3, 4 = direct-to-LDS
+// loads with 3 or 4 stages (row-major A and B only).
 int f32_variant() {
   static const int v = [] {
     const char *e = std::getenv("MPS2_GEMM_VARIANT");
     int x = e ? std::atoi(e) : 0;
-    return x < 0 || x > 2 ? 0 : x;
+    return x < 0 || x > 4 ? 0 : x;
   }();
   return v;
 }
@@ -307,6 +518,16 @@ void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const Ti
                   const BlkD *blks, const PairD *pairs, int sub) {
   if constexpr (sizeof(T) == 4 && BM == 128) {
     const int v = f32_variant();
+    if (v == 3 && la == 0 && lb == 0) {
+      grouped_gemm_dma_kernel<3, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,
+                                                                                     ntiles, sub);
+      return;
+    }
+    if (v == 4 && la == 0 && lb == 0) {
+      grouped_gemm_dma_kernel<4, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,
+                                                                                     ntiles, sub);
+      return;
+    }
     if (v == 1) return launch_tiles_v<T, BM, BN, TAG, 32, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
     if (v == 2) return launch_tiles_v<T, BM, BN, TAG, 32, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
   }

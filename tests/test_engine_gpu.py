@@ -451,3 +451,31 @@ def test_tensor_outlives_its_context():
     assert H.max_rel_diff(d3.get(), A) == 0.0
     d3.free()
     c2.close()
+
+
+def test_fp32_large_tile_launch_vs_numpy(ctx):
+    """A contraction large enough for the FP32 128x128 tile launch (16 output
+    blocks of 500..512 square, at least one tile per compute unit), with
+    one to two block pairs per output block and K extents that are not multiples of
+    the slab depth (5 x 3 = 15, 6 x 5 = 30, ...): every block against the oracle to the FP32
+    bound 1e-5 sum_k |a||b|, and bit-identical on a second run."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(17)
+    qs = list(range(16))
+    m = Leg(IN, [((q,), 500 + (q * 7) % 13) for q in qs])
+    n = Leg(OUT, [((q,), 512 - (q * 5) % 11) for q in qs])
+    k1 = Leg(OUT, [((q,), 5 + q % 3) for q in range(0, 14, 2)])
+    k2 = Leg(OUT, [((q,), 3 + 2 * q) for q in range(4)])
+    A = BTensor.random([m, k1, k2], (0,), rng)
+    B = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
+    dA, dB = ctx.put(A), ctx.put(B)
+    got = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
+    again = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
+    ref = O.contract(A, B, [[1, 2], [0, 1]])
+    assert sorted(got.blocks) == sorted(ref.blocks) and len(ref.blocks) == 16
+    absA = BTensor(A.legs, A.qw, {c: np.abs(b) for c, b in A.blocks.items()})
+    absB = BTensor(B.legs, B.qw, {c: np.abs(b) for c, b in B.blocks.items()})
+    bound = O.contract(absA, absB, [[1, 2], [0, 1]])
+    for c in ref.blocks:
+        assert np.array_equal(got.blocks[c], again.blocks[c]), c
+        assert (np.abs(got.blocks[c] - ref.blocks[c]) <= 1e-5 * bound.blocks[c] + 1e-30).all(), c
