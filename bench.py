@@ -40,6 +40,9 @@ sys.path.insert(0, ROOT)
 METRIC = "sweep wall-time + H·ψ TFLOP/s at D=4096: FP64 vs mixed, |ΔE|/|E|≤1e-10"
 PEAK = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense matrix-core TFLOP/s
 T_START = time.perf_counter()
+# The GEMM autotuner's decisions of this run go to a fresh file, summarised in
+# the output (heuristic against chosen kernel, per tuned launch shape).
+TUNE_CACHE = os.environ.setdefault("MPS2_TUNE_CACHE", os.path.join("/tmp", f"mps2_bench_tune_{os.getpid()}.txt"))
 
 
 def log(*a):
@@ -123,6 +126,21 @@ def cpu_baseline(tensors, flops, budget_s, gpu_fp64_matvecs_per_update, n_update
         out["heisenberg_l16_full_run"] = dict(config="Heisenberg OBC L=16, Sz=0, Dmax=64, 4 sweeps, Lanczos 1e-10",
                                               **runs)
     return out
+
+
+def tune_summary():
+    """The autotuner's decisions of this run: for each tuned launch shape the
+    heuristic kernel's and the chosen kernel's best probe time."""
+    try:
+        rows = [ln.split() for ln in open(TUNE_CACHE) if ln.strip()]
+    except OSError:
+        return None
+    heur = sum(float(r[5]) for r in rows)
+    best = sum(float(r[4]) for r in rows)
+    changed = sum(1 for r in rows if float(r[4]) < float(r[5]))
+    return dict(shapes_tuned=len(rows), shapes_changed=changed, heuristic_ms_sum=round(heur, 3),
+                tuned_ms_sum=round(best, 3), speedup=round(heur / best, 4) if best > 0 else None,
+                choices=sorted({f"{'fp64' if r[1] == '1' else 'fp32'} tile {r[2]} variant {r[3]}" for r in rows}))
 
 
 def measured_traffic(args):
@@ -223,10 +241,17 @@ def main():
 
     centre = N // 2 - 1
     t0 = time.perf_counter()
-    mb = model.bench_bond(centre, warmup=2, reps=args.matvec_reps)
+    mb = model.bench_bond(centre, warmup=8, reps=args.matvec_reps)   # warm-up covers the tuning probes
     log(f"centre-bond per-stage timing ({time.perf_counter() - t0:.1f}s): stage ms fp64 {mb['fp64']['stage_ms']} "
         f"mixed {mb['mixed']['stage_ms']}")
     L_, W1_, W2_, R_, psi_ = model.bench_tensors()
+    # the same centre-bond matvec with the heuristic kernels only
+    engine.check(engine.lib().mps2_ctx_set_tuning(ctxh, 0))
+    heur = {k: engine.heff_bench(L_, W1_, W2_, R_, psi_, "cent", p, warmup=1, reps=args.matvec_reps)
+            for k, p in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED))}
+    engine.check(engine.lib().mps2_ctx_set_tuning(ctxh, 1))
+    log(f"centre-bond stage ms with heuristic kernels: fp64 {[round(x, 3) for x in heur['fp64']['stage_ms']]} "
+        f"mixed {[round(x, 3) for x in heur['mixed']['stage_ms']]}")
 
     # the steps: exactly --steps centre-bond matvecs per precision
     steps = {}
@@ -350,6 +375,8 @@ def main():
         "sweep_counters": {k: {c: res[k]["counters"][c] for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls",
                                                                   "hip_malloc")} for k in res},
         "stage_ms": {"fp64": mb["fp64"]["stage_ms"], "mixed": mb["mixed"]["stage_ms"]},
+        "stage_ms_heuristic": {k: heur[k]["stage_ms"] for k in heur},
+        "autotune": tune_summary(),
         "roofline_fp64": dict(bound="mfma", achieved=round(d64_ach, 3), peak=PEAK["fp64"], unit="TFLOP/s",
                               frac=round(d64_ach / PEAK["fp64"], 4), flops_per_launch=d64["dom_flops"],
                               ms_per_launch=d64["dom_ms"]),

@@ -76,6 +76,16 @@ int mps2_ctx_reset_stats(mps2_ctx ctx);
  * mps2_ctx_reset_stats. */
 int mps2_ctx_counter(mps2_ctx ctx, const char *name, double *value);
 
+/* Grouped-GEMM autotuning of the context's contraction plans (no reference
+ * counterpart: the reference's contractions run in MKL).  mode 1 (default):
+ * the first launches of an FP32 plan of at least 1e9 flops time the candidate
+ * kernels, whose results are identical, and keep the fastest, shared by
+ * plans of the same block-shape histogram and stored in a cache file
+ * ($MPS2_TUNE_CACHE, else ~/.cache/mps2_amd/gemm_tune_gfx950.txt); mode 0:
+ * the heuristic kernel only.  Counters "tune_probes", "tune_decisions",
+ * "tune_changed", "tune_cache_hits". */
+int mps2_ctx_set_tuning(mps2_ctx ctx, int mode);
+
 /* Tensors.  Replace GQTensor construction, element fill and the stream I/O
  * used by TenVec::LoadTen/DumpTen (one_dim_tn/framework/ten_vec.h:50-85). */
 int mps2_ten_import(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_ten *out);

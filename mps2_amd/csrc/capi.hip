@@ -119,6 +119,14 @@ int mps2_ctx_reset_stats(mps2_ctx ctx) {
   });
 }
 
+int mps2_ctx_set_tuning(mps2_ctx ctx, int mode) {
+  return guard([&] {
+    need(ctx, "ctx");
+    if (mode != 0 && mode != 1) throw mps2::Error(mps2::kErrArg, "tuning mode must be 0 or 1");
+    ctx->c->tune_mode = mode;
+  });
+}
+
 int mps2_ctx_counter(mps2_ctx ctx, const char *name, double *value) {
   return guard([&] {
     need(ctx, "ctx");

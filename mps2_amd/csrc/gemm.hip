@@ -501,35 +501,30 @@ void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const 
         <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
 }
 
-// FP32 128x128 slab variant (MPS2_GEMM_VARIANT): 0 = BK 16, double LDS
-// buffer; 1 = BK 32, double buffer; 2 = BK 32, one buffer; 3, 4 = direct-to-LDS
-// loads with 3 or 4 stages (row-major A and B only).
-int f32_variant() {
+// FP32 128x128 slab loading: 0 = registers -> LDS double buffer, 4 = direct-to-
+// LDS with four stages (row-major A and B).  MPS2_GEMM_VARIANT (0 or 4)
+// forces one for every such launch; otherwise the caller's choice runs.
+int forced_variant() {
   static const int v = [] {
     const char *e = std::getenv("MPS2_GEMM_VARIANT");
-    int x = e ? std::atoi(e) : 0;
-    return x < 0 || x > 4 ? 0 : x;
+    if (!e) return -1;
+    const int x = std::atoi(e);
+    return x == 0 || x == 4 ? x : -1;
   }();
   return v;
 }
 
 template <typename T, int BM, int BN, int TAG>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                  const BlkD *blks, const PairD *pairs, int sub) {
+                  const BlkD *blks, const PairD *pairs, int sub, int variant) {
   if constexpr (sizeof(T) == 4 && BM == 128) {
-    const int v = f32_variant();
-    if (v == 3 && la == 0 && lb == 0) {
-      grouped_gemm_dma_kernel<3, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,
-                                                                                     ntiles, sub);
-      return;
-    }
+    const int f = forced_variant();
+    const int v = f >= 0 ? f : variant;
     if (v == 4 && la == 0 && lb == 0) {
       grouped_gemm_dma_kernel<4, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,
                                                                                      ntiles, sub);
       return;
     }
-    if (v == 1) return launch_tiles_v<T, BM, BN, TAG, 32, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
-    if (v == 2) return launch_tiles_v<T, BM, BN, TAG, 32, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
   }
   launch_tiles_v<T, BM, BN, TAG, 16, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
 }
@@ -597,25 +592,26 @@ std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vect
 
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag) {
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag,
+                         int variant) {
   if (ntiles <= 0) return;
   const bool big = tile_m == 128 && tile_n == 128;
   if (tag == 1) {
-    if (big) launch_tiles<T, 128, 128, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
-    else launch_tiles<T, 64, 64, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    if (big) launch_tiles<T, 128, 128, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
+    else launch_tiles<T, 64, 64, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
   } else if (tag == 2) {
-    if (big) launch_tiles<T, 128, 128, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
-    else launch_tiles<T, 64, 64, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    if (big) launch_tiles<T, 128, 128, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
+    else launch_tiles<T, 64, 64, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
   } else {
-    if (big) launch_tiles<T, 128, 128, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
-    else launch_tiles<T, 64, 64, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+    if (big) launch_tiles<T, 128, 128, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
+    else launch_tiles<T, 64, 64, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
   }
   c->gemm_launches++;
 }
 
 template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *,
-                                          int, const BlkD *, const PairD *, int, int, int, int);
+                                          int, const BlkD *, const PairD *, int, int, int, int, int);
 template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
-                                         const BlkD *, const PairD *, int, int, int, int);
+                                         const BlkD *, const PairD *, int, int, int, int, int);
 
 }  // namespace mps2

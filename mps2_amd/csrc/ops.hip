@@ -118,9 +118,10 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
     count = (int)t.size();
     return upload_bytes(c, t.data(), t.size() * sizeof(TileD));
   };
-  dp->tiles = upload_tiles(ts, dp->ntiles);
-  if (ts != 64) dp->tiles64 = upload_tiles(64, dp->ntiles64);
-  else dp->tiles64 = dp->tiles, dp->ntiles64 = dp->ntiles;
+  dp->tiles64 = upload_tiles(64, dp->ntiles64);
+  dp->tiles128 = upload_tiles(128, dp->ntiles128);
+  if (ts == 128) dp->tiles = dp->tiles128, dp->ntiles = dp->ntiles128;
+  else dp->tiles = dp->tiles64, dp->ntiles = dp->ntiles64;
   dp->blks = upload_bytes(c, blks.data(), blks.size() * sizeof(BlkD));
   dp->pairs = upload_bytes(c, pairs.data(), pairs.size() * sizeof(PairD));
   if (hp->permute_a) {
@@ -235,9 +236,11 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   const int tag = (c->tag_stage >= 0 && c->stage_idx == c->tag_stage) ? (c->shard_g > 1 ? 2 : 1) : 0;
   c->stage_idx++;
   const bool fp64 = sizeof(T) == sizeof(double);
-  const TileD *tiles = static_cast<const TileD *>((fp64 ? dp.tiles64 : dp.tiles)->p);
-  int ntiles = fp64 ? dp.ntiles64 : dp.ntiles;
-  int tm_ = fp64 ? 64 : dp.tile_m, tn_ = fp64 ? 64 : dp.tile_n;
+  PlanTune::Probe probe;
+  const GemmChoice ch = tag == 2 ? GemmChoice{dp.tile_m, 0} : tune_select(c, dp, fp64, &probe);
+  const TileD *tiles = static_cast<const TileD *>((ch.tile == 128 ? dp.tiles128 : dp.tiles64)->p);
+  int ntiles = ch.tile == 128 ? dp.ntiles128 : dp.ntiles64;
+  int tm_ = ch.tile, tn_ = ch.tile;
   if (tag == 2) {
     const DevPlan::Share &sh = plan_share(c, dp, c->shard_g, c->shard_r);
     tiles = sh.ntiles ? static_cast<const TileD *>(sh.tiles->p) : nullptr;
@@ -245,8 +248,10 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
     tm_ = dp.tile_m, tn_ = dp.tile_n;  // the share's list is built with the plan's tile size
     c->shard_flops = sh.flops;
   }
+  tune_probe_begin(c, dp, fp64, &probe);
   launch_grouped_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, tiles, ntiles, static_cast<const BlkD *>(dp.blks->p),
-                         static_cast<const PairD *>(dp.pairs->p), tm_, tn_, 0, tag);
+                         static_cast<const PairD *>(dp.pairs->p), tm_, tn_, 0, tag, ch.variant);
+  tune_probe_end(c, dp, fp64, &probe);
   if (c->time_gemms) {
     MPS2_HIP(hipEventRecord(tm.stop, c->stream));
     tm.flops = dp.hp->flops;

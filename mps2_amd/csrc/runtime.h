@@ -101,6 +101,8 @@ struct Ctx {
   // pointer to its context.
   int64_t nbufs = 0;
   bool closing = false;
+  // Grouped-GEMM autotuning (tune.hip): 1 = tune large plans, 0 = heuristic.
+  int tune_mode = 1;
 
   explicit Ctx(int dev);
   ~Ctx();
@@ -184,12 +186,43 @@ struct PairD {
   int64_t a_off, b_off;
   int32_t K, pad;
 };
+// variant (FP32 128x128 tiles only): 0 = register-staged slabs, 4 = direct-to-LDS
+// slabs with four stages (row-major A and B only).
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0);
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0,
+                         int variant = 0);
 // Tile list of one grouped-GEMM launch (tile edge ts) ordered for the XCDs.
 // kc[i] is block i's summed K extent (its per-tile cost).  See gemm.hip.
 std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts);
+
+// ------------------------------------------------------------------ autotuning (tune.hip)
+// A grouped-GEMM kernel choice: tile edge (64 or 128) and, for FP32 128x128
+// tiles, the slab-loading variant of launch_grouped_gemm.
+struct GemmChoice {
+  int tile = 64;
+  int variant = 0;
+};
+// Tuning state of one plan and precision: candidates, the timed probes and
+// the decision (index into cand, -1 while undecided).
+struct PlanTune {
+  std::vector<GemmChoice> cand;
+  int decided = -1;
+  int next = 0;                 // probes issued
+  bool keyed = false;
+  uint64_t key = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  std::vector<int> ev_cand;
+  double best_ms = 0, heuristic_ms = 0;
+  struct Probe {
+    bool active = false;
+    int cand = 0;
+  };
+  PlanTune() = default;
+  PlanTune(const PlanTune &) = delete;
+  PlanTune &operator=(const PlanTune &) = delete;
+  ~PlanTune();
+};
 
 // ------------------------------------------------------------------ ops (ops.hip)
 // Device-side contraction plan (descriptors uploaded once, cached by layout).
@@ -197,9 +230,10 @@ struct DevPlan {
   std::shared_ptr<ContractPlan> hp;
   BufP tiles, blks, pairs;
   int ntiles = 0;
-  int tile_m = 64, tile_n = 64;
-  BufP tiles64;  // the same output tiled 64x64 (FP64 launches)
-  int ntiles64 = 0;
+  int tile_m = 64, tile_n = 64;   // heuristic FP32 tile (tiles holds its list)
+  BufP tiles64, tiles128;         // the output tiled 64x64 and 128x128
+  int ntiles64 = 0, ntiles128 = 0;
+  mutable PlanTune tune[2];       // [0] FP64, [1] FP32
   BufP perm_a, perm_b, perm_a_start, perm_b_start;   // permutation descriptors
   int nperm_a = 0, nperm_b = 0;
   // tiles of one owner's output blocks, by (owners, rank): tiles, count, flops
@@ -210,6 +244,12 @@ struct DevPlan {
   };
   mutable std::map<std::pair<int, int>, Share> shares;
 };
+
+// Kernel for one launch of plan dp; with probe->active the caller brackets
+// the launch with tune_probe_begin / tune_probe_end.
+GemmChoice tune_select(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe);
+void tune_probe_begin(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe);
+void tune_probe_end(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe);
 
 std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
                                   const std::vector<int> &bx);

@@ -1,0 +1,242 @@
+// Autotuner of the grouped-GEMM launches of contraction plans.
+//
+// Candidates are kernels whose per-element accumulation order is the same
+// (same slab depth and MFMA step order), so the choice changes the time of a
+// launch, never its result: for FP32, 128x128 tiles with register-staged
+// slabs, 128x128 tiles with direct-to-LDS slabs (row-major operands only) and
+// 64x64 tiles.  FP64 launches keep 64x64 tiles.
+// The first launches of a large plan run each candidate twice in turn, timed
+// with HIP events on the context stream (no host sync: the events are read
+// once the last one has completed, until then the heuristic kernel runs).
+// The fastest candidate is kept for the plan and, under the plan's shape key
+// (a hash of the histogram of its output blocks' M, N and summed K in
+// quarter-octave bins, blocks below 1e-3 of the launch's flops left out, with
+// the operand layouts), shared by every later plan of the same shape in
+// the process and appended to a cache file ($MPS2_TUNE_CACHE, else
+// $XDG_CACHE_HOME or ~/.cache, mps2_amd/gemm_tune_gfx950.txt) that later
+// processes read.  MPS2_TUNE=0, or mps2_ctx_set_tuning(ctx, 0), keeps the
+// heuristic choice.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+
+#include "runtime.h"
+
+namespace mps2 {
+
+namespace {
+
+constexpr double kTuneMinFlops = 1e9;   // smaller launches keep the heuristic
+constexpr int kTuneRounds = 2;          // timed runs per candidate
+
+struct CacheEntry {
+  GemmChoice choice;
+  double best_ms = 0, heuristic_ms = 0;
+};
+
+std::mutex g_tune_mu;
+std::map<std::pair<uint64_t, int>, CacheEntry> g_tune;   // (shape key, fp64) -> choice
+bool g_tune_loaded = false;
+
+std::string cache_path() {
+  if (const char *e = std::getenv("MPS2_TUNE_CACHE")) return e;
+  std::string base;
+  if (const char *x = std::getenv("XDG_CACHE_HOME")) base = x;
+  else if (const char *h = std::getenv("HOME")) base = std::string(h) + "/.cache";
+  else return "";
+  return base + "/mps2_amd/gemm_tune_gfx950.txt";
+}
+
+void load_cache_locked() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  const std::string p = cache_path();
+  if (p.empty()) return;
+  std::ifstream f(p);
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream s(line);
+    std::string key;
+    int fp64 = 0, tile = 0, variant = 0;
+    double best = 0, heur = 0;
+    if (!(s >> key >> fp64 >> tile >> variant >> best >> heur)) continue;
+    if ((tile != 64 && tile != 128) || (variant != 0 && variant != 4)) continue;
+    CacheEntry ce;
+    ce.choice = GemmChoice{tile, variant};
+    ce.best_ms = best;
+    ce.heuristic_ms = heur;
+    g_tune[{std::strtoull(key.c_str(), nullptr, 16), fp64}] = ce;
+  }
+}
+
+void append_cache(uint64_t key, int fp64, const CacheEntry &ce) {
+  const std::string p = cache_path();
+  if (p.empty()) return;
+  const auto slash = p.find_last_of('/');
+  if (slash != std::string::npos) {
+    // create the directory chain (errors leave the cache in memory only)
+    std::string dir = p.substr(0, slash);
+    for (size_t i = 1; i <= dir.size(); ++i)
+      if (i == dir.size() || dir[i] == '/') (void)mkdir(dir.substr(0, i).c_str(), 0755);
+  }
+  std::ofstream f(p, std::ios::app);
+  if (!f) return;
+  char buf[160];
+  std::snprintf(buf, sizeof buf, "%016llx %d %d %d %.6f %.6f\n", (unsigned long long)key, fp64, ce.choice.tile,
+                ce.choice.variant, ce.best_ms, ce.heuristic_ms);
+  f << buf;
+}
+
+bool tuning_enabled_env() {
+  static const bool on = [] {
+    const char *e = std::getenv("MPS2_TUNE");
+    return !(e && std::string(e) == "0");
+  }();
+  return on;
+}
+
+// Log bin of a size, four bins per octave: neighbouring bonds of a sweep,
+// whose sector dimensions differ by a few percent, share a key.
+int64_t lbin(int64_t x) { return x <= 0 ? -1 : (int64_t)std::lround(4.0 * std::log2((double)x)); }
+
+uint64_t shape_key(const ContractPlan &hp, bool fp64) {
+  std::vector<std::array<int64_t, 3>> s;
+  s.reserve(hp.blks.size());
+  for (const BlkH &b : hp.blks) {
+    int64_t kc = 0;
+    for (int p = b.pair_begin; p < b.pair_end; ++p) kc += hp.pairs[p].K;
+    // blocks too small to matter for the launch time stay out of the key
+    if (2.0 * b.M * (double)b.N * (double)kc < 1e-3 * hp.flops) continue;
+    s.push_back({lbin(b.M), lbin(b.N), lbin(kc)});
+  }
+  std::sort(s.begin(), s.end());
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    for (int i = 0; i < 8; ++i) {
+      h ^= (v >> (8 * i)) & 0xff;
+      h *= 1099511628211ull;
+    }
+  };
+  mix(fp64 ? 1 : 2);
+  mix((uint64_t)hp.la);
+  mix((uint64_t)hp.lb);
+  mix(s.size());
+  for (auto &x : s)
+    for (int64_t v : x) mix((uint64_t)v);
+  return h;
+}
+
+}  // namespace
+
+PlanTune::~PlanTune() {
+  for (auto &e : ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+}
+
+std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64) {
+  std::vector<GemmChoice> c;
+  const GemmChoice h = fp64 ? GemmChoice{64, 0} : GemmChoice{dp.tile_m, 0};
+  c.push_back(h);
+  // FP64 keeps its 64x64 tiles: 128x128 tiles allow one workgroup per compute
+  // unit and measured 19% slower on the D=4096 matvec and slower in every
+  // probe of a sweep, so timing them only cost time.
+  if (!fp64) {
+    if (h.tile != 128) c.push_back(GemmChoice{128, 0});
+    if (dp.hp->la == kNormal && dp.hp->lb == kNormal) c.push_back(GemmChoice{128, 4});
+    if (h.tile != 64) c.push_back(GemmChoice{64, 0});
+  }
+  return c;
+}
+
+GemmChoice tune_select(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe) {
+  PlanTune &t = dp.tune[fp64 ? 0 : 1];
+  probe->active = false;
+  if (t.cand.empty()) t.cand = tune_candidates(dp, fp64);
+  if (!tuning_enabled_env() || !c->tune_mode) return t.cand[0];
+  if (t.decided >= 0) return t.cand[t.decided];
+  if (dp.hp->flops < kTuneMinFlops || t.cand.size() < 2) return t.cand[0];
+  if (!t.keyed) {
+    t.key = shape_key(*dp.hp, fp64);
+    t.keyed = true;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    load_cache_locked();
+    auto it = g_tune.find({t.key, fp64 ? 1 : 0});
+    if (it != g_tune.end())
+      for (size_t i = 0; i < t.cand.size(); ++i)
+        if (t.cand[i].tile == it->second.choice.tile && t.cand[i].variant == it->second.choice.variant) {
+          t.decided = (int)i;
+          c->count("tune_cache_hits");
+          return t.cand[i];
+        }
+  }
+  const int total = (int)t.cand.size() * kTuneRounds;
+  if (t.next < total) {
+    const int k = t.next++ % (int)t.cand.size();
+    probe->active = true;
+    probe->cand = k;
+    c->count("tune_probes");
+    return t.cand[k];
+  }
+  // every probe issued: decide once the last one has completed
+  if (hipEventQuery(t.ev.back().second) != hipSuccess) {
+    (void)hipGetLastError();
+    return t.cand[0];
+  }
+  std::vector<double> best(t.cand.size(), 1e300);
+  for (size_t i = 0; i < t.ev.size(); ++i) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.ev[i].first, t.ev[i].second) != hipSuccess) {
+      (void)hipGetLastError();
+      continue;
+    }
+    best[t.ev_cand[i]] = std::min(best[t.ev_cand[i]], (double)ms);
+  }
+  int w = 0;
+  for (size_t i = 1; i < best.size(); ++i)
+    if (best[i] < best[w]) w = (int)i;
+  t.decided = w;
+  t.best_ms = best[w];
+  t.heuristic_ms = best[0];
+  c->count("tune_decisions");
+  if (w != 0) c->count("tune_changed");
+  CacheEntry ce;
+  ce.choice = t.cand[w];
+  ce.best_ms = best[w];
+  ce.heuristic_ms = best[0];
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune[{t.key, fp64 ? 1 : 0}] = ce;
+    append_cache(t.key, fp64 ? 1 : 0, ce);
+  }
+  return t.cand[w];
+}
+
+void tune_probe_begin(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe) {
+  if (!probe->active) return;
+  PlanTune &t = dp.tune[fp64 ? 0 : 1];
+  hipEvent_t a, b;
+  MPS2_HIP(hipEventCreate(&a));
+  MPS2_HIP(hipEventCreate(&b));
+  t.ev.emplace_back(a, b);
+  t.ev_cand.push_back(probe->cand);
+  MPS2_HIP(hipEventRecord(a, c->stream));
+}
+
+void tune_probe_end(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe) {
+  if (!probe->active) return;
+  MPS2_HIP(hipEventRecord(dp.tune[fp64 ? 0 : 1].ev.back().second, c->stream));
+}
+
+}  // namespace mps2

@@ -34,6 +34,7 @@ SIGNATURES = {
     "mps2_ctx_sync": (C.c_int, [_vp]),
     "mps2_ctx_stats": (C.c_int, [_vp, _dp, _i64p, _i64p, _i64p]),
     "mps2_ctx_reset_stats": (C.c_int, [_vp]),
+    "mps2_ctx_set_tuning": (C.c_int, [_vp, C.c_int]),
     "mps2_ten_import": (C.c_int, [_vp, _i32p, _dp, C.POINTER(_vp)]),
     "mps2_ten_export_sizes": (C.c_int, [_vp, _i64p, _i64p]),
     "mps2_ten_export": (C.c_int, [_vp, _i32p, _dp]),
@@ -274,7 +275,8 @@ def contract_ex(a: DTen, b: DTen, axes, precision=PREC_FP64) -> DTen:
 SIGNATURES["mps2_ctx_counter"] = (C.c_int, [_vp, C.c_char_p, _dp])
 COUNTERS = ("svd_calls", "svd_ms", "svd_eigh_ms", "svd_sectors", "svd_max_sector_sum", "syncs", "lanczos_calls",
             "matvec_fp64", "matvec_fp32", "hip_malloc", "alloc_trim", "svd_refine_rounds", "svd_refine_sectors",
-            "svd_refine_ms", "svd_refine_vec_ms", "svd_refine_eigh_ms")
+            "svd_refine_ms", "svd_refine_vec_ms", "svd_refine_eigh_ms", "tune_probes", "tune_decisions",
+            "tune_changed", "tune_cache_hits")
 
 
 def ctx_counters(ctx_handle, names=COUNTERS):

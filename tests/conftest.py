@@ -1,11 +1,15 @@
 import os
 import sys
+import tempfile
 
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
+# GEMM autotuning decisions of this test process go to a file of its own
+# (a fresh, empty cache), not to the user's cache.
+os.environ.setdefault("MPS2_TUNE_CACHE", os.path.join(tempfile.gettempdir(), f"mps2_tune_test_{os.getpid()}.txt"))
 
 
 def pytest_configure(config):

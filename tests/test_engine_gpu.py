@@ -479,3 +479,37 @@ def test_fp32_large_tile_launch_vs_numpy(ctx):
     for c in ref.blocks:
         assert np.array_equal(got.blocks[c], again.blocks[c]), c
         assert (np.abs(got.blocks[c] - ref.blocks[c]) <= 1e-5 * bound.blocks[c] + 1e-30).all(), c
+
+
+def test_autotuned_launches_are_bit_identical():
+    """The GEMM autotuner times its candidate kernels on the first launches of
+    a large plan (>= 1e9 flops) and keeps the fastest; every candidate has
+    the same per-element accumulation order, so all launches, before, during
+    and after the decision, and with tuning off, give bit-identical results,
+    and the decision is counted."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(23)
+    qs = list(range(8))
+    m = Leg(IN, [((q,), 500 + (q * 7) % 13) for q in qs])
+    n = Leg(OUT, [((q,), 512 - (q * 5) % 11) for q in qs])
+    k1 = Leg(OUT, [((q,), 60 + 5 * (q % 3)) for q in range(0, 8, 2)])
+    k2 = Leg(OUT, [((q,), 20 + 3 * q) for q in range(3)])
+    A = BTensor.random([m, k1, k2], (0,), rng)
+    B = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
+    c = E.Context(0)
+    try:
+        dA, dB = c.put(A), c.put(B)
+        for prec in (E.PREC_MIXED,):
+            E.check(E.lib().mps2_ctx_set_tuning(c.h, 0))
+            ref = E.contract_ex(dA, dB, [[1, 2], [0, 1]], prec).get()
+            E.check(E.lib().mps2_ctx_set_tuning(c.h, 1))
+            for it in range(12):
+                got = E.contract_ex(dA, dB, [[1, 2], [0, 1]], prec).get()
+                for cc in ref.blocks:
+                    assert np.array_equal(got.blocks[cc], ref.blocks[cc]), (prec, it, cc)
+        cnt = E.ctx_counters(c.h)
+        assert cnt["tune_decisions"] >= 1 and cnt["tune_probes"] >= 4, cnt
+        E.check(E.lib().mps2_ctx_set_tuning(c.h, 0))
+        assert E.lib().mps2_ctx_set_tuning(c.h, 7) != 0
+    finally:
+        c.close()
