@@ -85,6 +85,12 @@ struct SweepParams {
   LanczosParams lancz_params;
   std::string mps_path;
   std::string temp_path;
+  // Extension: write the whole MPS to mps_path at the end of every sweep
+  // (the reference keeps the MPS on disk by streaming every tensor around
+  // each update, two_site_update_finite_vmps_impl.h:343-494, so a run that
+  // stops can be continued from the files).  0 writes only when the call
+  // returns.
+  size_t checkpoint_every = 1;
 };
 
 // Per-update record (extension: the reference only prints it).
@@ -331,6 +337,12 @@ double TwoSiteFiniteVMPS(FiniteMPS<TenElemT, QNT> &mps, const MPO<GQTensor<TenEl
     e0 = eng.Sweep(sweep);
     sweep_timer.PrintElapsed();
     std::cout << "\n";
+    // Sweep-level checkpoint: a right-canonical MPS centred at site 0, the
+    // state TwoSiteFiniteVMPS starts from when it reads mps_path.
+    if (sweep_params.checkpoint_every && sweep % sweep_params.checkpoint_every == 0 && sweep < sweep_params.sweeps) {
+      const FiniteMPS<TenElemT, QNT> &cmps = mps;
+      cmps.Dump(sweep_params.mps_path);
+    }
   }
   (void)from_disk;
   mps.Dump(sweep_params.mps_path, true);

@@ -339,7 +339,82 @@ __global__ void calib_copy_kernel(const double2 *__restrict__ src, double2 *__re
     dst[i] = src[i];
 }
 
+// y = y + (sign * *a) * x with the coefficient read from device memory.
+__global__ void axpy_dev_kernel(int64_t n, double sign, const double *a, const double *x, double *y) {
+  const double c = sign * *a;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    y[i] = c * x[i] + y[i];
+}
+__global__ void scale_dev_kernel(int64_t n, const double *a, double *x) {
+  const double c = *a;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= c;
+}
+
+// Partial sums of <vecs[v], w> for v in [v0, v0 + kMultiV): block b covers a
+// strided share of the elements; partial[v * nblk + b].
+constexpr int kMultiV = 8;
+__global__ __launch_bounds__(kRedThreads) void multidot_partial_kernel(int64_t n, int nv, int v0,
+                                                                       const double *const *vecs, const double *w,
+                                                                       double *partial) {
+  __shared__ double sh[8];
+  double s[kMultiV];
+#pragma unroll
+  for (int j = 0; j < kMultiV; ++j) s[j] = 0;
+  const int nloc = min(kMultiV, nv - v0);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = w[i];
+#pragma unroll
+    for (int j = 0; j < kMultiV; ++j)
+      if (j < nloc) s[j] += vecs[v0 + j][i] * x;
+  }
+  for (int j = 0; j < nloc; ++j) {
+    const double r = block_sum(s[j], sh);
+    if (threadIdx.x == 0) partial[(int64_t)(v0 + j) * gridDim.x + blockIdx.x] = r;
+    __syncthreads();
+  }
+}
+// out[v] = sum of the partials of vector v (one block per vector, fixed order).
+__global__ __launch_bounds__(kRedThreads) void multidot_final_kernel(const double *partial, int np, double *out) {
+  __shared__ double sh[8];
+  double s = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += partial[(int64_t)blockIdx.x * np + i];
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = r;
+}
+// w = w - sum_v coef[v] * vecs[v]  (coefficients in device memory)
+__global__ void multiaxpy_kernel(int64_t n, int nv, const double *const *vecs, const double *coef, double *w) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0;
+    for (int v = 0; v < nv; ++v) s += coef[v] * vecs[v][i];
+    w[i] -= s;
+  }
+}
+
 }  // namespace
+
+void k_axpy_dev(Ctx *c, int64_t n, double sign, const double *d_a, const double *x, double *y) {
+  if (n > 0) axpy_dev_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, sign, d_a, x, y);
+}
+void k_scale_dev(Ctx *c, int64_t n, const double *d_a, double *x) {
+  if (n > 0) scale_dev_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, d_a, x);
+}
+void k_dot_to(Ctx *c, int64_t n, const double *x, const double *y, double *d_out) {
+  double *part = static_cast<double *>(c->red_partial->p);
+  dot_partial_kernel<<<kRedBlocks, kRedThreads, 0, c->stream>>>(n, x, y, part);
+  reduce_final_kernel<<<1, kRedThreads, 0, c->stream>>>(part, kRedBlocks, d_out);
+}
+void k_multidot(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *w, double *d_out,
+                double *d_partial) {
+  if (nv <= 0) return;
+  constexpr int nblk = 256;
+  for (int v0 = 0; v0 < nv; v0 += kMultiV)
+    multidot_partial_kernel<<<nblk, kRedThreads, 0, c->stream>>>(n, nv, v0, d_vecs, w, d_partial);
+  multidot_final_kernel<<<nv, kRedThreads, 0, c->stream>>>(d_partial, nblk, d_out);
+}
+void k_multiaxpy(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *d_coef, double *w) {
+  if (n > 0 && nv > 0) multiaxpy_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, nv, d_vecs, d_coef, w);
+}
 
 void k_calib_copy(Ctx *c, const void *src, void *dst, int64_t bytes) {
   const int64_t n = bytes / 16;
@@ -367,10 +442,7 @@ void k_cast_f2d(Ctx *c, int64_t n, const float *x, double *y) {
   if (n > 0) f2d_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(n, x, y);
 }
 void k_dot(Ctx *c, int64_t n, const double *x, const double *y, int slot) {
-  double *part = static_cast<double *>(c->red_partial->p);
-  dot_partial_kernel<<<kRedBlocks, kRedThreads, 0, c->stream>>>(n, x, y, part);
-  reduce_final_kernel<<<1, kRedThreads, 0, c->stream>>>(part, kRedBlocks,
-                                                        static_cast<double *>(c->dev_scalars->p) + slot);
+  k_dot_to(c, n, x, y, static_cast<double *>(c->dev_scalars->p) + slot);
 }
 void k_block_maxabs(Ctx *c, const double *x, const int64_t *d_off, const int64_t *d_size, int nblk, double *d_out) {
   if (nblk <= 0) return;

@@ -26,77 +26,140 @@ int64_t eff_dim_of(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R,
   throw Error(kErrArg, "where must be lend, cent or rend");
 }
 
-TenP final_combination(Ctx *c, const MetaP &m, const std::vector<TenP> &bases, int n, const std::vector<double> &z) {
+// ---- device-side scalar steps of the recurrences (one thread each)
+
+__global__ void lz_set_ptr_kernel(const double **table, int i, const double *p) { table[i] = p; }
+
+// From dot = <g, g>: b = |g|, c2 = sqrt(b * b) (the coefficient the
+// reference forms from N = b^2 one step later), inv = 1 / b (0 when b = 0).
+__global__ void lz_norm_kernel(const double *dot, double *b, double *c2, double *inv) {
+  const double ng = sqrt(*dot);
+  *b = ng;
+  *c2 = sqrt(ng * ng);
+  *inv = ng > 0.0 ? 1.0 / ng : 0.0;
+}
+__global__ void lz_rsqrt_kernel(const double *dot, double *norm, double *inv) {
+  const double n = sqrt(*dot);
+  *norm = n;
+  *inv = n > 0.0 ? 1.0 / n : 0.0;
+}
+// Small coefficient vectors travel as kernel arguments (no host staging).
+struct CoefArg {
+  int n;
+  double v[64];
+};
+__global__ void lz_set_coef_kernel(CoefArg a, double *dst) {
+  const int i = threadIdx.x;
+  if (i < a.n) dst[i] = a.v[i];
+}
+__global__ void lz_pack_kernel(const double *src, int n, double *dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Per-solve device state: the recurrence scalars and the basis pointer table.
+struct LzDev {
+  Ctx *c;
+  BufP buf;        // scalars
+  BufP table;      // const double * per basis vector
+  BufP work;       // tridiagonal work space
+  BufP partial;    // multi-dot partial sums
+  int cap;
+  double *a, *b, *c2, *scal, *out;
+  LzDev(Ctx *ctx, int max_vec) : c(ctx), cap(max_vec) {
+    const int m = max_vec + 2;
+    buf = c->alloc((size_t)(4 * m + 8 + (m + 1)) * sizeof(double));
+    double *p = static_cast<double *>(buf->p);
+    a = p, b = p + m, c2 = p + 2 * m, scal = p + 3 * m, out = p + 3 * m + 8;
+    table = c->alloc((size_t)m * sizeof(void *));
+    work = c->alloc((size_t)(5 * m + 8) * sizeof(double));
+    partial = c->alloc((size_t)m * 256 * sizeof(double));
+  }
+  const double **ptrs() const { return static_cast<const double **>(table->p); }
+  void set_ptr(int i, const double *p) {
+    if (i >= cap + 2) throw Error(kErrState, "Lanczos: basis table overflow");
+    lz_set_ptr_kernel<<<1, 1, 0, c->stream>>>(const_cast<const double **>(ptrs()), i, p);
+  }
+  // n doubles from device to the context's pinned host scalars, then sync
+  const double *read(const double *src, int n) {
+    MPS2_HIP(hipMemcpyAsync(c->host_scalars, src, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->sync();
+    return c->host_scalars;
+  }
+};
+
+TenP combine_dev(Ctx *c, const MetaP &m, const LzDev &dv, int n, const double *d_coef) {
   auto gs = ten_alloc(c, m);
-  std::vector<const double *> ptrs(n);
-  for (int i = 0; i < n; ++i) ptrs[i] = bases[i]->d();
-  auto dp = c->alloc(n * sizeof(void *));
-  auto dc = c->alloc(n * sizeof(double));
-  MPS2_HIP(hipMemcpyAsync(dp->p, ptrs.data(), n * sizeof(void *), hipMemcpyHostToDevice, c->stream));
-  MPS2_HIP(hipMemcpyAsync(dc->p, z.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  k_lincomb(c, m->total, n, static_cast<const double *const *>(dp->p), static_cast<const double *>(dc->p), gs->d());
-  c->sync();
+  k_lincomb(c, m->total, n, dv.ptrs(), d_coef, gs->d());
   return gs;
 }
 
 }  // namespace
 
+// Plain FP64 form (lanczos_solver_impl.h:79-225) with the recurrence scalars
+// kept on the device: alpha, beta and the coefficients are reduced into and
+// read from device memory, the tridiagonal ground energy is solved on the
+// device from those arrays, and the host reads two numbers per iteration
+// (energy, |gamma|) for the stopping rule.
 LanczosOut lanczos_plain(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
                          const std::string &where, double error, int64_t max_iter) {
   Ctx *c = init->ctx;
   MPS2_REQUIRE(max_iter >= 2, "Lanczos: max_iterations must be >= 2");
   int64_t eff_dim = eff_dim_of(L, W1, W2, R, where);
   MetaP full = full_meta(init->meta->qw, init->meta->legs, init->meta->div());
+  const int64_t n = full->total;
+  LzDev dv(c, (int)max_iter);
   std::vector<TenP> bases;
-  std::vector<double> a(max_iter, 0.0), b(max_iter, 0.0), N(max_iter, 0.0);
   bases.push_back(embed(init, full));
-  normalize(*bases[0]);
+  k_dot_to(c, n, bases[0]->d(), bases[0]->d(), dv.scal);
+  lz_rsqrt_kernel<<<1, 1, 0, c->stream>>>(dv.scal, dv.scal + 1, dv.scal + 2);
+  k_scale_dev(c, n, dv.scal + 2, bases[0]->d());
+  dv.set_ptr(0, bases[0]->d());
   TenP last = ten_alloc(c, full);
   heff_mul(L, W1, W2, R, bases[0], where, last, kF64);
   LanczosOut res;
   res.matvecs_hi = 1;
-  a[0] = dot(*last, *bases[0]);
-  double energy0 = a[0];
+  k_dot_to(c, n, last->d(), bases[0]->d(), dv.a);
+  double energy0 = dv.read(dv.a, 1)[0];
   int64_t m = 0;
   while (true) {
     m += 1;
     TenP gamma = last;
-    if (m == 1) {
-      k_axpby(c, gamma->n(), -a[m - 1], bases[m - 1]->d(), 1.0, gamma->d());
-    } else {
-      k_axpby(c, gamma->n(), -a[m - 1], bases[m - 1]->d(), 1.0, gamma->d());
-      k_axpby(c, gamma->n(), -std::sqrt(N[m - 1]), bases[m - 2]->d(), 1.0, gamma->d());
-    }
-    double ng = normalize(*gamma);
+    k_axpy_dev(c, n, -1.0, dv.a + (m - 1), bases[m - 1]->d(), gamma->d());
+    if (m > 1) k_axpy_dev(c, n, -1.0, dv.c2 + (m - 1), bases[m - 2]->d(), gamma->d());
+    k_dot_to(c, n, gamma->d(), gamma->d(), dv.scal);
+    lz_norm_kernel<<<1, 1, 0, c->stream>>>(dv.scal, dv.b + (m - 1), dv.c2 + m, dv.scal + 2);
+    k_scale_dev(c, n, dv.scal + 2, gamma->d());
+    bases.push_back(gamma);
+    dv.set_ptr((int)m, gamma->d());
+    last = ten_alloc(c, full);
+    heff_mul(L, W1, W2, R, bases[m], where, last, kF64);
+    res.matvecs_hi++;
+    k_dot_to(c, n, last->d(), bases[m]->d(), dv.a + m);
+    tridiag_lowest_async(c, (int)m + 1, dv.a, dv.b, 0, dv.out, static_cast<double *>(dv.work->p));
+    lz_pack_kernel<<<1, 64, 0, c->stream>>>(dv.b + (m - 1), 1, dv.scal + 3);
+    lz_pack_kernel<<<1, 64, 0, c->stream>>>(dv.out, 1, dv.scal + 4);
+    const double *h = dv.read(dv.scal + 3, 2);
+    const double ng = h[0], e_new = h[1];
     if (ng == 0.0) {
+      // gamma vanished: the Krylov space is invariant (the reference returns
+      // the ground state of the m x m tridiagonal, lanczos_solver_impl.h:161-170)
       res.iters = m;
       res.energy = energy0;
       if (m == 1) {
         res.gs = ten_copy(*bases[0]);
       } else {
-        double ev;
-        std::vector<double> z(m);
-        tridiag_lowest_dev(c, (int)m, a.data(), b.data(), &ev, z.data());
-        res.gs = final_combination(c, full, bases, (int)m, z);
+        tridiag_lowest_async(c, (int)m, dv.a, dv.b, 1, dv.out, static_cast<double *>(dv.work->p));
+        res.gs = combine_dev(c, full, dv, (int)m, dv.out + 1);
       }
       return res;
     }
-    N[m] = ng * ng;
-    b[m - 1] = ng;
-    bases.push_back(gamma);
-    last = ten_alloc(c, full);
-    heff_mul(L, W1, W2, R, bases[m], where, last, kF64);
-    res.matvecs_hi++;
-    a[m] = dot(*last, *bases[m]);
-    double e_new;
-    tridiag_lowest_dev(c, (int)m + 1, a.data(), b.data(), &e_new, nullptr);
     if ((energy0 - e_new) < error || m == eff_dim || m == max_iter - 1) {
-      std::vector<double> z(m + 1);
-      double ev;
-      tridiag_lowest_dev(c, (int)m + 1, a.data(), b.data(), &ev, z.data());
+      tridiag_lowest_async(c, (int)m + 1, dv.a, dv.b, 1, dv.out, static_cast<double *>(dv.work->p));
       res.iters = m;
       res.energy = e_new;
-      res.gs = final_combination(c, full, bases, (int)m + 1, z);
+      res.gs = combine_dev(c, full, dv, (int)m + 1, dv.out + 1);
+      c->sync();
       return res;
     }
     energy0 = e_new;
@@ -182,8 +245,12 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
   TenP L32 = cast_to(L, kF32), W132 = cast_to(W1, kF32), W232 = cast_to(W2, kF32), R32 = cast_to(R, kF32);
   const int64_t n = full->total;
   const int kInnerMax = (int)std::min<int64_t>(std::max<int64_t>(max_iter, 2), 40);
+  LzDev dv(c, kInnerMax + 2);
+  double *d_col = dv.a, *d_col2 = dv.b;   // re-orthogonalisation coefficients (device)
   TenP v = embed(init, full);
-  normalize(*v);
+  k_dot_to(c, n, v->d(), v->d(), dv.scal);
+  lz_rsqrt_kernel<<<1, 1, 0, c->stream>>>(dv.scal, dv.scal + 1, dv.scal + 2);
+  k_scale_dev(c, n, dv.scal + 2, v->d());
   TenP h = ten_alloc(c, full);
   TenP x32 = ten_alloc(c, full, kF32), y32 = ten_alloc(c, full, kF32);
   LanczosOut res;
@@ -192,10 +259,12 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
   for (int restart = 0; restart < 64; ++restart) {
     heff_mul(L, W1, W2, R, v, where, h, kF64);
     res.matvecs_hi++;
-    double E = dot(*h, *v);
-    // r = h - E v  (in h)
-    k_axpby(c, n, -E, v->d(), 1.0, h->d());
-    double beta = norm(*h);
+    // E = <h, v>, r = h - E v (in h), beta = |r|: one read for both
+    k_dot_to(c, n, h->d(), v->d(), dv.scal + 3);
+    k_axpy_dev(c, n, -1.0, dv.scal + 3, v->d(), h->d());
+    k_dot_to(c, n, h->d(), h->d(), dv.scal + 4);
+    const double *hs = dv.read(dv.scal + 3, 2);
+    const double E = hs[0], beta = std::sqrt(hs[1]);
     bool done = (restart > 0 && (E_prev - E) < error) || beta == 0.0 || total_inner >= max_iter - 1 ||
                 (eff_dim <= 1);
     if (done) {
@@ -206,10 +275,12 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
     }
     E_prev = E;
     std::vector<TenP> Q{v};
+    dv.set_ptr(0, v->d());
     auto q1 = ten_alloc(c, full);
     MPS2_HIP(hipMemcpyAsync(q1->d(), h->d(), n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     k_scale(c, n, 1.0 / beta, q1->d());
     Q.push_back(q1);
+    dv.set_ptr(1, q1->d());
     // projected matrix, grown column by column (kept as a full symmetric matrix)
     std::vector<std::vector<double>> T(2, std::vector<double>(2, 0.0));
     T[0][0] = E;
@@ -225,27 +296,29 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
       auto w = ten_alloc(c, full);
       k_cast_f2d(c, n, y32->f(), w->d());
       ++total_inner;
-      int k = (int)Q.size();
-      std::vector<double> col(k, 0.0);
-      for (int i = 1; i < k; ++i) col[i] = dot(*Q[i], *w);
+      const int k = (int)Q.size();
+      // Full re-orthogonalisation against Q[0..k-1], classical Gram-Schmidt
+      // twice in FP64 (two batched passes); the first pass's coefficients
+      // <Q_i, w> are the projected matrix's new column.
+      k_multidot(c, n, k, dv.ptrs(), w->d(), d_col, static_cast<double *>(dv.partial->p));
+      k_multiaxpy(c, n, k, dv.ptrs(), d_col, w->d());
+      k_multidot(c, n, k, dv.ptrs(), w->d(), d_col2, static_cast<double *>(dv.partial->p));
+      k_multiaxpy(c, n, k, dv.ptrs(), d_col2, w->d());
+      k_dot_to(c, n, w->d(), w->d(), d_col + k);
+      const double *hc = dv.read(d_col, k + 1);
+      std::vector<double> col(hc, hc + k);
+      const double wn = std::sqrt(hc[k]);
       col[0] = (j == 1) ? beta : 0.0;
       for (int i = 1; i <= j; ++i) {
         T[i][j] = col[i];
         T[j][i] = col[i];
       }
-      // full reorthogonalisation (two passes, FP64)
-      for (int pass = 0; pass < 2; ++pass)
-        for (int i = 0; i < k; ++i) {
-          double cf = pass == 0 && i > 0 ? col[i] : dot(*Q[i], *w);
-          k_axpby(c, n, -cf, Q[i]->d(), 1.0, w->d());
-        }
       int kk = j + 1;
       std::vector<double> A(kk * kk);
       for (int a = 0; a < kk; ++a)
         for (int b = 0; b < kk; ++b) A[a * kk + b] = T[a][b];
       double theta = small_sym_lowest(kk, A, y);
       m_in = j;
-      double wn = norm(*w);
       // Stagnation test of the plain form: the Ritz value moved by less than `error`.
       bool inner_done = std::fabs(theta_prev - theta) < error || wn <= 1e-14 * std::fabs(theta) ||
                         (int64_t)Q.size() >= eff_dim || j == kInnerMax || total_inner >= max_iter - 1;
@@ -253,14 +326,20 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
       if (inner_done) break;
       k_scale(c, n, 1.0 / wn, w->d());
       Q.push_back(w);
+      dv.set_ptr(k, w->d());
       for (auto &row : T) row.push_back(0.0);
       T.push_back(std::vector<double>(T[0].size(), 0.0));
     }
-    // v <- sum_i y_i Q_i  (y has m_in + 1 entries: Q[0..m_in])
-    int nv = m_in + 1;
-    std::vector<double> z(y.begin(), y.begin() + nv);
-    v = final_combination(c, full, Q, nv, z);
-    normalize(*v);
+    // v <- sum_i y_i Q_i  (y has m_in + 1 entries: Q[0..m_in]), normalised
+    const int nv = m_in + 1;
+    CoefArg ca{};
+    ca.n = nv;
+    for (int i = 0; i < nv; ++i) ca.v[i] = y[i];
+    lz_set_coef_kernel<<<1, 64, 0, c->stream>>>(ca, dv.c2);
+    v = combine_dev(c, full, dv, nv, dv.c2);
+    k_dot_to(c, n, v->d(), v->d(), dv.scal);
+    lz_rsqrt_kernel<<<1, 1, 0, c->stream>>>(dv.scal, dv.scal + 1, dv.scal + 2);
+    k_scale_dev(c, n, dv.scal + 2, v->d());
     // The projected solve lowers the energy by less than `error`: the
     // reference's stopping rule (lanczos_solver_impl.h:208-221) holds for this
     // Krylov space, so return its Ritz pair.  Row and column 0 of the

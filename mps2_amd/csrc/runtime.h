@@ -150,6 +150,16 @@ void k_dot(Ctx *c, int64_t n, const double *x, const double *y, int slot);
 void k_block_maxabs(Ctx *c, const double *x, const int64_t *d_off, const int64_t *d_size, int nblk, double *d_out);
 // out = sum_i coef[i] * vec[i] (device pointer table, device coefficients)
 void k_lincomb(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *d_coef, double *out);
+// Device-scalar forms (no host round trip): y += sign * (*d_a) * x; x *= *d_a;
+// *d_out = <x, y> (deterministic two-pass reduction).
+void k_axpy_dev(Ctx *c, int64_t n, double sign, const double *d_a, const double *x, double *y);
+void k_scale_dev(Ctx *c, int64_t n, const double *d_a, double *x);
+void k_dot_to(Ctx *c, int64_t n, const double *x, const double *y, double *d_out);
+// d_out[v] = <vecs[v], w> for v < nv (d_partial: nv * 256 doubles of work
+// space); w -= sum_v coef[v] vecs[v].  Fixed summation order.
+void k_multidot(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *w, double *d_out,
+                double *d_partial);
+void k_multiaxpy(Ctx *c, int64_t n, int nv, const double *const *d_vecs, const double *d_coef, double *w);
 // Block-wise permutation / copy (descriptors on device).
 struct PermDesc {
   int64_t src_off, dst_off, size;
@@ -291,6 +301,10 @@ int tridiag_lowest_dev(Ctx *c, int n, const double *a, const double *b, double *
 // Allocation-free form on device data: d_out[0] = eigenvalue, d_out[1..n] =
 // eigenvector (want_vec); work space in a static per-device array.
 void tridiag_lowest_noalloc(Ctx *c, int n, const double *d_a, const double *d_b, int want_vec, double *d_out);
+// The same on device data with the caller's work space (5 n + 8 doubles),
+// stream-ordered, no sync.
+void tridiag_lowest_async(Ctx *c, int n, const double *d_a, const double *d_b, int want_vec, double *d_out,
+                          double *d_work);
 // Takes the device address of that array; called once per device, with the
 // device current (context construction).
 void tridiag_scratch_init(int device);

@@ -194,6 +194,12 @@ int tridiag_lowest_dev(Ctx *c, int n, const double *a, const double *b, double *
   return 0;
 }
 
+void tridiag_lowest_async(Ctx *c, int n, const double *d_a, const double *d_b, int want_vec, double *d_out,
+                          double *d_work) {
+  if (n < 1) throw Error(kErrArg, "tridiag_lowest_async: n must be >= 1");
+  tridiag_lowest_kernel<<<1, 64, 0, c->stream>>>(n, d_a, n > 1 ? d_b : d_a, want_vec ? 1 : 0, d_out, d_work);
+}
+
 void tridiag_lowest_noalloc(Ctx *c, int n, const double *d_a, const double *d_b, int want_vec, double *d_out) {
   if (n < 1 || n > kNoAllocMaxN)
     throw Error(kErrArg, "tridiag_lowest_noalloc: n = " + std::to_string(n) + " outside 1.." +

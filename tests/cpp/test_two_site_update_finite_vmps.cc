@@ -6,7 +6,9 @@
 //     MeasureOneSiteOp / MeasureTwoSiteOp;
 //   * 1D Heisenberg chain (6 sites, D = 8): E0 = -2.493577133888 to 1e-12,
 //     then the "continue simulation" case: the container is cleared and a
-//     second TwoSiteFiniteVMPS call resumes from the MPS files.
+//     second TwoSiteFiniteVMPS call resumes from the MPS files;
+//   * sweep-level checkpoint (extension): a 10-site run stopped inside sweep
+//     3 resumes from the files of sweep 2 and reproduces sweeps 3 and 4.
 // Needs a GPU.  Exit status 0 = all cases pass.
 #include <cmath>
 #include <cstdio>
@@ -118,6 +120,66 @@ int main() {
                "MPS files and runtime directory exist after the run");
     dmps.clear();
     RunCase(dmps, mpo, sp, -2.493577133888, 1.0E-12, "Heisenberg E0, continued from the MPS files");
+    RemoveFolder(sp.mps_path);
+    RemoveFolder(sp.temp_path);
+  }
+
+  // ---- sweep-level checkpoint: a run stopped inside sweep 3 continues from
+  // the MPS files written at the end of sweep 2 and reproduces sweeps 3 and 4
+  // of the uninterrupted run (an own extension; the reference streams the MPS
+  // to disk around every update)
+  {
+    const size_t M = 10;
+    DSiteVec sites10(M, pb_out);
+    MPOGenerator<GQTEN_Double, U1QN> gen(sites10, qn0);
+    for (size_t i = 0; i + 1 < M; ++i) {
+      gen.AddTerm(1, {dsz, dsz}, {i, i + 1});
+      gen.AddTerm(0.5, {dsp, dsm}, {i, i + 1});
+      gen.AddTerm(0.5, {dsm, dsp}, {i, i + 1});
+    }
+    auto mpo = gen.Gen();
+    std::vector<size_t> labs10;
+    for (size_t i = 0; i < M; ++i) labs10.push_back(i % 2);
+    const size_t per_sweep = 2 * (M - 1);
+    SweepParams sp(4, 1, 6, 0.0, LanczosParams(1.0E-10), "ckpt_mps", "ckpt_tmp");
+    std::vector<double> ref, got;
+    DMPS a(sites10);
+    std::vector<size_t> labs10b;
+    for (size_t i = 0; i < M; ++i) labs10b.push_back((i + 1) % 2);
+    SetRandomSeed(99);
+    ExtendDirectRandomInitMps(a, {labs10, labs10b}, qn0, 3);
+    a.Dump(sp.mps_path, true);
+    TwoSiteFiniteVMPS(a, mpo, sp, [&](const UpdateRecord &r) { ref.push_back(r.energy); });
+    ExpectTrue(ref.size() == 4 * per_sweep, "uninterrupted run: four sweeps of updates");
+    RemoveFolder(sp.mps_path);
+    DMPS b(sites10);
+    SetRandomSeed(99);
+    ExtendDirectRandomInitMps(b, {labs10, labs10b}, qn0, 3);
+    b.Dump(sp.mps_path, true);
+    struct Stop {};
+    bool stopped = false;
+    try {
+      TwoSiteFiniteVMPS(b, mpo, sp, [&](const UpdateRecord &r) {
+        got.push_back(r.energy);
+        if (got.size() == 2 * per_sweep + 3) throw Stop{};  // inside sweep 3
+      });
+    } catch (const Stop &) {
+      stopped = true;
+    }
+    ExpectTrue(stopped, "the run stopped inside sweep 3");
+    DMPS c(sites10);
+    SweepParams sp2 = sp;
+    sp2.sweeps = 2;
+    std::vector<double> resumed;
+    TwoSiteFiniteVMPS(c, mpo, sp2, [&](const UpdateRecord &r) { resumed.push_back(r.energy); });
+    ExpectTrue(resumed.size() == 2 * per_sweep, "resumed run: two sweeps of updates");
+    double worst = 0;
+    for (size_t k = 0; k < resumed.size() && 2 * per_sweep + k < ref.size(); ++k)
+      worst = std::max(worst, std::fabs(resumed[k] - ref[2 * per_sweep + k]));
+    ExpectNear(worst, 0.0, 1e-12, "resumed sweeps 3 and 4 reproduce the uninterrupted run, every update");
+    for (size_t k = 0; k + 1 < got.size() && k < ref.size(); ++k)
+      worst = std::max(worst, std::fabs(got[k] - ref[k]));
+    ExpectNear(worst, 0.0, 1e-12, "the stopped run agrees with the uninterrupted one before the stop");
     RemoveFolder(sp.mps_path);
     RemoveFolder(sp.temp_path);
   }
