@@ -372,8 +372,11 @@ def main():
         "sweep_s": {k: round(res[k]["wall"], 3) for k in res},
         "sweep_phase_s": {k: res[k]["phase"] for k in res},
         "sweep_speedup_mixed_vs_fp64": round(res["fp64"]["wall"] / res["mixed"]["wall"], 3),
-        "sweep_counters": {k: {c: res[k]["counters"][c] for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls",
-                                                                  "hip_malloc")} for k in res},
+        "sweep_counters": {k: {c: round(res[k]["counters"][c], 1)
+                               for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls", "hip_malloc",
+                                         "svd_eigh_ms", "svd_refine_ms", "svd_refine_eigh_ms", "svd_refine_rounds",
+                                         "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
+                                         "svd_max_sector_sum")} for k in res},
         "stage_ms": {"fp64": mb["fp64"]["stage_ms"], "mixed": mb["mixed"]["stage_ms"]},
         "stage_ms_heuristic": {k: heur[k]["stage_ms"] for k in heur},
         "autotune": tune_summary(),

@@ -3,11 +3,12 @@
 //
 //  1. Householder tridiagonalisation, lower form (A = Q T Q^T,
 //     Q = H_0 H_1 ... H_{n-2}, H_c = I - tau_c v_c v_c^T, v_c[c+1] = 1), one
-//     column per step for every sector at once: one workgroup per sector
-//     finishes w_{c-1} = tau y - tau^2/2 (y^T v) v, forms row c of the updated
-//     matrix and builds the reflector v_c; then many workgroups apply the
-//     rank-2 update of column c-1 to the trailing square and form
-//     y = A_trail v_c in the same pass.
+//     column per step for every sector at once, with the trailing update
+//     deferred over panels of kPB reflectors (see the tridiagonalisation
+//     section): per column one workgroup per sector finishes w_{c-1} and
+//     builds the reflector v_c, and many workgroups form y = S v_c from the
+//     stored trailing square without writing it; every kPB columns a tiled
+//     pass applies the panel's rank-2kPB update.
 //  2. Tridiagonal divide and conquer (Cuppen), bottom-up from 1x1 blocks,
 //     every merge of one level in the same launches.  A merge of adjacent
 //     blocks diag(T1', T2') + rho u u^T (u = e_last + sign(beta) e_first,
@@ -29,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 #include "eigh.h"
@@ -38,8 +40,7 @@ namespace {
 
 constexpr int kNB = 128;  // compact-WY panel width of the back-transformation
 constexpr int kT = 256;   // threads of the per-sector kernels
-constexpr int kRowsPerWave = 1;
-constexpr int kSymvRows = 4 * kRowsPerWave;  // 4 wavefronts per workgroup
+constexpr int kSymvRows = 4;  // rows per product workgroup (one per wavefront)
 constexpr double kEps = 2.220446049250313e-16;
 
 // Order key of eigenvalue estimates in the merges: NaN sorts as +inf, so the
@@ -88,116 +89,256 @@ __device__ double block_max(double v, double *red) {
   return s;
 }
 
-// Column step c of one sector, run by one workgroup: finish w_{c-1} from
-// y = A' v_{c-1} (w = tau y - tau^2/2 (y.v) v), form row c of the matrix
-// updated by (v_{c-1}, w_{c-1}), take d[c], and build the reflector v_c of
-// row c (alpha = a[c+1], x = a[c+2..]) into VH column c, e[c], tau[c].
-__device__ void row_step(const ESec &S, int c, const double *__restrict__ A, double *VH, double *wbuf,
-                         const double *ybuf, double *abuf, double *dv, double *ev, double *tauv, double *red,
-                         double *sh) {
-  const int n = S.n, tid = threadIdx.x, nt = blockDim.x;
-  const double *arow = A + S.nn + (int64_t)c * n;
-  constexpr int P = 8;
-  if (n - c <= P * nt) {
-    // Rows r = c + tid + k nt (k < P) stay in this thread's registers: their
-    // loads are issued before the first reduction, and the reflector is
-    // written from the same registers (no round trip through memory).
-    double ar[P], yr[P], vr[P];
+// ------------------------------------------------------ tridiagonalisation
+// Blocked (deferred-update) Householder reduction, the scheme of LAPACK's
+// dsytrd/dlatrd: the trailing matrix S receives the rank-2 terms of kPB
+// reflectors at once, A_true = S - sum_{i pending} (v_i w_i^T + w_i v_i^T), so
+// the matrix-vector product of each column reads S without writing it (half
+// the HBM traffic of a rank-2 update per column) and the deferred update is
+// one tiled read-modify-write pass per kPB columns.
+//
+// Per column c, every sector of the batch in the same launches:
+//   row kernel (one workgroup per sector): from y = S v_{c-1} (previous
+//     product) and the panel projections p_i = w_i.v_{c-1}, s_i = v_i.v_{c-1}
+//     (i in the panel of c - 1, before it; reduced here together with
+//     y.v_{c-1}), finish w_{c-1} = tau (y - sum_i (v_i p_i + w_i s_i)) + alpha v_{c-1}
+//     with alpha = -tau^2/2 (y.v_{c-1} - 2 p.s); form column c of A_true from
+//     row c of S and the pending (v_i, w_i); build the reflector v_c (d[c],
+//     e[c], tau[c]);
+//   update kernel (first column c = k > 0 of a panel): S -= the previous
+//     panel's terms on rows and columns > c;
+//   product kernel: y = S v_c on rows > c (S only read).
+// S is kept as a full square (both triangles) on rows and columns > c.
+constexpr int kPB = 8;     // reflectors per deferred update
+constexpr int kTR = 512;   // threads of the row kernel
+constexpr int kUT = 64;    // update tile
+
+struct TrdArgs {
+  double *S;        // Gram matrices, n x n row-major per sector (overwritten)
+  double *VH;       // reflectors, column-major per sector (column c = v_c)
+  double *W0, *W1;  // w_i of even / odd panels, n x kPB column-major per sector (offset nv * kPB)
+  double *y;        // result of the last product kernel (n per sector)
+  double *a;        // column c for rows beyond the row kernel's registers (n per sector)
+  double *d, *e, *tau;
+};
+
+__device__ __forceinline__ double *wcol(const TrdArgs &A, const ESec &S, int i) {
+  return ((i / kPB) & 1 ? A.W1 : A.W0) + S.nv * kPB + (int64_t)(i % kPB) * S.n;
+}
+
+// Sums over the workgroup of v[0..16) (all threads receive them), fixed
+// order.  Within a wavefront a halving butterfly: at distance o = 32, 16, 8,
+// 4 each lane keeps half of its values (the upper half when lane & o) and
+// adds the partner's copy of that half, so after four steps a lane holds one
+// value, (lane >> 2) & 15, summed over 16 lanes; distances 2 and 1 complete
+// the wavefront sum.  17 shuffles of a double instead of 96.  red holds
+// (blockDim.x / 64 + 1) * 16 doubles.
+template <int O, int H>
+__device__ __forceinline__ void halve16(double (&v)[16], int lane) {
+  const bool up = lane & O;
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-      const int r = c + tid + k * nt;
-      ar[k] = r < n ? arow[r] : 0.0;
-    }
-    double ss = 0;
-    if (c > 0) {
-      const double tau = tauv[S.nv + c - 1];
-      const double *vp = VH + S.nn + (int64_t)(c - 1) * n;
-      const double *y = ybuf + S.nv;
-      double *w = wbuf + S.nv;
-      const double vc = vp[c], yc = y[c];
-      double part = 0;
+  for (int j = 0; j < H; ++j) {
+    const double send = up ? v[j] : v[j + H];
+    const double keep = up ? v[j + H] : v[j];
+    v[j] = keep + __shfl_xor(send, O);
+  }
+}
+
+__device__ void block_sum16(double (&v)[16], double *red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  halve16<32, 8>(v, lane);
+  halve16<16, 4>(v, lane);
+  halve16<8, 2>(v, lane);
+  halve16<4, 1>(v, lane);
+  double x = v[0];
+  x += __shfl_xor(x, 2);
+  x += __shfl_xor(x, 1);
+  __syncthreads();  // red may still be read by a previous sum
+  if ((lane & 3) == 0) red[w * 16 + (lane >> 2)] = x;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    double t = 0;
+    for (int i = 0; i < nw; ++i) t += red[i * 16 + threadIdx.x];
+    red[nw * 16 + threadIdx.x] = t;
+  }
+  __syncthreads();
 #pragma unroll
-      for (int k = 0; k < P; ++k) {
-        const int r = c + tid + k * nt;
-        yr[k] = r < n ? y[r] : 0.0;
-        vr[k] = r < n ? vp[r] : 0.0;
-        part += yr[k] * vr[k];
+  for (int j = 0; j < 16; ++j) v[j] = red[nw * 16 + j];
+}
+
+// Row kernel for column c (grid: one workgroup per sector).  Thread tid owns
+// rows r = c + tid + j * kTR.  With P > 0 (all rows fit P per thread) every
+// value the step reads is loaded into registers first, in one batch (rows
+// past the end read row n - 1; unused pending slots read reflector 0 with
+// zero weight; the W buffers are zero-filled), so the kernel waits on memory
+// once; with P = 0 the rows are looped over and re-read.  The entries at row
+// c that every thread needs are staged in LDS.
+template <int P>
+__global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ secs, int c, TrdArgs A) {
+  __shared__ double red[(kTR / 64 + 1) * 16];
+  __shared__ double sh[2];
+  __shared__ double un[2][kPB];  // v_i[c], w_i[c] of the pending slots
+  const ESec S = secs[blockIdx.x];
+  const int n = S.n;
+  if (c >= n) return;
+  constexpr int nt = kTR;
+  constexpr int PR = P > 0 ? P : 1;
+  const int tid = threadIdx.x;
+  const int k = c - c % kPB;                             // first column of c's panel
+  const int pb = c == k ? (c >= kPB ? c - kPB : c) : k;  // pending reflectors [pb, c)
+  const int npend = c - pb;
+  const int ncorr = npend > 0 ? npend - 1 : 0;           // [pb, c - 1): panel of c - 1, before it
+  const int slast = c - 1 - pb;                          // slot of reflector c - 1 (w_{c-1} is formed here)
+  const double *V = A.VH + S.nn;
+  const double *srow = A.S + S.nn + (int64_t)c * n;
+  const double *vp = V + (int64_t)(c > 0 ? c - 1 : 0) * n;
+  const double *y = A.y + S.nv;
+  double *ascr = A.a + S.nv;
+  // slot i: reflector pb + i if i < npend, else reflector 0 with weight 0
+  auto vcol = [&](int i) { return V + (int64_t)(i < npend ? pb + i : 0) * n; };
+  auto wcl = [&](int i) { return wcol(A, S, i < npend ? pb + i : 0); };
+  if (tid < 2 * kPB) {
+    const int i = tid % kPB;
+    un[tid / kPB][i] = (i < npend ? 1.0 : 0.0) * (tid < kPB ? vcol(i) : wcl(i))[c];
+  }
+  // ---- loads: row c of S, y, v_{c-1} and the pending reflectors' rows
+  double sr[PR], yr[PR], vr[PR], vv[PR][kPB], ww[PR][kPB];
+  if (P > 0) {
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+      const int r = c + tid + j * nt, rc = r < n ? r : n - 1;
+      sr[j] = srow[rc];
+      yr[j] = y[rc];
+      vr[j] = (r < n ? 1.0 : 0.0) * vp[rc];
+#pragma unroll
+      for (int i = 0; i < kPB; ++i) {
+        vv[j][i] = vcol(i)[rc];
+        ww[j][i] = wcl(i)[rc];
       }
-      const double kcoef = -0.5 * tau * tau * block_sum(part, red);
-      const double wc = tau * yc + kcoef * vc;
+    }
+  }
+  const double tp = c > 0 ? A.tau[S.nv + c - 1] : 0.0;
+  const double yc = y[c];
+  double wr[PR];
 #pragma unroll
-      for (int k = 0; k < P; ++k) {
-        const int r = c + tid + k * nt;
-        if (r < n) {
-          const double x = tau * yr[k] + kcoef * vr[k];
-          w[r] = x;
-          ar[k] = ar[k] - vc * x - wc * vr[k];
+  for (int j = 0; j < PR; ++j) wr[j] = 0.0;
+  double wc = 0.0;  // w_{c-1}[c]
+  if (c > 0) {
+    // 1. y.v_{c-1}, p_i = w_i.v_{c-1}, s_i = v_i.v_{c-1} (i < ncorr) over rows >= c
+    double sums[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sums[j] = 0.0;
+    if (P > 0) {
+#pragma unroll
+      for (int j = 0; j < PR; ++j) {
+        sums[0] = fma(yr[j], vr[j], sums[0]);
+#pragma unroll
+        for (int i = 0; i < kPB - 1; ++i) {
+          sums[1 + i] = fma(ww[j][i], vr[j], sums[1 + i]);
+          sums[8 + i] = fma(vv[j][i], vr[j], sums[8 + i]);
+        }
+      }
+    } else {
+      for (int r = c + tid; r < n; r += nt) {
+        const double x = vp[r];
+        sums[0] = fma(y[r], x, sums[0]);
+#pragma unroll
+        for (int i = 0; i < kPB - 1; ++i) {
+          sums[1 + i] = fma(wcl(i)[r], x, sums[1 + i]);
+          sums[8 + i] = fma(vcol(i)[r], x, sums[8 + i]);
         }
       }
     }
+    block_sum16(sums, red);  // also makes un[] visible
+    double pv[kPB - 1], sv[kPB - 1], ps = 0.0;
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-      const int r = c + tid + k * nt;
-      if (r < n && r >= c + 2) ss += ar[k] * ar[k];
+    for (int i = 0; i < kPB - 1; ++i) {
+      const double m = i < ncorr ? 1.0 : 0.0;
+      pv[i] = m * sums[1 + i];
+      sv[i] = m * sums[8 + i];
+      ps = fma(pv[i], sv[i], ps);
     }
-    if (tid == 0) dv[S.nv + c] = ar[0];  // row c is thread 0's first
-    if (c >= n - 1) return;
-    if (tid == 1) sh[0] = ar[0];  // a[c+1], read by all after the barrier below
-    const double xn = sqrt(block_sum(ss, red));
-    const double alpha = sh[0];
-    double beta, tau, sc;
-    if (xn == 0.0) {
-      beta = alpha;
-      tau = 0.0;
-      sc = 0.0;
+    const double al = -0.5 * tp * tp * (sums[0] - 2.0 * ps);
+    // 2. w_{c-1} on rows >= c: tau (y - sum_i (v_i p_i + w_i s_i)) + alpha v_{c-1}
+    double *w = wcol(A, S, c - 1);
+    {
+      double t = yc;
+#pragma unroll
+      for (int i = 0; i < kPB - 1; ++i) {
+        t = fma(-un[0][i], pv[i], t);
+        t = fma(-un[1][i], sv[i], t);
+      }
+      wc = fma(al, 1.0, tp * t);  // v_{c-1}[c] = 1
+    }
+    if (P > 0) {
+#pragma unroll
+      for (int j = 0; j < PR; ++j) {
+        double t = yr[j];
+#pragma unroll
+        for (int i = 0; i < kPB - 1; ++i) {
+          t = fma(-vv[j][i], pv[i], t);
+          t = fma(-ww[j][i], sv[i], t);
+        }
+        wr[j] = fma(al, vr[j], tp * t);
+        const int r = c + tid + j * nt;
+        if (r < n) w[r] = wr[j];
+      }
     } else {
-      beta = -copysign(hypot(alpha, xn), alpha);
-      tau = (beta - alpha) / beta;
-      sc = 1.0 / (alpha - beta);
-    }
-    double *vh = VH + S.nn + (int64_t)c * n;
+      for (int r = c + tid; r < n; r += nt) {
+        double t = y[r];
 #pragma unroll
-    for (int k = 0; k < P; ++k) {
-      const int r = c + tid + k * nt;
-      if (r > c && r < n) vh[r] = r == c + 1 ? 1.0 : ar[k] * sc;
-    }
-    if (tid == 0) {
-      ev[S.nv + c] = beta;
-      tauv[S.nv + c] = tau;
-    }
-    return;
-  }
-  double *a = abuf + S.nv;
-  // One pass forms w_{c-1}, the corrected row c and the squared norm of its
-  // tail; w_{c-1}[c] is recomputed by every thread instead of broadcast.
-  double ss = 0;
-  if (c > 0) {
-    const double tau = tauv[S.nv + c - 1];
-    const double *vp = VH + S.nn + (int64_t)(c - 1) * n;
-    const double *y = ybuf + S.nv;
-    double *w = wbuf + S.nv;
-    double part = 0;
-    for (int r = c + tid; r < n; r += nt) part += y[r] * vp[r];
-    const double kcoef = -0.5 * tau * tau * block_sum(part, red);
-    const double vc = vp[c], wc = tau * y[c] + kcoef * vc;
-    for (int r = c + tid; r < n; r += nt) {
-      const double x = tau * y[r] + kcoef * vp[r];
-      w[r] = x;
-      const double av = arow[r] - vc * x - wc * vp[r];
-      a[r] = av;
-      if (r >= c + 2) ss += av * av;
+        for (int i = 0; i < kPB - 1; ++i) {
+          t = fma(-vcol(i)[r], pv[i], t);
+          t = fma(-wcl(i)[r], sv[i], t);
+        }
+        w[r] = fma(al, vp[r], tp * t);
+      }
     }
   } else {
-    for (int r = tid; r < n; r += nt) {
-      const double av = arow[r];
-      a[r] = av;
-      if (r >= 2) ss += av * av;
+    __syncthreads();  // un[] (the sums above carry this barrier otherwise)
+  }
+  // 3. column c of A_true, a[r] = S[c][r] - sum_{i in [pb, c)} (v_i[r] w_i[c] + w_i[r] v_i[c])
+  double vic[kPB], wic[kPB];
+#pragma unroll
+  for (int i = 0; i < kPB; ++i) {
+    vic[i] = un[0][i];
+    wic[i] = i == slast ? wc : un[1][i];
+  }
+  double ss = 0.0, ar[PR];
+  if (P > 0) {
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+      const int r = c + tid + j * nt;
+      double a = sr[j];
+#pragma unroll
+      for (int i = 0; i < kPB; ++i) {
+        a = fma(-vv[j][i], wic[i], a);
+        a = fma(-(i == slast ? wr[j] : ww[j][i]), vic[i], a);
+      }
+      ar[j] = a;
+      const double am = (r < n && r >= c + 2) ? a : 0.0;
+      ss = fma(am, am, ss);
+    }
+  } else {
+    ar[0] = 0.0;
+    for (int r = c + tid; r < n; r += nt) {
+      double a = srow[r];
+#pragma unroll
+      for (int i = 0; i < kPB; ++i) {
+        const double wm = wcl(i)[r];
+        a = fma(-vcol(i)[r], wic[i], a);
+        a = fma(-wm, vic[i], a);  // slot c - 1: w_{c-1}[r], written above by this thread
+      }
+      ascr[r] = a;
+      if (r >= c + 2) ss = fma(a, a, ss);
     }
   }
-  if (tid == 0) dv[S.nv + c] = a[c];  // a[c] was written by this thread
+  if (tid == 0) A.d[S.nv + c] = P > 0 ? ar[0] : ascr[c];  // row c is thread 0's first
   if (c >= n - 1) return;
+  // 4. reflector of column c: alpha = a[c+1], x = a[c+2:]
+  if (tid == 1) sh[0] = P > 0 ? ar[0] : ascr[c + 1];  // read after the barriers of the sum
   const double xn = sqrt(block_sum(ss, red));
-  const double alpha = a[c + 1];
+  const double alpha = sh[0];
   double beta, tau, sc;
   if (xn == 0.0) {
     beta = alpha;
@@ -208,38 +349,92 @@ __device__ void row_step(const ESec &S, int c, const double *__restrict__ A, dou
     tau = (beta - alpha) / beta;
     sc = 1.0 / (alpha - beta);
   }
-  double *vh = VH + S.nn + (int64_t)c * n;
-  for (int r = c + 1 + tid; r < n; r += nt) vh[r] = r == c + 1 ? 1.0 : a[r] * sc;
+  double *vh = A.VH + S.nn + (int64_t)c * n;
+  if (P > 0) {
+#pragma unroll
+    for (int j = 0; j < PR; ++j) {
+      const int r = c + tid + j * nt;
+      if (r > c && r < n) vh[r] = r == c + 1 ? 1.0 : ar[j] * sc;
+    }
+  } else {
+    for (int r = c + tid; r < n; r += nt)
+      if (r > c) vh[r] = r == c + 1 ? 1.0 : ascr[r] * sc;
+  }
   if (tid == 0) {
-    ev[S.nv + c] = beta;
-    tauv[S.nv + c] = tau;
+    A.e[S.nv + c] = beta;
+    A.tau[S.nv + c] = tau;
   }
 }
 
-// Column step c of every sector (one workgroup of kTR threads per sector).
-constexpr int kTR = 512;
-__global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ secs, int c,
-                                                      const double *__restrict__ A, double *VH, double *wbuf,
-                                                      const double *__restrict__ ybuf, double *abuf, double *dv,
-                                                      double *ev, double *tauv) {
-  __shared__ double red[kTR / 64];
-  __shared__ double sh[2];
-  const ESec S = secs[blockIdx.x];
-  if (c >= S.n) return;
-  row_step(S, c, A, VH, wbuf, ybuf, abuf, dv, ev, tauv, red, sh);
+// Deferred update at the first column c of a panel:
+//   S[r][q] -= sum_{i in [c - kPB, c)} (v_i[r] w_i[q] + w_i[r] v_i[q])   for r, q > c,
+// 64 x 64 tiles (the full square), the panel entries of the tile's rows and
+// columns staged in LDS; thread t updates column t % 64 of 16 rows.
+__global__ __launch_bounds__(256) void trd_update_kernel(const ESec *__restrict__ secs, int nsec, int c,
+                                                         TrdArgs A) {
+  int b = blockIdx.x, q = 0, nb = 0;
+  for (; q < nsec; ++q) {
+    const int n = secs[q].n;
+    if (c < n - 1) {
+      nb = (n - c - 1 + kUT - 1) / kUT;
+      if (b < nb * nb) break;
+      b -= nb * nb;
+    }
+  }
+  if (q >= nsec) return;
+  const ESec S = secs[q];
+  const int n = S.n, t = threadIdx.x;
+  const int r0 = c + 1 + (b / nb) * kUT, q0 = c + 1 + (b % nb) * kUT;
+  __shared__ double pv[4][kPB][kUT];  // v_i, w_i on the tile's rows; v_i, w_i on its columns
+  const int i0 = c - kPB;
+  for (int idx = t; idx < 4 * kPB * kUT; idx += 256) {
+    const int which = idx / (kPB * kUT), i = (idx / kUT) % kPB, x = idx % kUT;
+    const int rr = (which < 2 ? r0 : q0) + x;
+    double val = 0.0;
+    if (rr < n) val = (which & 1) ? wcol(A, S, i0 + i)[rr] : A.VH[S.nn + (int64_t)(i0 + i) * n + rr];
+    pv[which][i][x] = val;
+  }
+  __syncthreads();
+  const int x = t & 63, ry = (t >> 6) * 16;
+  const int qq = q0 + x;
+  if (qq >= n) return;
+  double vq[kPB], wq[kPB];
+#pragma unroll
+  for (int i = 0; i < kPB; ++i) {
+    vq[i] = pv[2][i][x];
+    wq[i] = pv[3][i][x];
+  }
+  double *Sm = A.S + S.nn;
+  // all 16 loads in flight first, then the rows one at a time (the panel
+  // entries of a row read from LDS at one address per wavefront: broadcast)
+  double sv[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int r = r0 + ry + j;
+    sv[j] = r < n ? Sm[(int64_t)r * n + qq] : 0.0;
+  }
+#pragma unroll 2
+  for (int j = 0; j < 16; ++j) {
+    double s = sv[j];
+#pragma unroll
+    for (int i = 0; i < kPB; ++i) {
+      s = fma(-pv[0][i][ry + j], wq[i], s);
+      s = fma(-pv[1][i][ry + j], vq[i], s);
+    }
+    sv[j] = s;
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const int r = r0 + ry + j;
+    if (r < n) Sm[(int64_t)r * n + qq] = sv[j];
+  }
 }
 
-// Rows r > c of the trailing square, one pass: apply the rank-2 update of
-// column c-1, A[r][q] -= v[r] w[q] + w[r] v[q] (q > c), store, and form
-// y[r] = sum_{q > c} A[r][q] v_c[q]; kSymvRows rows per workgroup, kRowsPerWave
-// per wavefront, lanes along the row.  (A device-scope fence per workgroup,
-// to hand the column step to the last workgroup, would write back the whole
-// L2 this kernel dirties; the column step is a launch of its own instead.)
-__global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restrict__ secs, int nsec, int c, double *A,
-                                                             const double *__restrict__ VH,
-                                                             const double *__restrict__ wbuf, double *ybuf) {
-  int b = blockIdx.x;
-  int q = 0;
+// Product kernel for column c, rows r > c (kSymvRows per workgroup, one per
+// wavefront, lanes along the row): y[r] = sum_{q > c} S[r][q] v_c[q].
+// S is only read.
+__global__ __launch_bounds__(kT) void trd_symv_kernel(const ESec *__restrict__ secs, int nsec, int c, TrdArgs A) {
+  int b = blockIdx.x, q = 0;
   for (; q < nsec; ++q) {
     const int n = secs[q].n;
     if (c < n - 1) {
@@ -251,87 +446,28 @@ __global__ __launch_bounds__(kT) void trd_update_symv_kernel(const ESec *__restr
   if (q >= nsec) return;
   const ESec S = secs[q];
   const int n = S.n, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const double *v = VH + S.nn + (int64_t)c * n;
-  const double *vp = VH + S.nn + (int64_t)(c > 0 ? c - 1 : 0) * n;
-  const double *w = wbuf + S.nv;
-  const int r0 = c + 1 + b * kSymvRows + wv * kRowsPerWave;
-  double *ar[kRowsPerWave];
-  double vr[kRowsPerWave], wr[kRowsPerWave], acc[kRowsPerWave];
-  bool ok[kRowsPerWave];
-#pragma unroll
-  for (int i = 0; i < kRowsPerWave; ++i) {
-    const int r = r0 + i;
-    ok[i] = r < n;
-    ar[i] = A + S.nn + (int64_t)(ok[i] ? r : c + 1) * n;
-    vr[i] = (ok[i] && c > 0) ? vp[r] : 0.0;
-    wr[i] = (ok[i] && c > 0) ? w[r] : 0.0;
-    acc[i] = 0.0;
-  }
-  // Columns in chunks of four 64-wide strides: the loads of a chunk are all
-  // issued before its arithmetic (memory-level parallelism for a wave that
-  // is otherwise alone on its SIMD); each lane still sums its columns in
-  // increasing order.  Rows that are not ok point at row c + 1 (< n), so the
-  // chunk loads stay in bounds; only ok rows are written.
+  const int r = c + 1 + b * kSymvRows + wv;
+  if (r >= n) return;  // whole wavefront; no barrier follows
+  const double *row = A.S + S.nn + (int64_t)r * n;
+  const double *v = A.VH + S.nn + (int64_t)c * n;
+  // Columns in chunks of four 64-wide strides, the loads of a chunk issued
+  // before its arithmetic; each lane sums its columns in increasing order.
   constexpr int U = 4;
+  double acc = 0;
   int col = c + 1 + lane;
-  if (c > 0) {
-    for (; col + (U - 1) * 64 < n; col += U * 64) {
-      double x[U], vq[U], wq[U], t[U][kRowsPerWave];
+  for (; col + (U - 1) * 64 < n; col += U * 64) {
+    double x[U], t[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        x[u] = v[col + 64 * u];
-        vq[u] = vp[col + 64 * u];
-        wq[u] = w[col + 64 * u];
-#pragma unroll
-        for (int i = 0; i < kRowsPerWave; ++i) t[u][i] = ar[i][col + 64 * u];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int i = 0; i < kRowsPerWave; ++i)
-          if (ok[i]) {
-            const double tt = t[u][i] - vr[i] * wq[u] - wr[i] * vq[u];
-            ar[i][col + 64 * u] = tt;
-            acc[i] += tt * x[u];
-          }
+    for (int u = 0; u < U; ++u) {
+      x[u] = v[col + 64 * u];
+      t[u] = row[col + 64 * u];
     }
-    for (; col < n; col += 64) {
-      const double x = v[col], vq = vp[col], wq = w[col];
 #pragma unroll
-      for (int i = 0; i < kRowsPerWave; ++i)
-        if (ok[i]) {
-          const double t = ar[i][col] - vr[i] * wq - wr[i] * vq;
-          ar[i][col] = t;
-          acc[i] += t * x;
-        }
-    }
-  } else {
-    for (; col + (U - 1) * 64 < n; col += U * 64) {
-      double x[U], t[U][kRowsPerWave];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        x[u] = v[col + 64 * u];
-#pragma unroll
-        for (int i = 0; i < kRowsPerWave; ++i) t[u][i] = ar[i][col + 64 * u];
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int i = 0; i < kRowsPerWave; ++i)
-          if (ok[i]) acc[i] += t[u][i] * x[u];
-    }
-    for (; col < n; col += 64) {
-      const double x = v[col];
-#pragma unroll
-      for (int i = 0; i < kRowsPerWave; ++i)
-        if (ok[i]) acc[i] += ar[i][col] * x;
-    }
+    for (int u = 0; u < U; ++u) acc = fma(t[u], x[u], acc);
   }
-#pragma unroll
-  for (int i = 0; i < kRowsPerWave; ++i) {
-    const double sum = wave_sum(acc[i]);
-    if (lane == 0 && ok[i]) ybuf[S.nv + r0 + i] = sum;
-  }
+  for (; col < n; col += 64) acc = fma(row[col], v[col], acc);
+  const double y = wave_sum(acc);
+  if (lane == 0) A.y[S.nv + r] = y;
 }
 
 // T block of panel p (compact WY, forward columnwise) from the Gram of its
@@ -838,20 +974,49 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
   st->Tb = c->alloc(std::max<int64_t>(tb, 1) * sizeof(double));
   st->Z = c->alloc(nn * sizeof(double));
   auto bd = c->alloc(nv * sizeof(double)), be = c->alloc(nv * sizeof(double)), btau = c->alloc(nv * sizeof(double)),
-       by = c->alloc(nv * sizeof(double)), bw = c->alloc(nv * sizeof(double)), ba = c->alloc(nv * sizeof(double));
+       by = c->alloc(nv * sizeof(double)), ba = c->alloc(nv * sizeof(double)),
+       bw0 = c->alloc(nv * kPB * sizeof(double)), bw1 = c->alloc(nv * kPB * sizeof(double));
   double *VH = dp<double>(st->VH), *Tb = dp<double>(st->Tb);
-  double *dv = dp<double>(bd), *ev = dp<double>(be), *tauv = dp<double>(btau), *yb = dp<double>(by);
+  double *dv = dp<double>(bd), *ev = dp<double>(be), *tauv = dp<double>(btau);
   k_fill(c, VH, nn, 0.0);
   k_fill(c, ev, nv, 0.0);
   k_fill(c, tauv, nv, 0.0);
-  // ---- 1. tridiagonalisation, one column per step for every sector
-  for (int col = 0; col < maxn; ++col) {
-    trd_row_kernel<<<nsec, kTR, 0, c->stream>>>(SD, col, G, VH, dp<double>(bw), yb, dp<double>(ba), dv, ev, tauv);
-    int64_t nwg = 0;
-    for (int q = 0; q < nsec; ++q)
-      if (col < ns[q] - 1) nwg += (ns[q] - col - 1 + kSymvRows - 1) / kSymvRows;
-    if (nwg > 0) trd_update_symv_kernel<<<(int)nwg, kT, 0, c->stream>>>(SD, nsec, col, G, VH, dp<double>(bw), yb);
+  k_fill(c, dp<double>(bw0), nv * kPB, 0.0);  // read with zero weight before written
+  k_fill(c, dp<double>(bw1), nv * kPB, 0.0);
+  auto clock_ms = [](std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  };
+  const auto t_trd = std::chrono::steady_clock::now();
+  {
+    double n3 = 0;
+    for (int q = 0; q < nsec; ++q) n3 += (double)ns[q] * ns[q] * ns[q];
+    c->count("eigh_n3_sum_g", n3 * 1e-9);   // sum of n^3 over sectors, 1e9 units
   }
+  // ---- 1. tridiagonalisation, one column per step for every sector
+  const TrdArgs TA{G, VH, dp<double>(bw0), dp<double>(bw1), dp<double>(by), dp<double>(ba), dv, ev, tauv};
+  auto launch_col = [&](int col) {
+    int rows = 0;
+    int64_t nwg = 0, nut = 0;
+    for (int q = 0; q < nsec; ++q) {
+      if (col < ns[q]) rows = std::max(rows, ns[q] - col);
+      if (col < ns[q] - 1) {
+        nwg += (ns[q] - col - 1 + kSymvRows - 1) / kSymvRows;
+        const int64_t nb = (ns[q] - col - 1 + kUT - 1) / kUT;
+        nut += nb * nb;
+      }
+    }
+    switch ((rows + kTR - 1) / kTR) {
+      case 1: trd_row_kernel<1><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
+      case 2: trd_row_kernel<2><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
+      case 3:
+      case 4: trd_row_kernel<4><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
+      default: trd_row_kernel<0><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
+    }
+    if (col % kPB == 0 && col >= kPB && nut > 0)
+      trd_update_kernel<<<(unsigned)nut, 256, 0, c->stream>>>(SD, nsec, col, TA);
+    if (nwg > 0) trd_symv_kernel<<<(unsigned)nwg, kT, 0, c->stream>>>(SD, nsec, col, TA);
+  };
+  for (int col = 0; col < maxn; ++col) launch_col(col);
   // WY blocks: Gram of each panel's reflectors (MFMA), then the T recurrence
   {
     std::vector<PanelRef> prs;
@@ -882,6 +1047,8 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
     c->sync();
   }
   c->count("eigh_trd_columns", maxn);
+  c->count("eigh_trd_ms", clock_ms(t_trd));
+  const auto t_dc = std::chrono::steady_clock::now();
   // ---- 2. divide and conquer on (d, e)
   auto bl = c->alloc(nv * sizeof(double));
   DcBufs B{};
@@ -980,6 +1147,7 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
                             hipMemcpyDeviceToHost, c->stream));
   }
   c->sync();
+  c->count("eigh_dc_ms", clock_ms(t_dc));
   for (int q = 0; q < nsec; ++q)
     for (double x : res.evals[q])
       if (!std::isfinite(x)) throw Error(kErrNumeric, "eigh: non-finite eigenvalue (input holds NaN or Inf)");
@@ -990,6 +1158,7 @@ void eigh_vectors_all(Ctx *c, const EighResult &r, const std::vector<std::vector
                       const std::vector<int64_t> &eoff) {
   const EighState &st = *r.st;
   const int nsec = st.nsec;
+  const auto t_bt = std::chrono::steady_clock::now();
   // E_q = Z_q[:, cols_q]
   std::vector<BufP> keep;
   std::vector<int64_t> xoff(nsec, 0);
@@ -1054,6 +1223,8 @@ void eigh_vectors_all(Ctx *c, const EighResult &r, const std::vector<std::vector
                                 dp<BlkD>(db), dp<PairD>(dpp), 64, 64, 1);
   }
   c->sync();
+  c->count("eigh_back_ms",
+           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_bt).count());
 }
 
 }  // namespace mps2

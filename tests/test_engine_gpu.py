@@ -194,6 +194,51 @@ def test_svd_parity(ctx, orth, dims):
 
 
 @pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+def test_svd_large_sector_known_spectrum(ctx, orth):
+    """A 3300-wide sector beside sectors of 700 and 64, each with a known
+    spectrum graded over 10 decades.  The eigensolver's first 228 columns of
+    the large sector take its through-memory row path, the rest its one-,
+    two- and three-rows-per-thread paths, across 412 deferred panel updates.
+    The singular values match the constructed ones (1e-9 s_max above
+    1e-2 s_max, 1e-6 s_max everywhere), U S Vt reproduces the matrix to
+    2e-9 s_max (a direction whose Gram eigenvalue comes out <= 0, at
+    rounding level, gets s = 0 and drops its component, of size s < 1e-8
+    s_max here), and a cut at 2000 (singular value 1.3e-5 s_max, so the
+    tail is refined) keeps per sector what the exact values keep (the gap at
+    the cut is 0.2%)."""
+    rng = np.random.default_rng(31 + orth)
+    dims, qns = (3300, 700, 64), (-2, 0, 2)
+    l, r = H.bond(IN, dims, qns), H.bond(OUT, dims, qns)
+    psi = BTensor([l, r])
+    exact = []
+    for i, n in enumerate(dims):
+        s = 10.0 ** (-10.0 * np.arange(n) / n) * (1.0 + 0.37 * i)
+        u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        psi.blocks[(i, i)] = (u * s) @ v.T
+        exact += [(x, i) for x in s]
+    exact.sort(key=lambda t: -t[0])
+    smax = exact[0][0]
+    dpsi = ctx.put(psi)
+    u, s, vt, eg, Dg = E.svd(dpsi, 1, (0,), 0.0, 1, sum(dims), orth)
+    Ug, Sg, Vg = u.get(), s.get(), vt.get()
+    sg = np.sort(np.concatenate([np.diag(b) for b in Sg.blocks.values()]))[::-1]
+    se = np.array([x for x, _ in exact])
+    assert Dg == sum(dims) and sg.shape == se.shape
+    big = se >= 1e-2 * smax
+    assert np.abs(sg - se)[big].max() <= 1e-9 * smax, np.abs(sg - se)[big].max() / smax
+    assert np.abs(sg - se).max() <= 1e-6 * smax, np.abs(sg - se).max() / smax
+    rec = np.einsum("ak,kl,lb->ab", Ug.dense(), Sg.dense(), Vg.dense(), optimize=True)
+    assert np.abs(rec - psi.dense()).max() < 2e-9 * smax, np.abs(rec - psi.dense()).max() / smax
+    u, s, vt, eg, Dg = E.svd(dpsi, 1, (0,), 0.0, 1, 2000, orth)
+    kept = {q: 0 for q in range(len(dims))}
+    for _, q in exact[:2000]:
+        kept[q] += 1
+    assert Dg == 2000
+    assert u.get().legs[-1].key() == H.bond(OUT, tuple(kept[q] for q in range(3)), qns).key()
+
+
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
 def test_svd_rank_deficient_parity(ctx, orth):
     """psi = A B through a bond of total dimension 12: every sector matrix is
     numerically rank-deficient (112 of 124 singular values at rounding level,
