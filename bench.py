@@ -155,6 +155,48 @@ def measured_traffic(args):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def svd_workload(args):
+    """Truncated SVD of a block-diagonal matrix with the sector sizes
+    --svd-sectors (block q is U_q diag(s_q) V_q^T with seeded random
+    orthogonal U_q, V_q and s_q graded over --svd-decades decades), Dmax half
+    the total dimension: --warmup untimed and --steps timed calls, each with
+    the SVD's and the eigensolver's phase counters.  One JSON line."""
+    from mps2_amd import engine as E
+    from mps2_amd.flat import BTensor, IN, OUT, Leg
+    dims = [int(x) for x in args.svd_sectors.split(",") if x]
+    qns = [2 * i - 2 * (len(dims) // 2) for i in range(len(dims))]
+    rng = np.random.default_rng(args.seed)
+    psi = BTensor([Leg(IN, [((q,), d) for q, d in zip(qns, dims)]), Leg(OUT, [((q,), d) for q, d in zip(qns, dims)])])
+    for i, n in enumerate(dims):
+        s = 10.0 ** (-args.svd_decades * np.arange(n) / n) / (1 + i)
+        u, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        psi.blocks[(i, i)] = (u * s) @ v.T
+    ctx = E.Context(0)
+    dpsi = ctx.put(psi)
+    dmax = sum(dims) // 2
+    names = ("svd_ms", "svd_eigh_ms", "svd_refine_ms", "svd_refine_eigh_ms", "svd_refine_rounds", "eigh_trd_ms",
+             "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns")
+    calls = []
+    for rep in range(args.warmup + args.steps):
+        E.check(E.lib().mps2_ctx_sync(ctx.h))
+        before = E.ctx_counters(ctx.h, names)
+        t0 = time.perf_counter()
+        E.svd(dpsi, 1, (0,), 0.0, 1, dmax, E.ORTH_LEFT)
+        E.check(E.lib().mps2_ctx_sync(ctx.h))
+        wall = (time.perf_counter() - t0) * 1e3
+        after = E.ctx_counters(ctx.h, names)
+        if rep >= args.warmup:
+            calls.append(dict(wall_ms=round(wall, 3), **{k: round(after[k] - before[k], 3) for k in names}))
+        log(f"svd call {rep}: {wall:.1f} ms")
+    out = {"metric": "truncated SVD wall-time per call, centre-bond sector structure", "value": min(c["wall_ms"] for c in calls),
+           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "dtype": "f64", "data": "synthetic (seeded random orthogonal factors, graded spectra)",
+           "config": {"workload": "truncated SVD, block-diagonal sectors", "sectors": dims,
+                      "decades": args.svd_decades, "dmax": dmax}, "calls": calls}
+    print(json.dumps(out))
+
+
 def timed_steps(engine, ctxh, op, warmup, steps):
     """`warmup` untimed then exactly `steps` timed applications of op; the
     context stream is synchronised on both sides of the timed region."""
@@ -184,6 +226,10 @@ def main():
     ap.add_argument("--cpu-budget-s", type=float, default=120.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1234)
+    ap.add_argument("--svd-sectors", default="",
+                    help="instead of the sweep benchmark: time the truncated SVD of a block-diagonal matrix "
+                         "with these sector sizes (comma separated)")
+    ap.add_argument("--svd-decades", type=float, default=12.0, help="spectral range of each sector (--svd-sectors)")
     ap.add_argument("--profile", action="store_true",
                     help="profiler runs: state, steps, per-stage timing and the calibration copy only")
     args = ap.parse_args()
@@ -197,6 +243,9 @@ def main():
         print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
         sys.exit(2)
 
+    if args.svd_sectors:
+        svd_workload(args)
+        return
     from mps2_amd import engine
     from mps2_amd import gqmps2 as G
     from mps2_amd import hamiltonians as HM
