@@ -38,7 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "sweep wall-time + H·ψ TFLOP/s at D=4096: FP64 vs mixed, |ΔE|/|E|≤1e-10"
-PEAK = {"fp64": 78.6, "fp32": 157.3}   # MI355X dense matrix-core TFLOP/s
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}   # MI355X dense matrix-core TFLOP/s
 T_START = time.perf_counter()
 # The GEMM autotuner's decisions of this run go to a fresh file, summarised in
 # the output (heuristic against chosen kernel, per tuned launch shape).
@@ -140,7 +140,8 @@ def tune_summary():
     changed = sum(1 for r in rows if float(r[4]) < float(r[5]))
     return dict(shapes_tuned=len(rows), shapes_changed=changed, heuristic_ms_sum=round(heur, 3),
                 tuned_ms_sum=round(best, 3), speedup=round(heur / best, 4) if best > 0 else None,
-                choices=sorted({f"{'fp64' if r[1] == '1' else 'fp32'} tile {r[2]} variant {r[3]}" for r in rows}))
+                choices=sorted({f"{ {'1': 'fp64', '0': 'fp32', '2': 'bf16x3'}.get(r[1], r[1])} tile {r[2]} "
+                                f"variant {r[3]}" for r in rows}))
 
 
 def measured_traffic(args):
@@ -271,7 +272,8 @@ def main():
     ctxh = G.lib().gqmps2_engine_ctx()
 
     res = {}
-    for name, prec in (() if args.profile else (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED))):
+    for name, prec in (() if args.profile else (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED),
+                                                 ("mixed_bf16x3", engine.PREC_MIXED_BF16X3))):
         model.restore()
         engine.check(engine.lib().mps2_ctx_sync(ctxh))
         G.engine_reset_stats()
@@ -294,17 +296,24 @@ def main():
     log(f"centre-bond per-stage timing ({time.perf_counter() - t0:.1f}s): stage ms fp64 {mb['fp64']['stage_ms']} "
         f"mixed {mb['mixed']['stage_ms']}")
     L_, W1_, W2_, R_, psi_ = model.bench_tensors()
+    # the split-BF16x3 matvec on the same tensors (its dominant stage is timed
+    # as the tagged instance grouped_gemm_bf3_kernel<..., 1>)
+    mb["mixed_bf16x3"] = engine.heff_bench(L_, W1_, W2_, R_, psi_, "cent", engine.PREC_MIXED_BF16X3, warmup=8,
+                                           reps=args.matvec_reps)
+    log(f"centre-bond stage ms split-bf16x3 {[round(x, 3) for x in mb['mixed_bf16x3']['stage_ms']]}")
     # the same centre-bond matvec with the heuristic kernels only
     engine.check(engine.lib().mps2_ctx_set_tuning(ctxh, 0))
     heur = {k: engine.heff_bench(L_, W1_, W2_, R_, psi_, "cent", p, warmup=1, reps=args.matvec_reps)
-            for k, p in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED))}
+            for k, p in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED),
+                         ("mixed_bf16x3", engine.PREC_MIXED_BF16X3))}
     engine.check(engine.lib().mps2_ctx_set_tuning(ctxh, 1))
     log(f"centre-bond stage ms with heuristic kernels: fp64 {[round(x, 3) for x in heur['fp64']['stage_ms']]} "
         f"mixed {[round(x, 3) for x in heur['mixed']['stage_ms']]}")
 
     # the steps: exactly --steps centre-bond matvecs per precision
     steps = {}
-    for name, prec in (("mixed", engine.PREC_MIXED), ("fp64", engine.PREC_FP64)):
+    for name, prec in (("mixed", engine.PREC_MIXED), ("fp64", engine.PREC_FP64),
+                       ("mixed_bf16x3", engine.PREC_MIXED_BF16X3)):
         op = engine.HeffOp(L_, W1_, W2_, R_, psi_, "cent", prec)
         dt = timed_steps(engine, ctxh, op, args.warmup, args.steps)
         steps[name] = dict(seconds=dt, ms=dt / args.steps * 1e3, flops=op.flops,
@@ -328,6 +337,16 @@ def main():
                     algorithmic_bytes_per_launch=dom.get("dom_bytes"))
     d64 = mb["fp64"]
     d64_ach = d64["dom_flops"] / (d64["dom_ms"] * 1e-3) / 1e12 if d64["dom_ms"] > 0 else 0.0
+    dbf = mb["mixed_bf16x3"]
+    dbf_ach = dbf["dom_flops"] / (dbf["dom_ms"] * 1e-3) / 1e12 if dbf["dom_ms"] > 0 else 0.0
+    # split-BF16x3 issues three BF16 products per algorithmic multiply-add
+    roofline_bf = dict(bound="mfma", achieved=round(dbf_ach, 3), peak=PEAK["bf16"], unit="TFLOP/s",
+                       frac=round(dbf_ach / PEAK["bf16"], 4),
+                       executed_frac=round(3 * dbf_ach / PEAK["bf16"], 4),
+                       kernel=f"grouped_gemm_bf3_kernel<..., 1> (stage {dbf['dom_stage'] + 1} of the centre-bond "
+                              f"matvec, run as the tagged instance)",
+                       flops_per_launch=dbf["dom_flops"], ms_per_launch=dbf["dom_ms"],
+                       algorithmic_bytes_per_launch=dbf.get("dom_bytes"))
 
     if args.profile:
         print(json.dumps({"metric": METRIC, "profile": True, "steps": args.steps,
@@ -354,7 +373,8 @@ def main():
         for k in bt.blocks:
             bt.blocks[k] = rng.standard_normal(bt.blocks[k].shape)
         lz = {}
-        for name, prec in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED)):
+        for name, prec in (("fp64", engine.PREC_FP64), ("mixed", engine.PREC_MIXED),
+                           ("mixed_bf16x3", engine.PREC_MIXED_BF16X3)):
             init = engine.put_on(ctxh, bt)
             engine.check(engine.lib().mps2_ctx_sync(ctxh))
             t0 = time.perf_counter()
@@ -362,17 +382,24 @@ def main():
             engine.check(engine.lib().mps2_ctx_sync(ctxh))
             lz[name] = dict(iters=it, energy=e, s=time.perf_counter() - t0, gs=gs)
             init.free()
-        ov = abs(engine.dot(lz["fp64"]["gs"], lz["mixed"]["gs"])) / np.sqrt(
-            engine.dot(lz["fp64"]["gs"], lz["fp64"]["gs"]) * engine.dot(lz["mixed"]["gs"], lz["mixed"]["gs"]))
+        def overlap(a, b):
+            return abs(engine.dot(a, b)) / np.sqrt(engine.dot(a, a) * engine.dot(b, b))
+        ov = overlap(lz["fp64"]["gs"], lz["mixed"]["gs"])
+        ov_bf = overlap(lz["fp64"]["gs"], lz["mixed_bf16x3"]["gs"])
         lanczos_cmp = dict(start="random normal (seed 42) in the centre-bond state's blocks",
                            iters={k: lz[k]["iters"] for k in lz}, energy={k: lz[k]["energy"] for k in lz},
                            seconds={k: round(lz[k]["s"], 3) for k in lz},
                            speedup_mixed_vs_fp64=round(lz["fp64"]["s"] / lz["mixed"]["s"], 3),
+                           speedup_bf16x3_vs_fp64=round(lz["fp64"]["s"] / lz["mixed_bf16x3"]["s"], 3),
                            dE_rel=abs(lz["mixed"]["energy"] - lz["fp64"]["energy"]) / abs(lz["fp64"]["energy"]),
-                           overlap=float(ov))
+                           dE_rel_bf16x3=abs(lz["mixed_bf16x3"]["energy"] - lz["fp64"]["energy"])
+                           / abs(lz["fp64"]["energy"]),
+                           overlap=float(ov),
+                           overlap_bf16x3=float(ov_bf))
         log(f"centre-bond Lanczos from a random start: fp64 {lz['fp64']['iters']} iters {lz['fp64']['s']:.2f}s, "
             f"mixed {lz['mixed']['iters']} iters {lz['mixed']['s']:.2f}s, dE_rel {lanczos_cmp['dE_rel']:.2e}, "
-            f"overlap {ov:.15f}")
+            f"overlap {ov:.15f}; split-bf16x3 {lz['mixed_bf16x3']['iters']} iters {lz['mixed_bf16x3']['s']:.2f}s, "
+            f"dE_rel {lanczos_cmp['dE_rel_bf16x3']:.2e}")
         for v in lz.values():
             v["gs"].free()
     else:
@@ -392,9 +419,14 @@ def main():
 
     e64, emx = res["fp64"]["r"]["energy"], res["mixed"]["r"]["energy"]
     de_rel = abs(emx - e64) / abs(e64)
-    upd_rel = float(max(abs(a - b) / abs(b) for a, b in zip(res["mixed"]["r"]["energies"],
-                                                           res["fp64"]["r"]["energies"])))
+
+    def upd_rel_of(k):
+        return float(max(abs(a - b) / abs(b) for a, b in zip(res[k]["r"]["energies"], res["fp64"]["r"]["energies"])))
+    upd_rel = upd_rel_of("mixed")
     dims_equal = bool((res["mixed"]["r"]["dims"] == res["fp64"]["r"]["dims"]).all())
+    ebf = res["mixed_bf16x3"]["r"]["energy"]
+    bf_cmp = dict(energy=ebf, dE_rel_final=abs(ebf - e64) / abs(e64), dE_rel_max_update=upd_rel_of("mixed_bf16x3"),
+                  kept_dims_identical=bool((res["mixed_bf16x3"]["r"]["dims"] == res["fp64"]["r"]["dims"]).all()))
     out = {
         "metric": METRIC,
         "value": round(steps["mixed"]["tflops"], 4),
@@ -421,19 +453,22 @@ def main():
         "sweep_s": {k: round(res[k]["wall"], 3) for k in res},
         "sweep_phase_s": {k: res[k]["phase"] for k in res},
         "sweep_speedup_mixed_vs_fp64": round(res["fp64"]["wall"] / res["mixed"]["wall"], 3),
+        "sweep_speedup_bf16x3_vs_fp64": round(res["fp64"]["wall"] / res["mixed_bf16x3"]["wall"], 3),
         "sweep_counters": {k: {c: round(res[k]["counters"][c], 1)
                                for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls", "hip_malloc",
                                          "svd_eigh_ms", "svd_refine_ms", "svd_refine_eigh_ms", "svd_refine_rounds",
                                          "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
-                                         "svd_max_sector_sum")} for k in res},
-        "stage_ms": {"fp64": mb["fp64"]["stage_ms"], "mixed": mb["mixed"]["stage_ms"]},
+                                         "svd_max_sector_sum", "gemm_narrow", "gemm_narrow_rowmajor_a")}
+                           for k in res},
+        "stage_ms": {k: mb[k]["stage_ms"] for k in ("fp64", "mixed", "mixed_bf16x3")},
         "stage_ms_heuristic": {k: heur[k]["stage_ms"] for k in heur},
         "autotune": tune_summary(),
+        "roofline_bf16x3": roofline_bf,
         "roofline_fp64": dict(bound="mfma", achieved=round(d64_ach, 3), peak=PEAK["fp64"], unit="TFLOP/s",
                               frac=round(d64_ach / PEAK["fp64"], 4), flops_per_launch=d64["dom_flops"],
                               ms_per_launch=d64["dom_ms"]),
         "energy": {"fp64": e64, "mixed": emx, "dE_rel_final": de_rel, "dE_rel_max_update": upd_rel,
-                   "kept_dims_identical": dims_equal},
+                   "kept_dims_identical": dims_equal, "mixed_bf16x3": bf_cmp},
         "truncation_error": {k: {"max": float(res[k]["r"]["trunc_errs"].max()),
                                  "mean": float(res[k]["r"]["trunc_errs"].mean())} for k in res},
         "centre_bond_lanczos": lanczos_cmp,

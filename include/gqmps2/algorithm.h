@@ -23,8 +23,13 @@
 namespace gqmps2 {
 
 // Precision of the Lanczos matvec contractions (extension; FP64 is the
-// reference behaviour).
-enum class LanczosPrecision { FP64 = MPS2_PREC_FP64, Mixed = MPS2_PREC_MIXED };
+// reference behaviour): Mixed runs them on the FP32 matrix cores, MixedBf16x3
+// split-BF16x3 on the BF16 matrix cores, both with FP64 residual correction.
+enum class LanczosPrecision {
+  FP64 = MPS2_PREC_FP64,
+  Mixed = MPS2_PREC_MIXED,
+  MixedBf16x3 = MPS2_PREC_MIXED_BF16X3
+};
 
 struct LanczosParams {
   LanczosParams(double err, size_t max_iter, LanczosPrecision prec = LanczosPrecision::FP64)

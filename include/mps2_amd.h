@@ -25,7 +25,11 @@
  *    Block divergence = sum(qn of OUT legs) - sum(qn of IN legs).
  *  - Precision for mps2_lanczos: MPS2_PREC_FP64 is the plain FP64 form (all
  *    contractions FP64 on FP64 MFMA); MPS2_PREC_MIXED runs the matvec
- *    contractions in FP32 on MFMA with FP64 residual correction.
+ *    contractions in FP32 on MFMA with FP64 residual correction;
+ *    MPS2_PREC_MIXED_BF16X3 is the same restarted form with the matvec
+ *    products split-BF16x3 (each FP32 operand as two BF16 parts, three BF16
+ *    products per multiply on the BF16 matrix cores, FP32 accumulation).
+ *    Any other precision code is an argument error.
  */
 #ifndef MPS2_AMD_H
 #define MPS2_AMD_H
@@ -46,6 +50,7 @@ extern "C" {
 
 #define MPS2_PREC_FP64 0
 #define MPS2_PREC_MIXED 1
+#define MPS2_PREC_MIXED_BF16X3 2
 
 #define MPS2_ORTH_LEFT 0
 #define MPS2_ORTH_RIGHT 1
@@ -107,7 +112,8 @@ int mps2_ten_div(mps2_ten t, int32_t *qn);
  * two_site_update_finite_vmps_impl.h:118,120,124,127,129,243,272,276,290-317. */
 int mps2_contract(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, mps2_ten *out);
 /* Contract with the arithmetic precision selectable (MPS2_PREC_MIXED: operands
- * rounded to FP32, FP32 MFMA with FP32 accumulation, result widened to FP64).
+ * rounded to FP32, FP32 MFMA with FP32 accumulation, result widened to FP64;
+ * MPS2_PREC_MIXED_BF16X3: the same FP32 operands multiplied split-BF16x3).
  * Same semantics as mps2_contract otherwise; used by the mixed matvec. */
 int mps2_contract_ex(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, const int32_t *axes_b, int precision,
                      mps2_ten *out);
@@ -141,8 +147,8 @@ int mps2_svd(mps2_ten t, int ldims, const int32_t *ldiv, double trunc_err, int64
 int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi, const char *where, mps2_ten *out);
 
 /* The effective Hamiltonian bound once to its operands: L, W1, W2, R and the
- * block layout of `psi`, at one precision (MPS2_PREC_MIXED makes the FP32
- * operand copies once).  It is the operator that eff_ham_mul_state_{lend,
+ * block layout of `psi`, at one precision (MPS2_PREC_MIXED and
+ * MPS2_PREC_MIXED_BF16X3 make the FP32 operand copies once).  It is the operator that eff_ham_mul_state_{lend,
  * cent,rend} (lanczos_solver_impl.h:228-256) applies at every Lanczos step
  * (:149,171), with its setup hoisted out of the loop.  FP64 operands are
  * shared with the caller's tensors (kept alive by the operator; they must not

@@ -218,11 +218,13 @@ int mps2_contract_ex(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, con
       need(axes_b, "axes_b");
     }
     std::vector<int> xa(axes_a, axes_a + nax), xb(axes_b, axes_b + nax);
-    if (precision != MPS2_PREC_MIXED) {
+    const mps2::Arith ar = mps2::arith_of(precision);
+    if (ar == mps2::kArithF64) {
       *out = wrap(mps2::contract(a->t, b->t, xa, xb));
       return;
     }
     mps2::Ctx *c = a->t->ctx;
+    mps2::SplitScope scope(c, ar == mps2::kArithBf16x3);
     auto to32 = [&](const mps2::TenP &t) {
       auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
       mps2::k_cast_d2f(c, t->n(), t->d(), r->f());
@@ -309,6 +311,7 @@ struct mps2_heff_s {
   mps2::TenP L, W1, W2, R, x, y;
   std::string where;
   mps2::Prec prec = mps2::kF64;
+  bool split = false;   // split-BF16x3 FP32 GEMMs
   double flops = 0, bytes = 0;
 };
 
@@ -329,7 +332,10 @@ int mps2_heff_create(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten 
     mps2::Ctx *c = psi->t->ctx;
     auto h = std::make_unique<mps2_heff_s>();
     h->where = w;
-    h->prec = precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64;
+    const mps2::Arith ar = mps2::arith_of(precision);
+    h->prec = mps2::storage_of(ar);
+    h->split = ar == mps2::kArithBf16x3;
+    mps2::SplitScope scope(c, h->split);
     auto cast = [&](const mps2::TenP &t) -> mps2::TenP {
       if (!t || h->prec == mps2::kF64) return t;
       auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
@@ -359,6 +365,7 @@ int mps2_heff_run(mps2_heff h, int n) {
   return guard([&] {
     need(h, "h");
     if (n < 0) throw mps2::Error(mps2::kErrArg, "heff_run: n must be >= 0");
+    mps2::SplitScope scope(h->x->ctx, h->split);
     for (int i = 0; i < n; ++i) mps2::heff_mul(h->L, h->W1, h->W2, h->R, h->x, h->where, h->y, h->prec);
   });
 }
@@ -386,8 +393,7 @@ int mps2_lanczos(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten init
     std::string w(where);
     if (w == "cent" || w == "rend") need(L, "L");
     if (w == "cent" || w == "lend") need(R, "R");
-    auto r = mps2::lanczos(opt(L), W1->t, W2->t, opt(R), init->t, w, error, max_iter,
-                           precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64);
+    auto r = mps2::lanczos(opt(L), W1->t, W2->t, opt(R), init->t, w, error, max_iter, mps2::arith_of(precision));
     if (iters) *iters = r.iters;
     if (energy) *energy = r.energy;
     *gs = wrap(r.gs);
@@ -408,7 +414,9 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     if (w == "cent" || w == "lend") need(R, "R");
     mps2::Ctx *c = psi->t->ctx;
     auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
-    mps2::Prec p = precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64;
+    const mps2::Arith ar = mps2::arith_of(precision);
+    const mps2::Prec p = mps2::storage_of(ar);
+    mps2::SplitScope scope(c, ar == mps2::kArithBf16x3);
     auto cast = [&](const mps2::TenP &t) -> mps2::TenP {
       if (!t || p == mps2::kF64) return t;
       auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);
@@ -527,7 +535,9 @@ int mps2_heff_bench_shard(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2
     if (w == "cent" || w == "lend") need(R, "R");
     mps2::Ctx *c = psi->t->ctx;
     auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
-    mps2::Prec p = precision == MPS2_PREC_MIXED ? mps2::kF32 : mps2::kF64;
+    const mps2::Arith ar = mps2::arith_of(precision);
+    const mps2::Prec p = mps2::storage_of(ar);
+    mps2::SplitScope scope(c, ar == mps2::kArithBf16x3);
     auto cast = [&](const mps2::TenP &t) -> mps2::TenP {
       if (!t || p == mps2::kF64) return t;
       auto r = mps2::ten_alloc(c, t->meta, mps2::kF32);

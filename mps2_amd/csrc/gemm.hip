@@ -483,6 +483,282 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_dma_kernel(const float 
     }
 }
 
+// ---------------------------------------------------------------- FP32 operands, split-BF16x3 products
+// FP32 operands x are split as x = hi + lo with hi = bf16(x) and lo =
+// bf16(x - hi) (round to nearest; x - hi is exact in FP32), and each product
+// a b is formed as hi_a hi_b + hi_a lo_b + lo_a hi_b on the BF16 matrix cores
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.  The representation
+// error |x - hi - lo| <= 2^-16 |x| and the dropped lo_a lo_b term leave each
+// product accurate to a few units of 2^-16, independent from product to
+// product, so a K-term sum carries about that error times its root-sum-square
+// rather than its sum: the same order as FP32 accumulation over the K depths
+// of the DMRG stages.
+//
+// Tiles BM x BN, 4 wavefronts in 2 x 2, 32-deep slabs loaded as FP32 into
+// registers (the FP32 kernel's Slab), split when stored to LDS.  LDS row r of
+// A (row m) and of B (column n): hi for k = 0..31, lo for k = 0..31, 8 bf16
+// of padding (144 bytes per row).  Lane l of a wavefront feeds the MFMA rows
+// l % 16 with k = 8 (l / 16) .. +7 of both operands (one 16-byte read per
+// fragment per half); accumulator element r of lane l is row 4 (l / 16) + r,
+// column l % 16 (measured on gfx950 with an in-bounds probe).
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+constexpr int kBf3BK = 32;
+constexpr int kBf3Row = 2 * kBf3BK + 8;   // bf16 per LDS row
+
+__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+template <int BM, int BN, bool TA, bool TB, int TAG>
+__global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float *__restrict__ A,
+                                                                     const float *__restrict__ B,
+                                                                     float *__restrict__ C,
+                                                                     const TileD *__restrict__ tiles,
+                                                                     const BlkD *__restrict__ blks,
+                                                                     const PairD *__restrict__ pairs, int ntiles,
+                                                                     int sub) {
+  using V4 = Mfma<float>::v4;
+  using SlabT = Slab<float, BM, BN, TA, TB, kBf3BK>;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int FM = WM / 16, FN = WN / 16;
+  constexpr int RS = kBf3Row, BK = kBf3BK;
+  __shared__ __align__(16) __bf16 As[2][BM * RS];
+  __shared__ __align__(16) __bf16 Bs[2][BN * RS];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int tix = blockIdx.x;
+  if (tix >= ntiles) return;
+  const TileD tile = tiles[tix];
+  if (tile.blk < 0) return;
+  const BlkD blk = blks[tile.blk];
+  const int M = blk.M, N = blk.N;
+  const int m0 = tile.m0, n0 = tile.n0;
+  if (m0 >= M || n0 >= N || blk.pb >= blk.pe) return;
+
+  // registers -> LDS, element (m, k) of A at As[m RS + k] (hi) and
+  // As[m RS + BK + k] (lo); B (k, n) at Bs[n RS + k] likewise.  The thread's
+  // (row, k) pattern is the one Slab::load used.
+  auto store = [&](const SlabT &s, __bf16 *as, __bf16 *bs) {
+    if (!TA) {
+      constexpr int RSTEP = kThreads / BK;
+      const int kk = tid % BK;
+#pragma unroll
+      for (int i = 0; i < SlabT::NA; ++i) {
+        __bf16 h, l;
+        split_bf16(s.ra[i], h, l);
+        const int m = tid / BK + i * RSTEP;
+        as[m * RS + kk] = h;
+        as[m * RS + BK + kk] = l;
+      }
+    } else {
+      constexpr int KSTEP = kThreads / BM;
+      const int mm = tid % BM;
+#pragma unroll
+      for (int i = 0; i < SlabT::NA; ++i) {
+        __bf16 h, l;
+        split_bf16(s.ra[i], h, l);
+        const int k = tid / BM + i * KSTEP;
+        as[mm * RS + k] = h;
+        as[mm * RS + BK + k] = l;
+      }
+    }
+    if (!TB) {
+      constexpr int KSTEP = kThreads / BN;
+      const int nn = tid % BN;
+#pragma unroll
+      for (int i = 0; i < SlabT::NB; ++i) {
+        __bf16 h, l;
+        split_bf16(s.rb[i], h, l);
+        const int k = tid / BN + i * KSTEP;
+        bs[nn * RS + k] = h;
+        bs[nn * RS + BK + k] = l;
+      }
+    } else {
+      constexpr int RSTEP = kThreads / BK;
+      const int kk = tid % BK;
+#pragma unroll
+      for (int i = 0; i < SlabT::NB; ++i) {
+        __bf16 h, l;
+        split_bf16(s.rb[i], h, l);
+        const int n = tid / BK + i * RSTEP;
+        bs[n * RS + kk] = h;
+        bs[n * RS + BK + kk] = l;
+      }
+    }
+  };
+
+  V4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+  SlabT slab;
+  int p = blk.pb, k0 = 0;
+  PairD pr = pairs[p];
+  slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, 0, tid);
+  store(slab, As[0], Bs[0]);
+  __syncthreads();
+  int cur = 0;
+  const int frag_row = (lane & 15) * RS + 8 * (lane >> 4);
+  while (true) {
+    int np = p, nk0 = k0 + BK;
+    PairD npr = pr;
+    if (nk0 >= pr.K) {
+      np = p + 1;
+      nk0 = 0;
+      if (np < blk.pe) npr = pairs[np];
+    }
+    const bool has_next = np < blk.pe;
+    if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
+    {
+      const __bf16 *as = As[cur] + wm * WM * RS + frag_row;
+      const __bf16 *bs = Bs[cur] + wn * WN * RS + frag_row;
+      bf16x8_t ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        ah[i] = *reinterpret_cast<const bf16x8_t *>(as + i * 16 * RS);
+        al[i] = *reinterpret_cast<const bf16x8_t *>(as + i * 16 * RS + BK);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        bh[j] = *reinterpret_cast<const bf16x8_t *>(bs + j * 16 * RS);
+        bl[j] = *reinterpret_cast<const bf16x8_t *>(bs + j * 16 * RS + BK);
+      }
+      // small cross terms first, then hi x hi; every accumulator is touched
+      // once per pass, so consecutive MFMAs are independent
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+    if (has_next) store(slab, As[cur ^ 1], Bs[cur ^ 1]);
+    __syncthreads();
+    if (!has_next) break;
+    cur ^= 1;
+    p = np;
+    k0 = nk0;
+    pr = npr;
+  }
+  float *Cb = C + blk.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int gn = n0 + wn * WN + j * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+        if (gm < M && gn < N) {
+          if (sub) Cb[(int64_t)gm * N + gn] -= acc[i][j][r];
+          else Cb[(int64_t)gm * N + gn] = acc[i][j][r];
+        }
+      }
+    }
+}
+
+template <int BM, int BN, int TAG>
+void launch_tiles_bf3(Ctx *c, int la, int lb, const float *A, const float *B, float *C, const TileD *tiles,
+                      int ntiles, const BlkD *blks, const PairD *pairs, int sub) {
+  dim3 grid(ntiles), block(kThreads);
+  if (la == 0 && lb == 0)
+    grouped_gemm_bf3_kernel<BM, BN, false, false, TAG>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else if (la == 0 && lb == 1)
+    grouped_gemm_bf3_kernel<BM, BN, false, true, TAG>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else if (la == 1 && lb == 0)
+    grouped_gemm_bf3_kernel<BM, BN, true, false, TAG>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else
+    grouped_gemm_bf3_kernel<BM, BN, true, true, TAG>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+}
+
+// ---------------------------------------------------------------- narrow outputs (vector ALU)
+// Contractions whose output blocks are at most 16 columns wide (the two-site
+// MPO stage of the matvec: K and N are an MPO bond sector times the physical
+// dimensions, a few to a few tens) do about 2N flops per element of A read,
+// so they are HBM-bound, and a matrix-core tile, which pads N to 64 or 128
+// and K to its slab depth, spends most of its time on padding.  Here one
+// thread owns one row of one output block and NC of its columns: it walks the
+// block's pairs in their fixed order, reads its row of A (coalesced across
+// the wavefront when A is stored K x M, the layout of that stage) and
+// accumulates the NC outputs in registers with fused multiply-adds.  B's
+// values are the same for the whole wavefront.  Fixed order, so the result
+// is bit-reproducible.  A tile is (block, 256 rows from m0, NC columns from n0).
+template <typename T, bool TA, bool TB, int NC, int TAG>
+__global__ __launch_bounds__(256) void narrow_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
+                                                          T *__restrict__ C, const TileD *__restrict__ tiles,
+                                                          const BlkD *__restrict__ blks,
+                                                          const PairD *__restrict__ pairs, int ntiles, int sub) {
+  const int tix = blockIdx.x;
+  if (tix >= ntiles) return;
+  const TileD tile = tiles[tix];
+  if (tile.blk < 0) return;
+  const BlkD blk = blks[tile.blk];
+  const int M = blk.M, N = blk.N;
+  const int m = tile.m0 + (int)threadIdx.x, n0 = tile.n0;
+  if (m >= M || n0 >= N) return;
+  const int nn = min(NC, N - n0);
+  T acc[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) acc[j] = T(0);
+  for (int p = blk.pb; p < blk.pe; ++p) {
+    const PairD pr = pairs[p];
+    const int K = pr.K;
+    const T *Ab = A + pr.a_off;
+    const T *Bb = B + pr.b_off;
+    for (int k = 0; k < K; ++k) {
+      const T a = TA ? Ab[(int64_t)k * M + m] : Ab[(int64_t)m * K + k];
+#pragma unroll
+      for (int j = 0; j < NC; ++j)
+        if (j < nn) {
+          const T b = TB ? Bb[(int64_t)(n0 + j) * K + k] : Bb[(int64_t)k * N + n0 + j];
+          acc[j] = fma(a, b, acc[j]);
+        }
+    }
+  }
+  T *Cr = C + blk.c_off + (int64_t)m * N + n0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j)
+    if (j < nn) Cr[j] = sub ? Cr[j] - acc[j] : acc[j];
+}
+
+template <typename T, int NC, int TAG>
+void launch_narrow_nc(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                      const BlkD *blks, const PairD *pairs, int sub) {
+  dim3 grid(ntiles), block(256);
+  if (la == 0 && lb == 0)
+    narrow_gemm_kernel<T, false, false, NC, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else if (la == 0 && lb == 1)
+    narrow_gemm_kernel<T, false, true, NC, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else if (la == 1 && lb == 0)
+    narrow_gemm_kernel<T, true, false, NC, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+  else
+    narrow_gemm_kernel<T, true, true, NC, TAG><<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+}
+
+template <typename T, int TAG>
+void launch_narrow_tag(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                       const BlkD *blks, const PairD *pairs, int nc, int sub) {
+  if (nc == 4) launch_narrow_nc<T, 4, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  else if (nc == 8) launch_narrow_nc<T, 8, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+  else launch_narrow_nc<T, 16, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+}
+
 template <typename T, int BM, int BN, int TAG, int BK, int NBUF>
 void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                     const BlkD *blks, const PairD *pairs, int sub) {
@@ -517,6 +793,12 @@ int forced_variant() {
 template <typename T, int BM, int BN, int TAG>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                   const BlkD *blks, const PairD *pairs, int sub, int variant) {
+  if constexpr (sizeof(T) == 4) {
+    if (c->f32_split) {
+      launch_tiles_bf3<BM, BN, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+      return;
+    }
+  }
   if constexpr (sizeof(T) == 4 && BM == 128) {
     const int f = forced_variant();
     const int v = f >= 0 ? f : variant;
@@ -608,6 +890,29 @@ void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, c
   }
   c->gemm_launches++;
 }
+
+template <typename T>
+void launch_narrow_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                        const BlkD *blks, const PairD *pairs, int nc, int sub, int tag) {
+  if (ntiles <= 0) return;
+  if (tag == 1) launch_narrow_tag<T, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, nc, sub);
+  else if (tag == 2) launch_narrow_tag<T, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, nc, sub);
+  else launch_narrow_tag<T, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, nc, sub);
+  c->gemm_launches++;
+}
+
+std::vector<TileD> narrow_tiles(const std::vector<BlkD> &blks, int nc) {
+  std::vector<TileD> t;
+  for (size_t b = 0; b < blks.size(); ++b)
+    for (int m0 = 0; m0 < blks[b].M; m0 += kNarrowRows)
+      for (int n0 = 0; n0 < blks[b].N; n0 += nc) t.push_back(TileD{(int32_t)b, m0, n0, 0});
+  return t;
+}
+
+template void launch_narrow_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *, int,
+                                         const BlkD *, const PairD *, int, int, int);
+template void launch_narrow_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
+                                        const BlkD *, const PairD *, int, int, int);
 
 template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *,
                                           int, const BlkD *, const PairD *, int, int, int, int, int);

@@ -225,7 +225,8 @@ double small_sym_lowest(int n, std::vector<double> A, std::vector<double> &vec) 
 // Mixed-precision form: restarted Lanczos with FP64 residual correction.
 // Each restart k computes, in FP64, h = H v_k, E_k = <v_k|h> and the residual
 // r = h - E_k v_k.  The Krylov space {v_k, r, H~ r, ...} is then extended with
-// the FP32-MFMA operator H~ (FP32 operands, FP32 accumulation), every new
+// the low-precision operator H~ (FP32 operands and accumulation; products on
+// the FP32 matrix cores, or split-BF16x3 on the BF16 ones), every new
 // vector orthogonalised twice in FP64 against the whole basis.  The projected
 // matrix uses the exact FP64 row of v_k (E_k, |r|, 0, ...) and FP64 dot
 // products of the FP32 images, so the error left by H~ enters the Ritz vector
@@ -237,7 +238,7 @@ double small_sym_lowest(int n, std::vector<double> A, std::vector<double> &vec) 
 // its FP64 Rayleigh quotient).  A converged state thus costs one FP64 and one
 // FP32 matvec, where the plain form spends two FP64 matvecs.
 LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
-                         const std::string &where, double error, int64_t max_iter) {
+                         const std::string &where, double error, int64_t max_iter, bool split) {
   Ctx *c = init->ctx;
   MPS2_REQUIRE(max_iter >= 2, "Lanczos: max_iterations must be >= 2");
   int64_t eff_dim = eff_dim_of(L, W1, W2, R, where);
@@ -291,7 +292,10 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
     for (int j = 1; j <= kInnerMax; ++j) {
       // w = H~ q_j
       k_cast_d2f(c, n, Q[j]->d(), x32->f());
-      heff_mul(L32, W132, W232, R32, x32, where, y32, kF32);
+      {
+        SplitScope scope(c, split);
+        heff_mul(L32, W132, W232, R32, x32, where, y32, kF32);
+      }
       res.matvecs_lo++;
       auto w = ten_alloc(c, full);
       k_cast_f2d(c, n, y32->f(), w->d());
@@ -361,10 +365,11 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
 }
 
 LanczosOut lanczos(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
-                   const std::string &where, double error, int64_t max_iter, Prec prec) {
+                   const std::string &where, double error, int64_t max_iter, Arith arith) {
   Ctx *c = init->ctx;
-  LanczosOut r = prec == kF64 ? lanczos_plain(L, W1, W2, R, init, where, error, max_iter)
-                              : lanczos_mixed(L, W1, W2, R, init, where, error, max_iter);
+  LanczosOut r = arith == kArithF64 ? lanczos_plain(L, W1, W2, R, init, where, error, max_iter)
+                                    : lanczos_mixed(L, W1, W2, R, init, where, error, max_iter,
+                                                    arith == kArithBf16x3);
   c->count("lanczos_calls");
   c->count("matvec_fp64", (double)r.matvecs_hi);
   c->count("matvec_fp32", (double)r.matvecs_lo);

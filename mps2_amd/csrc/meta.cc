@@ -215,9 +215,25 @@ std::shared_ptr<ContractPlan> plan_contract(const MetaP &a, const MetaP &b, cons
     b_lead &= bxn[i] == i;
     b_trail &= bxn[i] == nb - kn + i;
   }
+  // Widest output block (the product of B's free dimensions over B's blocks):
+  // a narrow contraction runs on the vector-ALU kernel, one thread per output
+  // row, which reads A coalesced when A is stored K x M, so a permuted A is
+  // then laid out with the contracted legs leading.
+  int64_t max_n = 0;
+  for (int32_t j = 0; j < (int32_t)b->coords.size(); ++j) {
+    int64_t n = 1;
+    for (int i : bfree) n *= b->dim_of(j, i);
+    max_n = std::max(max_n, n);
+  }
+  const bool narrow = max_n <= kNarrowMaxN;
   if (a_trail) P->la = kNormal;
   else if (a_lead) P->la = kTrans;
-  else {
+  else if (narrow) {
+    P->permute_a = true;
+    P->la = kTrans;
+    P->aperm = ax;
+    P->aperm.insert(P->aperm.end(), afree.begin(), afree.end());
+  } else {
     P->permute_a = true;
     P->la = kNormal;
     P->aperm = afree;

@@ -143,6 +143,10 @@ struct BlkH {
   int32_t pair_begin, pair_end;
 };
 
+// Contractions whose output blocks are all at most kNarrowMaxN columns wide
+// run on the vector ALUs (narrow_gemm_kernel), the others on the matrix cores.
+constexpr int kNarrowMaxN = 16;
+
 struct ContractPlan {
   MetaP a, b, out;
   std::vector<int> ax, bx;

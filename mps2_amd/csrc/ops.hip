@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -118,6 +119,19 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
     count = (int)t.size();
     return upload_bytes(c, t.data(), t.size() * sizeof(TileD));
   };
+  int max_n = 0;
+  for (auto &b : blks) max_n = std::max(max_n, (int)b.N);
+  // MPS2_NARROW=0 keeps narrow contractions on the matrix-core kernels.
+  static const bool narrow_on = [] {
+    const char *e = std::getenv("MPS2_NARROW");
+    return !(e && std::string(e) == "0");
+  }();
+  if (narrow_on && max_n <= kNarrowMaxN && !hp->blks.empty()) {
+    dp->narrow_nc = max_n <= 4 ? 4 : (max_n <= 8 ? 8 : 16);
+    std::vector<TileD> t = narrow_tiles(blks, dp->narrow_nc);
+    dp->ntiles_narrow = (int)t.size();
+    dp->tiles_narrow = upload_bytes(c, t.data(), t.size() * sizeof(TileD));
+  }
   dp->tiles64 = upload_tiles(64, dp->ntiles64);
   dp->tiles128 = upload_tiles(128, dp->ntiles128);
   if (ts == 128) dp->tiles = dp->tiles128, dp->ntiles = dp->ntiles128;
@@ -236,6 +250,20 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   const int tag = (c->tag_stage >= 0 && c->stage_idx == c->tag_stage) ? (c->shard_g > 1 ? 2 : 1) : 0;
   c->stage_idx++;
   const bool fp64 = sizeof(T) == sizeof(double);
+  if (dp.narrow_nc && tag != 2) {
+    c->count(dp.hp->la == kTrans ? "gemm_narrow" : "gemm_narrow_rowmajor_a");
+    launch_narrow_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, static_cast<const TileD *>(dp.tiles_narrow->p),
+                          dp.ntiles_narrow, static_cast<const BlkD *>(dp.blks->p),
+                          static_cast<const PairD *>(dp.pairs->p), dp.narrow_nc, 0, tag);
+    if (c->time_gemms) {
+      MPS2_HIP(hipEventRecord(tm.stop, c->stream));
+      tm.flops = dp.hp->flops;
+      tm.bytes = (double)sizeof(T) * ((double)a->n() + (double)b->n() + (double)dp.hp->out->total);
+      c->gemm_timings.push_back(tm);
+    }
+    c->gemm_flops += dp.hp->flops;
+    return;
+  }
   PlanTune::Probe probe;
   const GemmChoice ch = tag == 2 ? GemmChoice{dp.tile_m, 0} : tune_select(c, dp, fp64, &probe);
   const TileD *tiles = static_cast<const TileD *>((ch.tile == 128 ? dp.tiles128 : dp.tiles64)->p);
@@ -262,6 +290,12 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
 }
 
 }  // namespace
+
+Arith arith_of(int precision) {
+  if (precision < kArithF64 || precision > kArithBf16x3)
+    throw Error(kErrArg, "precision must be MPS2_PREC_FP64, MPS2_PREC_MIXED or MPS2_PREC_MIXED_BF16X3");
+  return static_cast<Arith>(precision);
+}
 
 void plan_cache_clear(Ctx *c) {
   std::vector<PlanEntry> dropped;   // released after the lock (their buffers go back to c)

@@ -103,6 +103,10 @@ struct Ctx {
   bool closing = false;
   // Grouped-GEMM autotuning (tune.hip): 1 = tune large plans, 0 = heuristic.
   int tune_mode = 1;
+  // Arithmetic of FP32 grouped GEMMs: 0 = FP32 matrix cores, 1 = split-BF16x3
+  // (operands split into two BF16 parts, three BF16 products per multiply,
+  // FP32 accumulation).  Set for the duration of a call by SplitScope.
+  int f32_split = 0;
 
   explicit Ctx(int dev);
   ~Ctx();
@@ -137,6 +141,24 @@ TenP ten_upload(Ctx *c, MetaP m, const double *host);
 void ten_download(const Ten &t, double *host);
 TenP ten_copy(const Ten &t);
 TenP ten_dag(const TenP &t);   // real tensors: flip directions, share data
+
+// Arithmetic of a matvec, contraction or Lanczos call (the C ABI's precision
+// argument): FP64, FP32 on the FP32 matrix cores, or split-BF16x3 on the BF16
+// matrix cores.  The last two keep their working copies in FP32.
+enum Arith : int { kArithF64 = 0, kArithF32 = 1, kArithBf16x3 = 2 };
+inline Prec storage_of(Arith a) { return a == kArithF64 ? kF64 : kF32; }
+// The C ABI's precision code (0, 1, 2) as an Arith; anything else is an
+// argument error.
+Arith arith_of(int precision);
+// Sets ctx->f32_split (split-BF16x3 FP32 GEMMs) for the lifetime of the scope.
+struct SplitScope {
+  Ctx *c;
+  int old;
+  SplitScope(Ctx *c_, bool on) : c(c_), old(c_->f32_split) { c->f32_split = on ? 1 : 0; }
+  ~SplitScope() { c->f32_split = old; }
+  SplitScope(const SplitScope &) = delete;
+  SplitScope &operator=(const SplitScope &) = delete;
+};
 
 // ------------------------------------------------------------------ kernels (kernels.hip)
 void k_fill(Ctx *c, double *p, int64_t n, double v);
@@ -202,6 +224,14 @@ template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0,
                          int variant = 0);
+// Narrow contractions (kNarrowMaxN, meta.h) run on the vector ALUs
+// (narrow_gemm_kernel, gemm.hip): one thread per output row and nc (4, 8 or
+// 16) columns, kNarrowRows rows per workgroup.
+constexpr int kNarrowRows = 256;
+template <typename T>
+void launch_narrow_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                        const BlkD *blks, const PairD *pairs, int nc, int sub = 0, int tag = 0);
+std::vector<TileD> narrow_tiles(const std::vector<BlkD> &blks, int nc);
 // Tile list of one grouped-GEMM launch (tile edge ts) ordered for the XCDs.
 // kc[i] is block i's summed K extent (its per-tile cost).  See gemm.hip.
 std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts);
@@ -243,7 +273,12 @@ struct DevPlan {
   int tile_m = 64, tile_n = 64;   // heuristic FP32 tile (tiles holds its list)
   BufP tiles64, tiles128;         // the output tiled 64x64 and 128x128
   int ntiles64 = 0, ntiles128 = 0;
-  mutable PlanTune tune[2];       // [0] FP64, [1] FP32
+  // narrow plans (every output block at most kNarrowMaxN wide): the vector-ALU
+  // kernel's column chunk (0 = matrix-core kernels) and its tile list
+  int narrow_nc = 0;
+  BufP tiles_narrow;
+  int ntiles_narrow = 0;
+  mutable PlanTune tune[3];       // [0] FP64, [1] FP32, [2] split-BF16x3
   BufP perm_a, perm_b, perm_a_start, perm_b_start;   // permutation descriptors
   int nperm_a = 0, nperm_b = 0;
   // tiles of one owner's output blocks, by (owners, rank): tiles, count, flops
@@ -317,6 +352,6 @@ struct LanczosOut {
   int64_t matvecs_lo = 0, matvecs_hi = 0;   // low-precision / FP64 matvecs
 };
 LanczosOut lanczos(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
-                   const std::string &where, double error, int64_t max_iter, Prec prec);
+                   const std::string &where, double error, int64_t max_iter, Arith arith);
 
 }  // namespace mps2

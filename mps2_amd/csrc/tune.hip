@@ -4,7 +4,9 @@
 // (same slab depth and MFMA step order), so the choice changes the time of a
 // launch, never its result: for FP32, 128x128 tiles with register-staged
 // slabs, 128x128 tiles with direct-to-LDS slabs (row-major operands only) and
-// 64x64 tiles.  FP64 launches keep 64x64 tiles.
+// 64x64 tiles; for split-BF16x3, 128x128 and 64x64 tiles.  FP64 launches keep
+// 64x64 tiles.  Decisions are kept per arithmetic class (FP64, FP32,
+// split-BF16x3).
 // The first launches of a large plan run each candidate twice in turn, timed
 // with HIP events on the context stream (no host sync: the events are read
 // once the last one has completed, until then the heuristic kernel runs).
@@ -45,7 +47,9 @@ struct CacheEntry {
 };
 
 std::mutex g_tune_mu;
-std::map<std::pair<uint64_t, int>, CacheEntry> g_tune;   // (shape key, fp64) -> choice
+// (shape key, class code) -> choice; class codes as in the cache file:
+// 1 = FP64, 0 = FP32, 2 = split-BF16x3.
+std::map<std::pair<uint64_t, int>, CacheEntry> g_tune;
 bool g_tune_loaded = false;
 
 std::string cache_path() {
@@ -67,19 +71,19 @@ void load_cache_locked() {
   while (std::getline(f, line)) {
     std::istringstream s(line);
     std::string key;
-    int fp64 = 0, tile = 0, variant = 0;
+    int code = 0, tile = 0, variant = 0;
     double best = 0, heur = 0;
-    if (!(s >> key >> fp64 >> tile >> variant >> best >> heur)) continue;
-    if ((tile != 64 && tile != 128) || (variant != 0 && variant != 4)) continue;
+    if (!(s >> key >> code >> tile >> variant >> best >> heur)) continue;
+    if ((tile != 64 && tile != 128) || (variant != 0 && variant != 4) || code < 0 || code > 2) continue;
     CacheEntry ce;
     ce.choice = GemmChoice{tile, variant};
     ce.best_ms = best;
     ce.heuristic_ms = heur;
-    g_tune[{std::strtoull(key.c_str(), nullptr, 16), fp64}] = ce;
+    g_tune[{std::strtoull(key.c_str(), nullptr, 16), code}] = ce;
   }
 }
 
-void append_cache(uint64_t key, int fp64, const CacheEntry &ce) {
+void append_cache(uint64_t key, int code, const CacheEntry &ce) {
   const std::string p = cache_path();
   if (p.empty()) return;
   const auto slash = p.find_last_of('/');
@@ -92,7 +96,7 @@ void append_cache(uint64_t key, int fp64, const CacheEntry &ce) {
   std::ofstream f(p, std::ios::app);
   if (!f) return;
   char buf[160];
-  std::snprintf(buf, sizeof buf, "%016llx %d %d %d %.6f %.6f\n", (unsigned long long)key, fp64, ce.choice.tile,
+  std::snprintf(buf, sizeof buf, "%016llx %d %d %d %.6f %.6f\n", (unsigned long long)key, code, ce.choice.tile,
                 ce.choice.variant, ce.best_ms, ce.heuristic_ms);
   f << buf;
 }
@@ -109,7 +113,7 @@ bool tuning_enabled_env() {
 // whose sector dimensions differ by a few percent, share a key.
 int64_t lbin(int64_t x) { return x <= 0 ? -1 : (int64_t)std::lround(4.0 * std::log2((double)x)); }
 
-uint64_t shape_key(const ContractPlan &hp, bool fp64) {
+uint64_t shape_key(const ContractPlan &hp, int code) {
   std::vector<std::array<int64_t, 3>> s;
   s.reserve(hp.blks.size());
   for (const BlkH &b : hp.blks) {
@@ -127,7 +131,7 @@ uint64_t shape_key(const ContractPlan &hp, bool fp64) {
       h *= 1099511628211ull;
     }
   };
-  mix(fp64 ? 1 : 2);
+  mix(code == 1 ? 1 : code == 0 ? 2 : 3);
   mix((uint64_t)hp.la);
   mix((uint64_t)hp.lb);
   mix(s.size());
@@ -135,6 +139,11 @@ uint64_t shape_key(const ContractPlan &hp, bool fp64) {
     for (int64_t v : x) mix((uint64_t)v);
   return h;
 }
+
+// Arithmetic class of an FP64 / FP32 launch on context c: index into
+// DevPlan::tune (0 FP64, 1 FP32, 2 split-BF16x3) and the cache file's code.
+int tune_index(const Ctx *c, bool fp64) { return fp64 ? 0 : (c->f32_split ? 2 : 1); }
+int class_code(int idx) { return idx == 0 ? 1 : idx == 1 ? 0 : 2; }
 
 }  // namespace
 
@@ -145,10 +154,14 @@ PlanTune::~PlanTune() {
   }
 }
 
-std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64) {
+std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split) {
   std::vector<GemmChoice> c;
   const GemmChoice h = fp64 ? GemmChoice{64, 0} : GemmChoice{dp.tile_m, 0};
   c.push_back(h);
+  if (split) {
+    c.push_back(GemmChoice{h.tile == 128 ? 64 : 128, 0});
+    return c;
+  }
   // FP64 keeps its 64x64 tiles: 128x128 tiles allow one workgroup per compute
   // unit and measured 19% slower on the D=4096 matvec and slower in every
   // probe of a sweep, so timing them only cost time.
@@ -161,18 +174,19 @@ std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64) {
 }
 
 GemmChoice tune_select(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe) {
-  PlanTune &t = dp.tune[fp64 ? 0 : 1];
+  const int idx = tune_index(c, fp64), code = class_code(idx);
+  PlanTune &t = dp.tune[idx];
   probe->active = false;
-  if (t.cand.empty()) t.cand = tune_candidates(dp, fp64);
+  if (t.cand.empty()) t.cand = tune_candidates(dp, fp64, idx == 2);
   if (!tuning_enabled_env() || !c->tune_mode) return t.cand[0];
   if (t.decided >= 0) return t.cand[t.decided];
   if (dp.hp->flops < kTuneMinFlops || t.cand.size() < 2) return t.cand[0];
   if (!t.keyed) {
-    t.key = shape_key(*dp.hp, fp64);
+    t.key = shape_key(*dp.hp, code);
     t.keyed = true;
     std::lock_guard<std::mutex> lock(g_tune_mu);
     load_cache_locked();
-    auto it = g_tune.find({t.key, fp64 ? 1 : 0});
+    auto it = g_tune.find({t.key, code});
     if (it != g_tune.end())
       for (size_t i = 0; i < t.cand.size(); ++i)
         if (t.cand[i].tile == it->second.choice.tile && t.cand[i].variant == it->second.choice.variant) {
@@ -217,15 +231,15 @@ GemmChoice tune_select(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *pr
   ce.heuristic_ms = best[0];
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    g_tune[{t.key, fp64 ? 1 : 0}] = ce;
-    append_cache(t.key, fp64 ? 1 : 0, ce);
+    g_tune[{t.key, code}] = ce;
+    append_cache(t.key, code, ce);
   }
   return t.cand[w];
 }
 
 void tune_probe_begin(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe) {
   if (!probe->active) return;
-  PlanTune &t = dp.tune[fp64 ? 0 : 1];
+  PlanTune &t = dp.tune[tune_index(c, fp64)];
   hipEvent_t a, b;
   MPS2_HIP(hipEventCreate(&a));
   MPS2_HIP(hipEventCreate(&b));
@@ -236,7 +250,7 @@ void tune_probe_begin(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *pro
 
 void tune_probe_end(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *probe) {
   if (!probe->active) return;
-  MPS2_HIP(hipEventRecord(dp.tune[fp64 ? 0 : 1].ev.back().second, c->stream));
+  MPS2_HIP(hipEventRecord(dp.tune[tune_index(c, fp64)].ev.back().second, c->stream));
 }
 
 }  // namespace mps2

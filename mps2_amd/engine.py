@@ -16,7 +16,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmps2_amd.so")
 _lib = None
 
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_BF16X3 = 0, 1, 2
 ORTH_LEFT, ORTH_RIGHT = 0, 1
 
 _vp = C.c_void_p
@@ -277,7 +277,7 @@ COUNTERS = ("svd_calls", "svd_ms", "svd_eigh_ms", "svd_sectors", "svd_max_sector
             "matvec_fp64", "matvec_fp32", "hip_malloc", "alloc_trim", "svd_refine_rounds", "svd_refine_sectors",
             "svd_refine_ms", "svd_refine_vec_ms", "svd_refine_eigh_ms", "tune_probes", "tune_decisions",
             "tune_changed", "tune_cache_hits", "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
-            "eigh_n3_sum_g")
+            "eigh_n3_sum_g", "gemm_narrow", "gemm_narrow_rowmajor_a")
 
 
 def ctx_counters(ctx_handle, names=COUNTERS):

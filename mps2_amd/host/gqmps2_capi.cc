@@ -393,8 +393,9 @@ struct Model : ModelBase {
     require_mps();
     require_mpo();
     if (sweeps < 0 || dmin < 1 || dmax < dmin) throw std::invalid_argument("dmrg: need sweeps >= 0 and 1 <= dmin <= dmax");
-    gqmps2::LanczosParams lp(lerr, (size_t)lmax,
-                             prec == MPS2_PREC_MIXED ? gqmps2::LanczosPrecision::Mixed : gqmps2::LanczosPrecision::FP64);
+    if (prec != MPS2_PREC_FP64 && prec != MPS2_PREC_MIXED && prec != MPS2_PREC_MIXED_BF16X3)
+      throw std::invalid_argument("dmrg: precision must be MPS2_PREC_FP64, MPS2_PREC_MIXED or MPS2_PREC_MIXED_BF16X3");
+    gqmps2::LanczosParams lp(lerr, (size_t)lmax, static_cast<gqmps2::LanczosPrecision>(prec));
     double e0 = 0;
     int64_t k = 0;
     phase_times.clear();

@@ -73,16 +73,18 @@ def test_heisenberg_l16_config1_matches_ed():
     assert abs(r["energy"] - ed["heisenberg_obc_L16_sz0"]) < 1e-8
 
 
-def test_mixed_sweep_matches_fp64():
-    """Mixed-precision sweeps against FP64 sweeps from one checkpoint:
-    every update's energy within 1e-9 relative, final within 1e-10."""
+@pytest.mark.parametrize("precision", [1, 2])
+def test_mixed_sweep_matches_fp64(precision):
+    """Mixed-precision sweeps (1: FP32-MFMA matvec, 2: split-BF16x3) against
+    FP64 sweeps from one checkpoint: every update's energy within 1e-9
+    relative, final within 1e-10."""
     m = HM.heisenberg_chain(20)
     m.init_mps([[i % 2 for i in range(20)], [(i + 1) % 2 for i in range(20)]], enlarged_dim=8, seed=3)
     m.dmrg(2, 1, 32, 0.0, 1e-10, 200)
     m.checkpoint()
     g64 = m.dmrg(2, 1, 48, 0.0, 1e-11, 200)
     m.restore()
-    gmx = m.dmrg(2, 1, 48, 0.0, 1e-11, 200, precision=1)
+    gmx = m.dmrg(2, 1, 48, 0.0, 1e-11, 200, precision=precision)
     rel = np.abs(gmx["energies"] - g64["energies"]) / np.abs(g64["energies"])
     assert rel.max() < 1e-9, rel.max()
     assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])

@@ -364,11 +364,24 @@ def test_linear_combine_and_normalize(ctx):
     assert abs(E.dot(out, out) - 1) < 1e-14
 
 
-@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED])
+def low_prec_bound(prec, K):
+    """Elementwise error bound of a low-precision product, as a multiple of
+    sum_k |a_k||b_k|.  FP32 MFMA: 1e-5 (operands rounded to FP32, FP32
+    accumulation).  Split-BF16x3: x = hi + lo with hi = bf16(x), lo =
+    bf16(x - hi) leaves |x - hi - lo| <= 2^-16 |x|, and dropping lo_a lo_b
+    (<= 2^-16 |ab|) gives at most 3 * 2^-16 per product (plus 2^-23 for the
+    FP32 rounding of the operands), then at most (K + 2) 2^-24 from FP32
+    accumulation."""
+    if prec == E.PREC_MIXED:
+        return 1e-5
+    return 3.1 * 2.0 ** -16 + (K + 2) * 2.0 ** -24
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED, E.PREC_MIXED_BF16X3])
 def test_grouped_gemm_shapes_vs_numpy(ctx, prec):
     """Single-block contractions over edge shapes and all four operand layouts
-    against numpy (FP64: 1e-12 relative; FP32 MFMA: 1e-5 relative to the
-    magnitude bound sum_k |a||b|)."""
+    against numpy (FP64: 1e-12 relative; FP32 MFMA and split-BF16x3: the
+    elementwise bound of low_prec_bound relative to sum_k |a||b|)."""
     from mps2_amd.flat import Leg
     rng = np.random.default_rng(5)
 
@@ -394,23 +407,24 @@ def test_grouped_gemm_shapes_vs_numpy(ctx, prec):
                 assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (M, K, N, case)
             else:
                 bound = np.abs(a) @ np.abs(b)
-                assert (np.abs(got - ref) <= 1e-5 * bound + 1e-30).all(), (M, K, N, case)
+                assert (np.abs(got - ref) <= low_prec_bound(prec, K) * bound + 1e-30).all(), (M, K, N, case)
 
 
-def test_lanczos_mixed_matches_fp64(ctx):
-    """FP32-MFMA matvec + FP64 residual restarts reach the FP64 ground state:
-    |dE|/|E| <= 1e-10 and state overlap >= 1 - 1e-9."""
+@pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_MIXED_BF16X3])
+def test_lanczos_mixed_matches_fp64(ctx, prec):
+    """FP32-MFMA (or split-BF16x3) matvec + FP64 residual restarts reach the
+    FP64 ground state: |dE|/|E| <= 1e-10 and state overlap >= 1 - 1e-9."""
     rng = np.random.default_rng(77)
     L, W1, W2, R, psi = H.hermitian_cent_problem(rng, dl=(9, 14, 7), dr=(8, 15, 6))
     d = [ctx.put(x) for x in (L, W1, W2, R, psi)]
     _, e64, g64 = E.lanczos(*d, "cent", 1e-12, 200, E.PREC_FP64)
-    _, emx, gmx = E.lanczos(*d, "cent", 1e-12, 200, E.PREC_MIXED)
+    _, emx, gmx = E.lanczos(*d, "cent", 1e-12, 200, prec)
     assert abs(emx - e64) <= 1e-10 * abs(e64)
     ov = abs(float((g64.get().dense() * gmx.get().dense()).sum()))
     assert ov >= 1 - 1e-9
 
 
-@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED])
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED, E.PREC_MIXED_BF16X3])
 def test_long_k_walks_over_several_pairs(ctx, prec):
     """Output blocks fed by long K walks over two block pairs (K = 800 and 750,
     two contracted legs, 97 K slabs per tile): values against numpy for
@@ -448,7 +462,7 @@ def test_long_k_walks_over_several_pairs(ctx, prec):
             if prec == E.PREC_FP64:
                 assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (M, N, case)
             else:
-                assert (np.abs(g - ref) <= 1e-5 * bound + 1e-30).all(), (M, N, case)
+                assert (np.abs(g - ref) <= low_prec_bound(prec, 1550) * bound + 1e-30).all(), (M, N, case)
 
 
 def test_u1xu1_contract_and_svd_parity(ctx):
@@ -498,12 +512,13 @@ def test_tensor_outlives_its_context():
     c2.close()
 
 
-def test_fp32_large_tile_launch_vs_numpy(ctx):
+@pytest.mark.parametrize("prec", [E.PREC_MIXED, E.PREC_MIXED_BF16X3])
+def test_fp32_large_tile_launch_vs_numpy(ctx, prec):
     """A contraction large enough for the FP32 128x128 tile launch (16 output
     blocks of 500..512 square, at least one tile per compute unit), with
     one to two block pairs per output block and K extents that are not multiples of
-    the slab depth (5 x 3 = 15, 6 x 5 = 30, ...): every block against the oracle to the FP32
-    bound 1e-5 sum_k |a||b|, and bit-identical on a second run."""
+    the slab depth (5 x 3 = 15, 6 x 5 = 30, ...): every block against the oracle to the
+    low-precision bound (low_prec_bound) of sum_k |a||b|, and bit-identical on a second run."""
     from mps2_amd.flat import Leg
     rng = np.random.default_rng(17)
     qs = list(range(16))
@@ -514,8 +529,8 @@ def test_fp32_large_tile_launch_vs_numpy(ctx):
     A = BTensor.random([m, k1, k2], (0,), rng)
     B = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
     dA, dB = ctx.put(A), ctx.put(B)
-    got = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
-    again = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
+    got = E.contract_ex(dA, dB, [[1, 2], [0, 1]], prec).get()
+    again = E.contract_ex(dA, dB, [[1, 2], [0, 1]], prec).get()
     ref = O.contract(A, B, [[1, 2], [0, 1]])
     assert sorted(got.blocks) == sorted(ref.blocks) and len(ref.blocks) == 16
     absA = BTensor(A.legs, A.qw, {c: np.abs(b) for c, b in A.blocks.items()})
@@ -523,7 +538,7 @@ def test_fp32_large_tile_launch_vs_numpy(ctx):
     bound = O.contract(absA, absB, [[1, 2], [0, 1]])
     for c in ref.blocks:
         assert np.array_equal(got.blocks[c], again.blocks[c]), c
-        assert (np.abs(got.blocks[c] - ref.blocks[c]) <= 1e-5 * bound.blocks[c] + 1e-30).all(), c
+        assert (np.abs(got.blocks[c] - ref.blocks[c]) <= low_prec_bound(prec, 60) * bound.blocks[c] + 1e-30).all(), c
 
 
 def test_autotuned_launches_are_bit_identical():
@@ -544,7 +559,7 @@ def test_autotuned_launches_are_bit_identical():
     c = E.Context(0)
     try:
         dA, dB = c.put(A), c.put(B)
-        for prec in (E.PREC_MIXED,):
+        for prec in (E.PREC_MIXED, E.PREC_MIXED_BF16X3):
             E.check(E.lib().mps2_ctx_set_tuning(c.h, 0))
             ref = E.contract_ex(dA, dB, [[1, 2], [0, 1]], prec).get()
             E.check(E.lib().mps2_ctx_set_tuning(c.h, 1))
@@ -558,3 +573,89 @@ def test_autotuned_launches_are_bit_identical():
         assert E.lib().mps2_ctx_set_tuning(c.h, 7) != 0
     finally:
         c.close()
+
+
+def test_bf16x3_error_is_fp32_class(ctx):
+    """Split-BF16x3 against FP32 MFMA on one large grouped launch (8 output
+    blocks of about 500 x 500, K = 330): the relative Frobenius error against
+    the FP64 product stays below 1e-5 (each product carries at most 3 * 2^-16
+    and about 5e-6 root-mean-square, independent from product to product; a
+    dropped cross term hi*lo alone would leave about 2e-3), FP32 MFMA below
+    1e-6, and the kernel is deterministic."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(29)
+    qs = list(range(8))
+    m = Leg(IN, [((q,), 490 + (q * 7) % 13) for q in qs])
+    n = Leg(OUT, [((q,), 500 - (q * 5) % 11) for q in qs])
+    k1 = Leg(OUT, [((q,), 60 + 5 * (q % 3)) for q in range(0, 8, 2)])
+    k2 = Leg(OUT, [((q,), 20 + 3 * q) for q in range(2)])
+    A = BTensor.random([m, k1, k2], (0,), rng)
+    B = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
+    dA, dB = ctx.put(A), ctx.put(B)
+    ref = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_FP64).get()
+    f32 = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
+    bf3 = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED_BF16X3).get()
+    bf3b = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED_BF16X3).get()
+    num_f, num_b, den = 0.0, 0.0, 0.0
+    for c, r in ref.blocks.items():
+        assert np.array_equal(bf3.blocks[c], bf3b.blocks[c]), c
+        num_f += float(((f32.blocks[c] - r) ** 2).sum())
+        num_b += float(((bf3.blocks[c] - r) ** 2).sum())
+        den += float((r ** 2).sum())
+    rel_f, rel_b = np.sqrt(num_f / den), np.sqrt(num_b / den)
+    print(f"relative Frobenius error: fp32 {rel_f:.3e}, split-bf16x3 {rel_b:.3e}")
+    assert rel_b < 1e-5, (rel_f, rel_b)
+    assert rel_f < 1e-6, (rel_f, rel_b)
+
+
+def test_precision_code_checked(ctx):
+    """An unknown precision code is an argument error, not a silent FP64 run."""
+    rng = np.random.default_rng(31)
+    L, W1, W2, R, psi = H.hermitian_cent_problem(rng, dl=(3, 4, 2), dr=(2, 4, 3))
+    d = [ctx.put(x) for x in (L, W1, W2, R, psi)]
+    with pytest.raises(E.EngineError):
+        E.lanczos(*d, "cent", 1e-10, 50, 3)
+
+
+@pytest.mark.parametrize("prec", [E.PREC_FP64, E.PREC_MIXED, E.PREC_MIXED_BF16X3])
+def test_narrow_outputs_vs_numpy(ctx, prec):
+    """Contractions whose output blocks are at most 16 columns wide run on the
+    vector-ALU kernel (one thread per output row): output blocks of 1..16
+    columns and 1..1000 rows (across the 256-row workgroup edge), each fed by
+    one or two block pairs over two contracted legs (K from 1 to 120), all four
+    operand layouts, against numpy (FP64 1e-12; the FP32 forms to
+    low_prec_bound), and bit-identical on a second run."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(37)
+    m = Leg(IN, [((0,), 1), ((1,), 255), ((2,), 257), ((3,), 1000)])
+    k1 = Leg(OUT, [((0,), 1), ((1,), 7), ((2,), 40)])
+    k2 = Leg(OUT, [((0,), 1), ((1,), 3)])
+    for nd in (1, 3, 8, 13, 16):
+        n = Leg(OUT, [((q,), max(1, nd - q)) for q in range(4)])
+        A0 = BTensor.random([m, k1, k2], (0,), rng)
+        B0 = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
+        ref = np.tensordot(A0.dense(), B0.dense(), axes=([1, 2], [0, 1]))
+        bound = np.tensordot(np.abs(A0.dense()), np.abs(B0.dense()), axes=([1, 2], [0, 1]))
+        for case in ("NN", "TN", "NT", "TT"):
+            if case[0] == "N":
+                A, ax = A0, [1, 2]
+            else:
+                A = BTensor([k1, k2, m], 1, {(c[1], c[2], c[0]): b.transpose(1, 2, 0).copy()
+                                             for c, b in A0.blocks.items()})
+                ax = [0, 1]
+            if case[1] == "N":
+                B, bx = B0, [0, 1]
+            else:
+                B = BTensor([n, k1.inverse(), k2.inverse()], 1,
+                            {(c[2], c[0], c[1]): b.transpose(2, 0, 1).copy() for c, b in B0.blocks.items()})
+                bx = [1, 2]
+            dA, dB = ctx.put(A), ctx.put(B)
+            got = E.contract_ex(dA, dB, [ax, bx], prec).get()
+            again = E.contract_ex(dA, dB, [ax, bx], prec).get()
+            for c in got.blocks:
+                assert np.array_equal(got.blocks[c], again.blocks[c]), (nd, case, c)
+            g = got.dense()
+            if prec == E.PREC_FP64:
+                assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (nd, case)
+            else:
+                assert (np.abs(g - ref) <= low_prec_bound(prec, 120) * bound + 1e-30).all(), (nd, case)

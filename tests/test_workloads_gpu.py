@@ -125,18 +125,28 @@ def assert_same_input_parity(m, site, dmax, lerr=1e-10):
         x.free()
 
 
-def mixed_vs_fp64(m, dmax, lerr, tag):
-    """One FP64 and one mixed sweep from the model's checkpoint."""
+def mixed_vs_fp64(m, dmax, lerr, tag, precision=1):
+    """One FP64 and one mixed sweep (precision 1: FP32 MFMA matvec; 2:
+    split-BF16x3) from the model's checkpoint."""
     m.restore()
     t0 = time.time()
     g64 = m.dmrg(1, 1, dmax, 0.0, lerr, 200)
     progress(f"{tag}: FP64 sweep D={dmax} {time.time() - t0:.1f}s E={g64['energy']:.14f}")
     m.restore()
     t0 = time.time()
-    gmx = m.dmrg(1, 1, dmax, 0.0, lerr, 200, precision=1)
-    progress(f"{tag}: mixed sweep D={dmax} {time.time() - t0:.1f}s E={gmx['energy']:.14f}")
+    gmx = m.dmrg(1, 1, dmax, 0.0, lerr, 200, precision=precision)
+    progress(f"{tag}: mixed (precision {precision}) sweep D={dmax} {time.time() - t0:.1f}s E={gmx['energy']:.14f}")
     rel = np.abs(gmx["energies"] - g64["energies"]) / np.abs(g64["energies"])
     return g64, gmx, rel
+
+
+def mixed_sweep(m, dmax, lerr, tag, precision):
+    """One mixed sweep from the model's checkpoint."""
+    m.restore()
+    t0 = time.time()
+    g = m.dmrg(1, 1, dmax, 0.0, lerr, 200, precision=precision)
+    progress(f"{tag}: mixed (precision {precision}) sweep D={dmax} {time.time() - t0:.1f}s E={g['energy']:.14f}")
+    return g
 
 
 # ------------------------------------------------------------------ J1-J2 cylinder (the metric's workload)
@@ -161,18 +171,23 @@ def test_j1j2_4x6_cylinder_per_update_parity():
     g64, gmx, rel = mixed_vs_fp64(m, 256, 1e-12, "j1j2 4x6")
     assert rel.max() < 1e-10, rel.max()
     assert np.array_equal(gmx["dims"], g64["dims"])
+    gbf = mixed_sweep(m, 256, 1e-12, "j1j2 4x6", 2)
+    rel = np.abs(gbf["energies"] - g64["energies"]) / np.abs(g64["energies"])
+    assert rel.max() < 1e-10, rel.max()
+    assert np.array_equal(gbf["dims"], g64["dims"])
 
 
 def test_j1j2_4x24_cylinder_D4096_mixed_vs_fp64():
     """The metric's workload at full size: J1-J2 4x24 cylinder (96 sites,
     J2 = 0.5), U(1) Sz = 0, FP64 ramp 256 -> 512 -> 1024 -> 2048 -> 4096 from a
-    seeded random start, then one FP64 and one mixed (FP32-MFMA matvec, FP64
-    residual restarts) sweep at D = 4096 from the same checkpoint.  No
-    reference output exists at this size; the checks are the precision
-    contract and what the variational method fixes: every update's energy of
-    the mixed sweep within 1e-10 relative of the FP64 sweep, identical kept
-    dimensions, D = 4096 reached, and the energy at the end of each sweep not
-    above the previous sweep's (within the Lanczos tolerance)."""
+    seeded random start, then one FP64 sweep and one sweep in each mixed form
+    (FP32-MFMA matvec, and split-BF16x3 matvec, both with FP64 residual
+    restarts) at D = 4096 from the same checkpoint.  No reference output
+    exists at this size; the checks are the precision contract and what the
+    variational method fixes: every update's energy of each mixed sweep within
+    1e-10 relative of the FP64 sweep, identical kept dimensions, D = 4096
+    reached, and the energy at the end of each sweep not above the previous
+    sweep's (within the Lanczos tolerance)."""
     N = 96
     m = HM.j1j2_cylinder(24, 4, 1.0, 0.5)
     m.init_mps(spin_labs(N), enlarged_dim=8, seed=1234)
@@ -191,6 +206,10 @@ def test_j1j2_4x24_cylinder_D4096_mixed_vs_fp64():
     assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
     assert np.array_equal(gmx["dims"], g64["dims"]), np.nonzero(gmx["dims"] != g64["dims"])[0]
     assert g64["energy"] <= e_prev + 1e-9 and g64["dims"].max() == 4096
+    gbf = mixed_sweep(m, 4096, 1e-10, "j1j2 4x24", 2)
+    rel = np.abs(gbf["energies"] - g64["energies"]) / np.abs(g64["energies"])
+    assert rel.max() <= 1e-10, rel.max()
+    assert np.array_equal(gbf["dims"], g64["dims"]), np.nonzero(gbf["dims"] != g64["dims"])[0]
 
 
 # ------------------------------------------------------------------ Fermi-Hubbard cylinders (U(1) x U(1))
