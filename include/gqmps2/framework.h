@@ -53,7 +53,8 @@ inline void CreatPath(const std::string &path) {
 }
 
 // ---------------------------------------------------------------- tensor files
-// Layout: "MPS2TEN1", int64 nmeta, int32 meta[nmeta], int64 ndata, double data[ndata].
+// Layout: "MPS2TEN1", int64 nmeta, int32 meta[nmeta], int64 ndata, double data[ndata]
+// (GQTEN_Complex tensors: data holds (real, imaginary) pairs, interleaved).
 template <typename TenT>
 inline void WriteGQTensorTOFile(const TenT &t, const std::string &file) {
   std::vector<int32_t> meta;

@@ -127,6 +127,20 @@ class MPOGenerator {
     // test_mpogen.cc:79-207.
     using Str = std::vector<size_t>;
     using Fut = std::map<Str, TenElemT>;
+    // total order of futures (complex weights ordered by real, then imaginary part)
+    struct FutLess {
+      bool operator()(const Fut &x, const Fut &y) const {
+        auto ex = x.begin(), ey = y.begin();
+        for (; ex != x.end() && ey != y.end(); ++ex, ++ey) {
+          if (ex->first != ey->first) return ex->first < ey->first;
+          const double xr = std::real(ex->second), yr = std::real(ey->second);
+          if (xr != yr) return xr < yr;
+          const double xi = std::imag(ex->second), yi = std::imag(ey->second);
+          if (xi != yi) return xi < yi;
+        }
+        return ex == x.end() && ey != y.end();
+      }
+    };
     std::vector<size_t> id_op(N_);
     for (size_t i = 0; i < N_; ++i) id_op[i] = Intern_(site_vec_.id_ops[i]);
     Fut root;
@@ -147,7 +161,7 @@ class MPOGenerator {
     std::vector<QNT> cur_qn{zero_div_};
     for (size_t i = 0; i < N_; ++i) {
       std::vector<Fut> next;
-      std::map<Fut, size_t> nidx;
+      std::map<Fut, size_t, FutLess> nidx;
       std::vector<QNT> next_qn;
       for (size_t j = 0; j < cur.size(); ++j) {
         std::map<size_t, Fut> by_op;
@@ -220,8 +234,8 @@ class MPOGenerator {
         size_t d = op.GetIndexes()[0].dim();
         for (size_t a = 0; a < d; ++a)
           for (size_t bb = 0; bb < d; ++bb) {
-            double v = op.GetElem({a, bb});
-            if (v == 0.0) continue;
+            TenElemT v = op.GetElem({a, bb});
+            if (v == TenElemT(0)) continue;
             v *= e.w;
             if (i == 0) (*mpo(i))({a, coor[0][e.to], bb}) += v;
             else if (i == N_ - 1) (*mpo(i))({a, coor[i - 1][e.from], bb}) += v;

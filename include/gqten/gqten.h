@@ -8,13 +8,16 @@
 // initial MPS) live on the host until an operation needs them on the device;
 // contractions, SVD, linear combinations, Dag and norms run on the GPU through
 // the C ABI, while the set-up edits of user code (element writes, Random,
-// Transpose, scalar multiples and sums) act on the host copy.  Real
-// (GQTEN_Double) element type only.
+// Transpose, scalar multiples and sums) act on the host copy.  Element types
+// GQTEN_Double and GQTEN_Complex (std::complex<double>); a contraction of a
+// real tensor with a complex one (the real singular values times a complex
+// factor) gives a complex tensor.
 #ifndef MPS2_AMD_GQTEN_H
 #define MPS2_AMD_GQTEN_H
 
 #include <chrono>
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -26,6 +29,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../mps2_amd.h"
@@ -33,6 +37,11 @@
 namespace gqten {
 
 using GQTEN_Double = double;
+using GQTEN_Complex = std::complex<double>;
+
+// Real part of a scalar (lanczos_solver_impl.h:64-67).
+inline double Real(const double x) { return x; }
+inline double Real(const GQTEN_Complex &x) { return x.real(); }
 
 enum class GQTenIndexDirType { IN = 0, OUT = 1, NDIR = 2 };
 // gqten's short names for the two directions (IndexT({...}, OUT) in user code)
@@ -198,10 +207,13 @@ struct DevTen {
 // ---------------------------------------------------------------- GQTensor
 template <typename ElemT, typename QNT>
 class GQTensor {
-  static_assert(std::is_same<ElemT, double>::value, "this build supports GQTEN_Double tensors");
+  static_assert(std::is_same<ElemT, GQTEN_Double>::value || std::is_same<ElemT, GQTEN_Complex>::value,
+                "GQTensor element type: GQTEN_Double or GQTEN_Complex");
 
  public:
   using BlockKey = std::vector<int32_t>;
+  static constexpr bool kComplex = std::is_same<ElemT, GQTEN_Complex>::value;
+  static constexpr int kDoubles = kComplex ? 2 : 1;   // doubles per element in the flat format
 
   GQTensor() : is_default_(true) {}
   explicit GQTensor(const IndexVec<QNT> &idxs) : indexes_(idxs), is_default_(false) {}
@@ -217,34 +229,34 @@ class GQTensor {
   bool IsDefault() const { return is_default_; }
 
   // ---- element access (host side)
-  double &operator()(const std::vector<size_t> &coors) {
+  ElemT &operator()(const std::vector<size_t> &coors) {
     ToHost();
     dev_.reset();
     return ElemRef(coors);
   }
   template <typename... Ints>
-  double &operator()(size_t c0, Ints... cs) {
+  ElemT &operator()(size_t c0, Ints... cs) {
     return (*this)(std::vector<size_t>{c0, (size_t)cs...});
   }
   // read-only element access on a const tensor (as gqten's const operator())
-  double operator()(const std::vector<size_t> &coors) const { return GetElem(coors); }
+  ElemT operator()(const std::vector<size_t> &coors) const { return GetElem(coors); }
   template <typename... Ints>
-  double operator()(size_t c0, Ints... cs) const {
+  ElemT operator()(size_t c0, Ints... cs) const {
     return GetElem(std::vector<size_t>{c0, (size_t)cs...});
   }
-  double GetElem(const std::vector<size_t> &coors) const {
+  ElemT GetElem(const std::vector<size_t> &coors) const {
     ToHost();
     BlockKey k;
     size_t off;
     Addr(coors, k, off);
     auto it = hblk_.find(k);
-    return it == hblk_.end() ? 0.0 : it->second[off];
+    return it == hblk_.end() ? ElemT(0) : it->second[off];
   }
   // rank-0 scalar
-  double operator()() const {
+  ElemT operator()() const {
     ToHost();
     auto it = hblk_.find(BlockKey());
-    return it == hblk_.end() ? 0.0 : it->second[0];
+    return it == hblk_.end() ? ElemT(0) : it->second[0];
   }
 
   double Normalize() {
@@ -262,19 +274,19 @@ class GQTensor {
   // gqten's tensor arithmetic: scalar multiples and sums of tensors with equal
   // indexes (used to build site operators, e.g. -t * c_up in
   // test_two_site_update_finite_vmps.cc:898-901).  Computed on the host copy.
-  GQTensor &operator*=(const double s) {
+  GQTensor &operator*=(const ElemT s) {
     ToHost();
     dev_.reset();
     for (auto &kv : hblk_)
       for (auto &x : kv.second) x *= s;
     return *this;
   }
-  GQTensor operator*(const double s) const {
+  GQTensor operator*(const ElemT s) const {
     GQTensor r(*this);
     r *= s;
     return r;
   }
-  GQTensor operator-() const { return (*this) * -1.0; }
+  GQTensor operator-() const { return (*this) * ElemT(-1.0); }
   GQTensor &operator+=(const GQTensor &o) {
     if (is_default_ || o.is_default_ || indexes_ != o.indexes_)
       throw std::invalid_argument("GQTensor sum: the tensors must have the same indexes");
@@ -313,6 +325,14 @@ class GQTensor {
   }
   // Adopt a device tensor produced by the engine; indexes are rebuilt from it.
   void AdoptDevice(mps2_ten h) {
+    int cz = 0;
+    CheckMps2(mps2_ten_is_complex(h, &cz), "is_complex");
+    if ((cz != 0) != kComplex) {
+      mps2_ten_free(h);
+      throw std::runtime_error(kComplex ? "GQTensor<GQTEN_Complex>: the engine returned a real tensor"
+                                        : "GQTensor<GQTEN_Double>: the engine returned a complex tensor "
+                                          "(a complex operand needs a GQTEN_Complex result)");
+    }
     dev_ = std::make_shared<DevTen>(h);
     is_default_ = false;
     hblk_.clear();
@@ -341,8 +361,10 @@ class GQTensor {
     std::vector<double> data;
     ToFlat(meta, data);
     if (data.empty()) data.push_back(0.0);
+    if (data.size() < 2) data.resize(2, 0.0);
     mps2_ten h = nullptr;
-    CheckMps2(mps2_ten_import(DefaultCtx(), meta.data(), data.data(), &h), "upload");
+    if (kComplex) CheckMps2(mps2_ten_import_complex(DefaultCtx(), meta.data(), data.data(), &h), "upload");
+    else CheckMps2(mps2_ten_import(DefaultCtx(), meta.data(), data.data(), &h), "upload");
     dev_ = std::make_shared<DevTen>(h);
   }
   void ToHost() const {
@@ -359,7 +381,7 @@ class GQTensor {
  private:
   IndexVec<QNT> indexes_;
   bool is_default_ = true;
-  mutable std::map<BlockKey, std::vector<double>> hblk_;
+  mutable std::map<BlockKey, std::vector<ElemT>> hblk_;
   mutable bool host_valid_ = true;
   mutable std::shared_ptr<DevTen> dev_;
 
@@ -379,12 +401,12 @@ class GQTensor {
       off = off * indexes_[i].GetQNSct(s).dim() + o;
     }
   }
-  double &ElemRef(const std::vector<size_t> &coors) {
+  ElemT &ElemRef(const std::vector<size_t> &coors) {
     BlockKey k;
     size_t off;
     Addr(coors, k, off);
     auto &b = hblk_[k];
-    if (b.empty()) b.assign(BlockSize(k), 0.0);
+    if (b.empty()) b.assign(BlockSize(k), ElemT(0));
     return b[off];
   }
   void ParseLegs(const std::vector<int32_t> &m) {
@@ -422,9 +444,15 @@ class GQTensor {
       BlockKey k(m.begin() + p, m.begin() + p + rank);
       p += rank;
       size_t sz = BlockSize(k);
-      hblk_[k] = std::vector<double>(data.begin() + off, data.begin() + off + sz);
-      off += sz;
+      std::vector<ElemT> v(sz);
+      for (size_t e = 0; e < sz; ++e) v[e] = ElemFrom(&data[off + kDoubles * e]);
+      hblk_[k] = std::move(v);
+      off += kDoubles * sz;
     }
+  }
+  static ElemT ElemFrom(const double *x) {
+    if constexpr (kComplex) return ElemT(x[0], x[1]);
+    else return x[0];
   }
 };
 
@@ -460,7 +488,14 @@ void GQTensor<ElemT, QNT>::Random(const QNT &div) {
     if (BlockDiv(indexes_, k) == div) {
       auto &b = hblk_[k];
       b.resize(BlockSize(k));
-      for (auto &x : b) x = u(RandomEngine());
+      for (auto &x : b) {
+        if constexpr (kComplex) {
+          const double re = u(RandomEngine());
+          x = ElemT(re, u(RandomEngine()));
+        } else {
+          x = u(RandomEngine());
+        }
+      }
     }
     int i = (int)r - 1;
     for (; i >= 0; --i) {
@@ -477,7 +512,7 @@ void GQTensor<ElemT, QNT>::Transpose(const std::vector<size_t> &perm) {
   dev_.reset();
   IndexVec<QNT> ni;
   for (auto p : perm) ni.push_back(indexes_[p]);
-  std::map<BlockKey, std::vector<double>> nb;
+  std::map<BlockKey, std::vector<ElemT>> nb;
   size_t r = perm.size();
   for (auto &kv : hblk_) {
     BlockKey k(r);
@@ -487,7 +522,7 @@ void GQTensor<ElemT, QNT>::Transpose(const std::vector<size_t> &perm) {
       k[i] = kv.first[perm[i]];
       nd[i] = od[perm[i]];
     }
-    std::vector<double> out(kv.second.size());
+    std::vector<ElemT> out(kv.second.size());
     std::vector<size_t> ostr(r, 1);
     for (int i = (int)r - 2; i >= 0; --i) ostr[i] = ostr[i + 1] * od[i + 1];
     std::vector<size_t> idx(r, 0);
@@ -524,7 +559,15 @@ void GQTensor<ElemT, QNT>::ToFlat(std::vector<int32_t> &meta, std::vector<double
   }
   for (auto &kv : hblk_)
     for (auto c : kv.first) meta.push_back(c);
-  for (auto &kv : hblk_) data.insert(data.end(), kv.second.begin(), kv.second.end());
+  for (auto &kv : hblk_)
+    for (const ElemT &x : kv.second) {
+      if constexpr (kComplex) {
+        data.push_back(x.real());
+        data.push_back(x.imag());
+      } else {
+        data.push_back(x);
+      }
+    }
 }
 
 // Structure check of flat input from outside the engine (tensor files, FFI
@@ -573,7 +616,8 @@ inline void CheckFlat(const std::vector<int32_t> &m, const size_t ndata) {
 
 template <typename ElemT, typename QNT>
 void GQTensor<ElemT, QNT>::FromFlat(const std::vector<int32_t> &meta, const std::vector<double> &data) {
-  CheckFlat<QNT>(meta, data.size());
+  if (data.size() % kDoubles) throw std::runtime_error("malformed tensor data: data size");
+  CheckFlat<QNT>(meta, data.size() / kDoubles);
   is_default_ = false;
   dev_.reset();
   ParseLegs(meta);
@@ -584,11 +628,17 @@ void GQTensor<ElemT, QNT>::FromFlat(const std::vector<int32_t> &meta, const std:
 // ---------------------------------------------------------------- free functions
 template <typename ElemT, typename QNT>
 GQTensor<ElemT, QNT> operator*(const double s, const GQTensor<ElemT, QNT> &t) {
+  return t * ElemT(s);
+}
+template <typename QNT>
+GQTensor<GQTEN_Complex, QNT> operator*(const GQTEN_Complex s, const GQTensor<GQTEN_Complex, QNT> &t) {
   return t * s;
 }
 
-template <typename T>
-void Contract(const T *a, const T *b, const std::vector<std::vector<size_t>> &axes, T *out) {
+// Contract of two tensors of either element type (a real operand with a
+// complex one gives a complex result, which `out` must hold).
+template <typename TA, typename TB, typename TO>
+void Contract(const TA *a, const TB *b, const std::vector<std::vector<size_t>> &axes, TO *out) {
   std::vector<int32_t> xa(axes[0].begin(), axes[0].end()), xb(axes[1].begin(), axes[1].end());
   mps2_ten h = nullptr;
   CheckMps2(mps2_contract(a->Handle(), b->Handle(), (int)xa.size(), xa.data(), xb.data(), &h), "Contract");
@@ -646,8 +696,15 @@ void LinearCombine(const std::vector<ElemT> &coefs, const std::vector<GQTensor<E
     CheckMps2(mps2_linear_combine(0, zc.data(), nullptr, 0.0, h0), "LinearCombine");
     *out = tmp;
   }
-  std::vector<double> c(coefs.begin(), coefs.end());
-  CheckMps2(mps2_linear_combine((int)hs.size(), c.data(), hs.data(), beta, out->Handle()), "LinearCombine");
+  // the engine takes real coefficients (those of the Lanczos recurrence,
+  // lanczos_solver_impl.h:151-158,173,215, are real for either element type)
+  std::vector<double> c;
+  for (const ElemT &x : coefs) {
+    if (std::imag(x) != 0.0) throw std::invalid_argument("LinearCombine: complex coefficients are not supported");
+    c.push_back(std::real(x));
+  }
+  if (std::imag(beta) != 0.0) throw std::invalid_argument("LinearCombine: complex coefficients are not supported");
+  CheckMps2(mps2_linear_combine((int)hs.size(), c.data(), hs.data(), std::real(beta), out->Handle()), "LinearCombine");
   out->ReleaseHost();
 }
 

@@ -94,6 +94,17 @@ int mps2_ctx_set_tuning(mps2_ctx ctx, int mode);
 /* Tensors.  Replace GQTensor construction, element fill and the stream I/O
  * used by TenVec::LoadTen/DumpTen (one_dim_tn/framework/ten_vec.h:50-85). */
 int mps2_ten_import(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_ten *out);
+/* A GQTensor<GQTEN_Complex> (gqten's std::complex<double> element type,
+ * test_two_site_update_finite_vmps.cc:374-484,704-786): the same meta, data
+ * holding two doubles (real, imaginary) per element, interleaved.  Complex
+ * tensors are exported in that form (mps2_ten_export_sizes counts doubles);
+ * Contract, Dag (conjugating), LinearCombine and Normalize (real
+ * coefficients), the effective Hamiltonian, Lanczos (MPS2_PREC_FP64) and SVD
+ * accept them; a real operand of a contraction with a complex one enters as
+ * complex; mps2_dot returns Re <a|b>. */
+int mps2_ten_import_complex(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_ten *out);
+/* 1 for a complex tensor, 0 for a real one. */
+int mps2_ten_is_complex(mps2_ten t, int *is_complex);
 int mps2_ten_export_sizes(mps2_ten t, int64_t *nmeta, int64_t *ndata);
 int mps2_ten_export(mps2_ten t, int32_t *meta, double *data);
 int mps2_ten_free(mps2_ten t);

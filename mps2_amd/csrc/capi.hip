@@ -47,6 +47,14 @@ void need(const void *p, const char *what) {
 mps2::TenP opt(mps2_ten h) { return h ? h->t : nullptr; }
 }  // namespace
 
+namespace {
+bool any_cplx(std::initializer_list<mps2::TenP> ts) {
+  for (auto &t : ts)
+    if (t && t->cplx) return true;
+  return false;
+}
+}  // namespace
+
 extern "C" {
 
 int mps2_version(void) { return 100; }
@@ -148,11 +156,30 @@ int mps2_ten_import(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_
   });
 }
 
+int mps2_ten_import_complex(mps2_ctx ctx, const int32_t *meta, const double *data, mps2_ten *out) {
+  return guard([&] {
+    need(ctx, "ctx");
+    need(meta, "meta");
+    need(out, "out");
+    auto m = mps2::meta_from_flat(meta, nullptr);
+    if (m->total > 0) need(data, "data");
+    *out = wrap(mps2::ten_upload(ctx->c.get(), m, data, true));
+  });
+}
+
+int mps2_ten_is_complex(mps2_ten t, int *is_complex) {
+  return guard([&] {
+    need(t, "t");
+    need(is_complex, "is_complex");
+    *is_complex = t->t->cplx ? 1 : 0;
+  });
+}
+
 int mps2_ten_export_sizes(mps2_ten t, int64_t *nmeta, int64_t *ndata) {
   return guard([&] {
     need(t, "t");
     if (nmeta) *nmeta = mps2::meta_flat_size(*t->t->meta);
-    if (ndata) *ndata = t->t->meta->total;
+    if (ndata) *ndata = t->t->nr();
   });
 }
 
@@ -223,6 +250,8 @@ int mps2_contract_ex(mps2_ten a, mps2_ten b, int nax, const int32_t *axes_a, con
       *out = wrap(mps2::contract(a->t, b->t, xa, xb));
       return;
     }
+    if (a->t->cplx || b->t->cplx)
+      throw mps2::Error(mps2::kErrArg, "Contract: complex tensors are contracted at MPS2_PREC_FP64");
     mps2::Ctx *c = a->t->ctx;
     mps2::SplitScope scope(c, ar == mps2::kArithBf16x3);
     auto to32 = [&](const mps2::TenP &t) {
@@ -299,7 +328,8 @@ int mps2_heff_mul(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten psi
     if (w == "cent" || w == "rend") need(L, "L");
     if (w == "cent" || w == "lend") need(R, "R");
     auto full = mps2::full_meta(psi->t->meta->qw, psi->t->meta->legs, psi->t->meta->div());
-    auto o = mps2::ten_alloc(psi->t->ctx, full);
+    const bool cz = any_cplx({opt(L), W1->t, W2->t, opt(R), psi->t});
+    auto o = mps2::ten_alloc(psi->t->ctx, full, mps2::kF64, cz);
     mps2::heff_mul(opt(L), W1->t, W2->t, opt(R), psi->t, w, o, mps2::kF64);
     *out = wrap(o);
   });
@@ -333,6 +363,9 @@ int mps2_heff_create(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten 
     auto h = std::make_unique<mps2_heff_s>();
     h->where = w;
     const mps2::Arith ar = mps2::arith_of(precision);
+    const bool cz = any_cplx({opt(L), W1->t, W2->t, opt(R), psi->t});
+    if (cz && ar != mps2::kArithF64)
+      throw mps2::Error(mps2::kErrArg, "complex effective Hamiltonians run at MPS2_PREC_FP64");
     h->prec = mps2::storage_of(ar);
     h->split = ar == mps2::kArithBf16x3;
     mps2::SplitScope scope(c, h->split);
@@ -347,15 +380,15 @@ int mps2_heff_create(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten 
     h->W1 = cast(W1->t);
     h->W2 = cast(W2->t);
     h->R = cast(opt(R));
-    h->x = cast(mps2::embed(psi->t, full));
-    h->y = mps2::ten_alloc(c, full, h->prec);
+    h->x = cast(mps2::embed(cz ? mps2::ten_promote(psi->t) : psi->t, full));
+    h->y = mps2::ten_alloc(c, full, h->prec, cz);
     // one application: builds (and caches) the plans and counts the flops
     const double f0 = c->gemm_flops;
     mps2::heff_mul(h->L, h->W1, h->W2, h->R, h->x, w, h->y, h->prec);
     h->flops = c->gemm_flops - f0;
     for (auto &t : {opt(L), W1->t, W2->t, opt(R)})
-      if (t) h->bytes += 8.0 * t->n();
-    h->bytes += 2 * 8.0 * full->total;
+      if (t) h->bytes += 8.0 * t->nr();
+    h->bytes += 2 * 8.0 * full->total * (cz ? 2 : 1);
     c->sync();
     *out = h.release();
   });
@@ -409,6 +442,8 @@ int mps2_heff_bench(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2_ten p
     need(where, "where");
     need(out, "out");
     if (reps < 1) throw mps2::Error(mps2::kErrArg, "reps must be >= 1");
+    if (any_cplx({opt(L), W1->t, W2->t, opt(R), psi->t}))
+      throw mps2::Error(mps2::kErrArg, "heff_bench: real tensors only");
     std::string w(where);
     if (w == "cent" || w == "rend") need(L, "L");
     if (w == "cent" || w == "lend") need(R, "R");
@@ -530,6 +565,8 @@ int mps2_heff_bench_shard(mps2_ten L, mps2_ten W1, mps2_ten W2, mps2_ten R, mps2
     need(out, "out");
     if (reps < 1 || nshard < 1 || shard < 0 || shard >= nshard || stage < 0)
       throw mps2::Error(mps2::kErrArg, "heff_bench_shard: bad reps, stage or shard");
+    if (any_cplx({opt(L), W1->t, W2->t, opt(R), psi->t}))
+      throw mps2::Error(mps2::kErrArg, "heff_bench_shard: real tensors only");
     std::string w(where);
     if (w == "cent" || w == "rend") need(L, "L");
     if (w == "cent" || w == "lend") need(R, "R");

@@ -133,27 +133,29 @@ void Ctx::sync() {
   MPS2_HIP(hipStreamSynchronize(stream));
 }
 
-TenP ten_alloc(Ctx *c, MetaP m, Prec p) {
+TenP ten_alloc(Ctx *c, MetaP m, Prec p, bool cplx) {
+  if (cplx && p != kF64) throw Error(kErrArg, "complex tensors are FP64");
   auto t = std::make_shared<Ten>();
   t->ctx = c;
   t->meta = std::move(m);
   t->prec = p;
+  t->cplx = cplx;
   size_t es = p == kF64 ? sizeof(double) : sizeof(float);
-  t->buf = c->alloc(std::max<int64_t>(t->meta->total, 1) * es);
+  t->buf = c->alloc(std::max<int64_t>(t->nr(), 1) * es);
   return t;
 }
 
-TenP ten_zeros(Ctx *c, MetaP m, Prec p) {
-  auto t = ten_alloc(c, std::move(m), p);
-  if (p == kF64) k_fill(c, t->d(), std::max<int64_t>(t->n(), 1), 0.0);
-  else k_fillf(c, t->f(), std::max<int64_t>(t->n(), 1), 0.0f);
+TenP ten_zeros(Ctx *c, MetaP m, Prec p, bool cplx) {
+  auto t = ten_alloc(c, std::move(m), p, cplx);
+  if (p == kF64) k_fill(c, t->d(), std::max<int64_t>(t->nr(), 1), 0.0);
+  else k_fillf(c, t->f(), std::max<int64_t>(t->nr(), 1), 0.0f);
   return t;
 }
 
-TenP ten_upload(Ctx *c, MetaP m, const double *host) {
-  auto t = ten_alloc(c, std::move(m));
-  if (t->n() > 0) {
-    MPS2_HIP(hipMemcpyAsync(t->d(), host, t->n() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+TenP ten_upload(Ctx *c, MetaP m, const double *host, bool cplx) {
+  auto t = ten_alloc(c, std::move(m), kF64, cplx);
+  if (t->nr() > 0) {
+    MPS2_HIP(hipMemcpyAsync(t->d(), host, t->nr() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     c->sync();
   }
   return t;
@@ -162,15 +164,23 @@ TenP ten_upload(Ctx *c, MetaP m, const double *host) {
 void ten_download(const Ten &t, double *host) {
   if (t.n() == 0) return;
   if (t.prec != kF64) throw Error(kErrState, "download of a non-FP64 tensor");
-  MPS2_HIP(hipMemcpyAsync(host, t.d(), t.n() * sizeof(double), hipMemcpyDeviceToHost, t.ctx->stream));
+  MPS2_HIP(hipMemcpyAsync(host, t.d(), t.nr() * sizeof(double), hipMemcpyDeviceToHost, t.ctx->stream));
   t.ctx->sync();
 }
 
 TenP ten_copy(const Ten &t) {
-  auto r = ten_alloc(t.ctx, t.meta, t.prec);
+  auto r = ten_alloc(t.ctx, t.meta, t.prec, t.cplx);
   size_t es = t.prec == kF64 ? 8 : 4;
   if (t.n() > 0)
-    MPS2_HIP(hipMemcpyAsync(r->buf->p, t.buf->p, t.n() * es, hipMemcpyDeviceToDevice, t.ctx->stream));
+    MPS2_HIP(hipMemcpyAsync(r->buf->p, t.buf->p, t.nr() * es, hipMemcpyDeviceToDevice, t.ctx->stream));
+  return r;
+}
+
+TenP ten_promote(const TenP &t) {
+  if (t->cplx) return t;
+  MPS2_REQUIRE(t->prec == kF64, "complex promotion of a non-FP64 tensor");
+  auto r = ten_alloc(t->ctx, t->meta, kF64, true);
+  k_promote(t->ctx, t->n(), t->d(), r->d());
   return r;
 }
 
@@ -189,6 +199,12 @@ TenP ten_dag(const TenP &t) {
       t->meta->dag.p = nm;
       m = nm;
     }
+  }
+  if (t->cplx) {
+    auto r = ten_copy(*t);
+    k_conj(t->ctx, t->n(), r->d());
+    r->meta = m;
+    return r;
   }
   auto r = std::make_shared<Ten>(*t);
   r->meta = m;
@@ -459,7 +475,53 @@ void k_permute(Ctx *c, const T *src, T *dst, const PermDesc *d_desc, const int64
     permute_kernel<T><<<grid_for(total, 256, 16384), 256, 0, c->stream>>>(src, dst, d_desc, d_dst_start, nblk, total);
 }
 template void k_permute<double>(Ctx *, const double *, double *, const PermDesc *, const int64_t *, int, int64_t);
+template void k_permute<Cplx>(Ctx *, const Cplx *, Cplx *, const PermDesc *, const int64_t *, int, int64_t);
 template void k_permute<float>(Ctx *, const float *, float *, const PermDesc *, const int64_t *, int, int64_t);
+
+namespace {
+// x[2i+1] = -x[2i+1]: complex conjugate in place (interleaved pairs).
+__global__ void conj_kernel(double *x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[2 * i + 1] = -x[2 * i + 1];
+}
+// out[2i] = re[i], out[2i+1] = 0.
+__global__ void promote_kernel(const double *re, double *out, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    out[2 * i] = re[i];
+    out[2 * i + 1] = 0.0;
+  }
+}
+// One descriptor per blockIdx.y; grid-stride over its K x N complex entries.
+__global__ void expand_cplx_kernel(const double *src, double *dst, const CplxExpand *desc) {
+  const CplxExpand d = desc[blockIdx.y];
+  const int64_t kn = (int64_t)d.K * d.N;
+  const int64_t ld = 2 * (int64_t)d.N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < kn; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e / d.N, n = e - k * d.N;
+    const int64_t so = d.src_off + (d.trans ? n * d.K + k : e);
+    const double re = src[2 * so], im = src[2 * so + 1];
+    double *row0 = dst + d.dst_off + 2 * k * ld + 2 * n;
+    row0[0] = re;
+    row0[1] = im;
+    row0[ld] = -im;
+    row0[ld + 1] = re;
+  }
+}
+}  // namespace
+
+void k_conj(Ctx *c, int64_t n, double *x) {
+  if (n > 0) conj_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(x, n);
+}
+
+void k_promote(Ctx *c, int64_t n, const double *re, double *out) {
+  if (n > 0) promote_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(re, out, n);
+}
+
+void k_expand_cplx(Ctx *c, const double *src, double *dst, const CplxExpand *d_desc, int n, int64_t max_kn) {
+  if (n <= 0) return;
+  dim3 g(grid_for(max_kn, 256, 1024), n);
+  expand_cplx_kernel<<<g, 256, 0, c->stream>>>(src, dst, d_desc);
+}
 
 void k_copy2d(Ctx *c, const double *src, double *dst, const Copy2D *d_desc, int n) {
   if (n <= 0) return;

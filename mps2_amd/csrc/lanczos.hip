@@ -88,10 +88,16 @@ struct LzDev {
   }
 };
 
-TenP combine_dev(Ctx *c, const MetaP &m, const LzDev &dv, int n, const double *d_coef) {
-  auto gs = ten_alloc(c, m);
-  k_lincomb(c, m->total, n, dv.ptrs(), d_coef, gs->d());
+TenP combine_dev(Ctx *c, const MetaP &m, const LzDev &dv, int n, const double *d_coef, bool cplx = false) {
+  auto gs = ten_alloc(c, m, kF64, cplx);
+  k_lincomb(c, m->total * (cplx ? 2 : 1), n, dv.ptrs(), d_coef, gs->d());
   return gs;
+}
+
+bool any_complex(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init) {
+  for (const TenP *t : {&L, &W1, &W2, &R, &init})
+    if (*t && (*t)->cplx) return true;
+  return false;
 }
 
 }  // namespace
@@ -100,22 +106,27 @@ TenP combine_dev(Ctx *c, const MetaP &m, const LzDev &dv, int n, const double *d
 // kept on the device: alpha, beta and the coefficients are reduced into and
 // read from device memory, the tridiagonal ground energy is solved on the
 // device from those arrays, and the host reads two numbers per iteration
-// (energy, |gamma|) for the stopping rule.
+// (energy, |gamma|) for the stopping rule.  Complex (GQTEN_Complex) problems
+// run the same recurrence on the interleaved real form of the vectors: the
+// Hermitian H_eff makes alpha = <v|Hv> real (the reference takes Real() of
+// it, lanczos_solver_impl.h:64-67,136-141), every coefficient is real, and
+// Re <a|b> is the real dot product of the two real forms.
 LanczosOut lanczos_plain(const TenP &L, const TenP &W1, const TenP &W2, const TenP &R, const TenP &init,
                          const std::string &where, double error, int64_t max_iter) {
   Ctx *c = init->ctx;
   MPS2_REQUIRE(max_iter >= 2, "Lanczos: max_iterations must be >= 2");
   int64_t eff_dim = eff_dim_of(L, W1, W2, R, where);
   MetaP full = full_meta(init->meta->qw, init->meta->legs, init->meta->div());
-  const int64_t n = full->total;
+  const bool cz = any_complex(L, W1, W2, R, init);
+  const int64_t n = full->total * (cz ? 2 : 1);
   LzDev dv(c, (int)max_iter);
   std::vector<TenP> bases;
-  bases.push_back(embed(init, full));
+  bases.push_back(embed(cz ? ten_promote(init) : init, full));
   k_dot_to(c, n, bases[0]->d(), bases[0]->d(), dv.scal);
   lz_rsqrt_kernel<<<1, 1, 0, c->stream>>>(dv.scal, dv.scal + 1, dv.scal + 2);
   k_scale_dev(c, n, dv.scal + 2, bases[0]->d());
   dv.set_ptr(0, bases[0]->d());
-  TenP last = ten_alloc(c, full);
+  TenP last = ten_alloc(c, full, kF64, cz);
   heff_mul(L, W1, W2, R, bases[0], where, last, kF64);
   LanczosOut res;
   res.matvecs_hi = 1;
@@ -132,7 +143,7 @@ LanczosOut lanczos_plain(const TenP &L, const TenP &W1, const TenP &W2, const Te
     k_scale_dev(c, n, dv.scal + 2, gamma->d());
     bases.push_back(gamma);
     dv.set_ptr((int)m, gamma->d());
-    last = ten_alloc(c, full);
+    last = ten_alloc(c, full, kF64, cz);
     heff_mul(L, W1, W2, R, bases[m], where, last, kF64);
     res.matvecs_hi++;
     k_dot_to(c, n, last->d(), bases[m]->d(), dv.a + m);
@@ -150,7 +161,7 @@ LanczosOut lanczos_plain(const TenP &L, const TenP &W1, const TenP &W2, const Te
         res.gs = ten_copy(*bases[0]);
       } else {
         tridiag_lowest_async(c, (int)m, dv.a, dv.b, 1, dv.out, static_cast<double *>(dv.work->p));
-        res.gs = combine_dev(c, full, dv, (int)m, dv.out + 1);
+        res.gs = combine_dev(c, full, dv, (int)m, dv.out + 1, cz);
       }
       return res;
     }
@@ -158,7 +169,7 @@ LanczosOut lanczos_plain(const TenP &L, const TenP &W1, const TenP &W2, const Te
       tridiag_lowest_async(c, (int)m + 1, dv.a, dv.b, 1, dv.out, static_cast<double *>(dv.work->p));
       res.iters = m;
       res.energy = e_new;
-      res.gs = combine_dev(c, full, dv, (int)m + 1, dv.out + 1);
+      res.gs = combine_dev(c, full, dv, (int)m + 1, dv.out + 1, cz);
       c->sync();
       return res;
     }
@@ -241,6 +252,8 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
                          const std::string &where, double error, int64_t max_iter, bool split) {
   Ctx *c = init->ctx;
   MPS2_REQUIRE(max_iter >= 2, "Lanczos: max_iterations must be >= 2");
+  MPS2_REQUIRE(!any_complex(L, W1, W2, R, init),
+               "mixed-precision Lanczos: real tensors only (complex problems run MPS2_PREC_FP64)");
   int64_t eff_dim = eff_dim_of(L, W1, W2, R, where);
   MetaP full = full_meta(init->meta->qw, init->meta->legs, init->meta->div());
   TenP L32 = cast_to(L, kF32), W132 = cast_to(W1, kF32), W232 = cast_to(W2, kF32), R32 = cast_to(R, kF32);

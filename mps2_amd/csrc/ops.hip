@@ -27,7 +27,7 @@ BufP upload_bytes(Ctx *c, const void *host, size_t bytes) {
   return b;
 }
 
-using PlanKey = std::tuple<Ctx *, const Meta *, const Meta *, std::vector<int>, std::vector<int>, const Meta *>;
+using PlanKey = std::tuple<Ctx *, const Meta *, const Meta *, std::vector<int>, std::vector<int>, const Meta *, bool>;
 
 struct PlanEntry {
   std::shared_ptr<DevPlan> plan;
@@ -67,8 +67,60 @@ void perm_descs(const Meta &m, const std::vector<int> &perm, std::vector<PermDes
   }
 }
 
-std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPlan> &hp, const Meta *target) {
+// The plan of a complex contraction in real terms.  A is read as an M x 2K
+// real matrix (row-major complex, contracted legs trailing: an A stored with
+// them leading is permuted first), B as its 2K x 2N real form built per call
+// (k_expand_cplx, four reals per element), C written as M x 2N reals; pair and
+// block offsets, K and N are rescaled accordingly, and the flops are the real
+// ones (four times 2 M N K).  exp lists the expansion of each B block used.
+std::shared_ptr<ContractPlan> complex_view(const std::shared_ptr<ContractPlan> &hp, std::vector<CplxExpand> &exp,
+                                           int64_t &max_kn) {
+  auto v = std::make_shared<ContractPlan>(*hp);
+  if (v->la == kTrans) {
+    std::vector<int> afree;
+    for (int i = 0; i < v->a->rank(); ++i)
+      if (std::find(v->ax.begin(), v->ax.end(), i) == v->ax.end()) afree.push_back(i);
+    v->permute_a = true;
+    v->aperm = afree;
+    v->aperm.insert(v->aperm.end(), v->ax.begin(), v->ax.end());
+  }
+  v->la = kNormal;
+  const int btrans = (!v->permute_b && v->lb == kTrans) ? 1 : 0;
+  std::map<int64_t, int> seen;
+  max_kn = 0;
+  for (const BlkH &b : hp->blks)
+    for (int p = b.pair_begin; p < b.pair_end; ++p) {
+      const PairH &ph = hp->pairs[p];
+      if (seen.count(ph.b_off)) continue;
+      seen[ph.b_off] = (int)exp.size();
+      exp.push_back(CplxExpand{ph.b_off, 4 * ph.b_off, ph.K, b.N, btrans, 0});
+      max_kn = std::max<int64_t>(max_kn, (int64_t)ph.K * b.N);
+    }
+  v->lb = kNormal;
+  for (auto &ph : v->pairs) {
+    ph.a_off *= 2;
+    ph.b_off *= 4;
+    ph.K *= 2;
+  }
+  for (auto &b : v->blks) {
+    b.c_off *= 2;
+    b.N *= 2;
+  }
+  v->flops *= 4;
+  return v;
+}
+
+std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPlan> &hp0, const Meta *target,
+                                        bool cplx) {
   auto dp = std::make_shared<DevPlan>();
+  std::shared_ptr<ContractPlan> hp = hp0;
+  if (cplx) {
+    std::vector<CplxExpand> exp;
+    hp = complex_view(hp0, exp, dp->expand_max_kn);
+    dp->cplx = true;
+    dp->nexpand = (int)exp.size();
+    if (!exp.empty()) dp->expand = upload_bytes(c, exp.data(), exp.size() * sizeof(CplxExpand));
+  }
   dp->hp = hp;
   // Tile shape: 128x128 tiles move half the operand bytes per flop of 64x64
   // tiles; FP32 uses them unless edge padding inflates the work by more than
@@ -105,7 +157,7 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
       const Coord &oc = hp->out->coords[i];
       int32_t tb = target->find(oc);
       if (tb < 0) throw Error(kErrArg, "contract_into: target layout lacks an output block");
-      blks[i].c_off = target->offset[tb];
+      blks[i].c_off = target->offset[tb] * (cplx ? 2 : 1);
     }
     blks[i].M = b.M;
     blks[i].N = b.N;
@@ -158,13 +210,13 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
 }
 
 std::shared_ptr<DevPlan> get_plan_t(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,
-                                    const std::vector<int> &bx, const MetaP &target) {
-  PlanKey key{c, a.get(), b.get(), ax, bx, target.get()};
+                                    const std::vector<int> &bx, const MetaP &target, bool cplx = false) {
+  PlanKey key{c, a.get(), b.get(), ax, bx, target.get(), cplx};
   std::lock_guard<std::mutex> lock(g_plan_mu);
   auto it = g_plans.find(key);
   if (it != g_plans.end() && it->second.plan->hp->a == a && it->second.plan->hp->b == b) return it->second.plan;
   auto hp = plan_contract(a, b, ax, bx);
-  auto dp = build_dev_plan(c, hp, target.get());
+  auto dp = build_dev_plan(c, hp, target.get(), cplx);
   if (it != g_plans.end()) {
     // same addresses, new layouts (the old ones were freed): replace in place
     it->second = PlanEntry{dp, target};
@@ -227,14 +279,34 @@ template <typename T>
 void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   const T *A = ptr<T>(a);
   const T *B = ptr<T>(b);
-  BufP pa, pb;
-  if (dp.hp->permute_a) {
+  BufP pa, pb, pe;
+  if (dp.cplx) {
+    // permute in complex elements, then B's real form
+    if (dp.hp->permute_a) {
+      pa = c->alloc(std::max<int64_t>(a->nr(), 2) * sizeof(double));
+      k_permute<Cplx>(c, reinterpret_cast<const Cplx *>(A), static_cast<Cplx *>(pa->p),
+                      static_cast<const PermDesc *>(dp.perm_a->p), static_cast<const int64_t *>(dp.perm_a_start->p),
+                      dp.nperm_a, a->n());
+      A = static_cast<const T *>(pa->p);
+    }
+    if (dp.hp->permute_b) {
+      pb = c->alloc(std::max<int64_t>(b->nr(), 2) * sizeof(double));
+      k_permute<Cplx>(c, reinterpret_cast<const Cplx *>(B), static_cast<Cplx *>(pb->p),
+                      static_cast<const PermDesc *>(dp.perm_b->p), static_cast<const int64_t *>(dp.perm_b_start->p),
+                      dp.nperm_b, b->n());
+      B = static_cast<const T *>(pb->p);
+    }
+    pe = c->alloc(std::max<int64_t>(4 * b->n(), 1) * sizeof(double));
+    k_expand_cplx(c, reinterpret_cast<const double *>(B), static_cast<double *>(pe->p),
+                  static_cast<const CplxExpand *>(dp.expand->p), dp.nexpand, dp.expand_max_kn);
+    B = static_cast<const T *>(pe->p);
+  } else if (dp.hp->permute_a) {
     pa = c->alloc(std::max<int64_t>(a->n(), 1) * sizeof(T));
     k_permute<T>(c, A, static_cast<T *>(pa->p), static_cast<const PermDesc *>(dp.perm_a->p),
                  static_cast<const int64_t *>(dp.perm_a_start->p), dp.nperm_a, a->n());
     A = static_cast<const T *>(pa->p);
   }
-  if (dp.hp->permute_b) {
+  if (!dp.cplx && dp.hp->permute_b) {
     pb = c->alloc(std::max<int64_t>(b->n(), 1) * sizeof(T));
     k_permute<T>(c, B, static_cast<T *>(pb->p), static_cast<const PermDesc *>(dp.perm_b->p),
                  static_cast<const int64_t *>(dp.perm_b_start->p), dp.nperm_b, b->n());
@@ -258,7 +330,7 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
     if (c->time_gemms) {
       MPS2_HIP(hipEventRecord(tm.stop, c->stream));
       tm.flops = dp.hp->flops;
-      tm.bytes = (double)sizeof(T) * ((double)a->n() + (double)b->n() + (double)dp.hp->out->total);
+      tm.bytes = (double)sizeof(T) * ((double)a->nr() + (double)b->nr() + (double)dp.hp->out->total * (dp.cplx ? 2 : 1));
       c->gemm_timings.push_back(tm);
     }
     c->gemm_flops += dp.hp->flops;
@@ -283,7 +355,7 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   if (c->time_gemms) {
     MPS2_HIP(hipEventRecord(tm.stop, c->stream));
     tm.flops = dp.hp->flops;
-    tm.bytes = (double)sizeof(T) * ((double)a->n() + (double)b->n() + (double)dp.hp->out->total);
+    tm.bytes = (double)sizeof(T) * ((double)a->nr() + (double)b->nr() + (double)dp.hp->out->total * (dp.cplx ? 2 : 1));
     c->gemm_timings.push_back(tm);
   }
   c->gemm_flops += dp.hp->flops;
@@ -319,10 +391,21 @@ std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const 
   return get_plan_t(c, a, b, ax, bx, nullptr);
 }
 
-TenP contract(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx) {
-  MPS2_REQUIRE(a->ctx == b->ctx, "Contract: tensors live on different contexts");
-  MPS2_REQUIRE(a->prec == b->prec, "Contract: operand precisions differ");
-  Ctx *c = a->ctx;
+TenP contract(const TenP &a0, const TenP &b0, const std::vector<int> &ax, const std::vector<int> &bx) {
+  MPS2_REQUIRE(a0->ctx == b0->ctx, "Contract: tensors live on different contexts");
+  MPS2_REQUIRE(a0->prec == b0->prec, "Contract: operand precisions differ");
+  Ctx *c = a0->ctx;
+  if (a0->cplx || b0->cplx) {
+    // a real operand enters in complex form (gqten promotes mixed products,
+    // e.g. the real S times a complex Vt of the MPS update)
+    TenP a = ten_promote(a0), b = ten_promote(b0);
+    auto dp = get_plan_t(c, a->meta, b->meta, ax, bx, nullptr, true);
+    auto out = ten_alloc(c, dp->hp->out, kF64, true);
+    if (dp->hp->out->total == 0) return out;
+    run_plan<double>(c, *dp, a, b, out->d());
+    return out;
+  }
+  const TenP &a = a0, &b = b0;
   auto dp = get_plan(c, a->meta, b->meta, ax, bx);
   auto out = ten_alloc(c, dp->hp->out, a->prec);
   if (dp->hp->out->total == 0) return out;
@@ -331,13 +414,16 @@ TenP contract(const TenP &a, const TenP &b, const std::vector<int> &ax, const st
   return out;
 }
 
-void contract_into(const TenP &a, const TenP &b, const std::vector<int> &ax, const std::vector<int> &bx,
+void contract_into(const TenP &a0, const TenP &b0, const std::vector<int> &ax, const std::vector<int> &bx,
                    const TenP &out) {
-  MPS2_REQUIRE(a->prec == b->prec && a->prec == out->prec, "Contract: operand precisions differ");
+  MPS2_REQUIRE(a0->prec == b0->prec && a0->prec == out->prec, "Contract: operand precisions differ");
+  const bool cplx = a0->cplx || b0->cplx;
+  MPS2_REQUIRE(out->cplx == cplx, "Contract: a complex product needs a complex output (and a real one a real output)");
+  TenP a = cplx ? ten_promote(a0) : a0, b = cplx ? ten_promote(b0) : b0;
   Ctx *c = a->ctx;
-  auto dp = get_plan_t(c, a->meta, b->meta, ax, bx, out->meta);
+  auto dp = get_plan_t(c, a->meta, b->meta, ax, bx, out->meta, cplx);
   if (!(dp->hp->out->coords == out->meta->coords)) {
-    if (out->prec == kF64) k_fill(c, out->d(), std::max<int64_t>(out->n(), 1), 0.0);
+    if (out->prec == kF64) k_fill(c, out->d(), std::max<int64_t>(out->nr(), 1), 0.0);
     else k_fillf(c, out->f(), std::max<int64_t>(out->n(), 1), 0.0f);
   }
   if (dp->hp->out->total == 0) return;
@@ -367,14 +453,15 @@ TenP embed(const TenP &t, const MetaP &m) {
     return r;
   }
   MPS2_REQUIRE(t->meta->legs == m->legs, "embed: indexes differ");
-  auto r = ten_zeros(c, m);
+  auto r = ten_zeros(c, m, kF64, t->cplx);
+  const int64_t w = t->cplx ? 2 : 1;   // doubles per element
   std::vector<Copy2D> d;
   for (size_t b = 0; b < t->meta->coords.size(); ++b) {
     int32_t tb = m->find(t->meta->coords[b]);
     MPS2_REQUIRE(tb >= 0, "embed: target layout lacks a block");
-    int64_t sz = t->meta->bsize[b];
+    int64_t sz = t->meta->bsize[b] * w;
     if (sz == 0) continue;
-    d.push_back(Copy2D{t->meta->offset[b], m->offset[tb], 1, (int32_t)sz, (int32_t)sz, (int32_t)sz, 1.0});
+    d.push_back(Copy2D{t->meta->offset[b] * w, m->offset[tb] * w, 1, (int32_t)sz, (int32_t)sz, (int32_t)sz, 1.0});
   }
   if (!d.empty()) {
     auto dd = upload_bytes(c, d.data(), d.size() * sizeof(Copy2D));
@@ -383,9 +470,15 @@ TenP embed(const TenP &t, const MetaP &m) {
   return r;
 }
 
+// Re <a|b> (for complex tensors the real part of sum conj(a) b, which is the
+// dot product of their interleaved real forms).
 double dot(const Ten &a, const Ten &b) {
   MPS2_REQUIRE(a.prec == kF64 && b.prec == kF64, "dot: FP64 tensors only");
   Ctx *c = a.ctx;
+  if (a.cplx != b.cplx) {
+    auto ta = ten_promote(std::make_shared<Ten>(a)), tb = ten_promote(std::make_shared<Ten>(b));
+    return dot(*ta, *tb);
+  }
   if (!a.meta->same_layout(*b.meta)) {
     auto m = union_meta(*a.meta, *b.meta);
     auto ta = std::make_shared<Ten>(a), tb = std::make_shared<Ten>(b);
@@ -393,7 +486,7 @@ double dot(const Ten &a, const Ten &b) {
     return dot(*ea, *eb);
   }
   if (a.n() == 0) return 0.0;
-  k_dot(c, a.n(), a.d(), b.d(), 0);
+  k_dot(c, a.nr(), a.d(), b.d(), 0);
   MPS2_HIP(hipMemcpyAsync(c->host_scalars, c->dev_scalars->p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   c->sync();
   return c->host_scalars[0];
@@ -414,7 +507,7 @@ double normalize(Ten &a) {
   double n = norm(a);
   if (n > 0) {
     make_unique_buf(a);
-    k_scale(a.ctx, a.n(), 1.0 / n, a.d());
+    k_scale(a.ctx, a.nr(), 1.0 / n, a.d());
   }
   return n;
 }
@@ -422,6 +515,8 @@ double normalize(Ten &a) {
 void lincomb(const std::vector<double> &coefs, const std::vector<TenP> &ts, double beta, const TenP &out) {
   MPS2_REQUIRE(coefs.size() == ts.size(), "LinearCombine: coefficient count differs from tensor count");
   Ctx *c = out->ctx;
+  for (auto &t : ts)
+    MPS2_REQUIRE(t->cplx == out->cplx, "LinearCombine: tensors and output must all be real or all complex");
   // Grow `out` to the union of all block sets (gqten LinearCombine semantics).
   MetaP m = out->meta;
   for (auto &t : ts)
@@ -442,11 +537,11 @@ void lincomb(const std::vector<double> &coefs, const std::vector<TenP> &ts, doub
   } else {
     make_unique_buf(*out);
   }
-  if (beta != 1.0) k_scale(c, out->n(), beta, out->d());
+  if (beta != 1.0) k_scale(c, out->nr(), beta, out->d());
   for (size_t i = 0; i < ts.size(); ++i) {
     TenP t = ts[i];
     if (!t->meta->same_layout(*out->meta)) t = embed(t, out->meta);
-    k_axpby(c, out->n(), coefs[i], t->d(), 1.0, out->d());
+    k_axpby(c, out->nr(), coefs[i], t->d(), 1.0, out->d());
   }
 }
 

@@ -124,23 +124,33 @@ struct Ctx {
 // A device-resident block-sparse tensor.  Data is FP64 unless prec == kF32
 // (mixed-precision working copies).  Tensors sharing a Buf (Dag views) never
 // write through it.
+// Complex tensors (gqten's GQTEN_Complex) hold FP64 (re, im) pairs,
+// interleaved: element i of the layout at doubles 2i, 2i + 1.
 struct Ten {
   Ctx *ctx = nullptr;
   MetaP meta;
   BufP buf;
   Prec prec = kF64;
+  bool cplx = false;
   double *d() const { return static_cast<double *>(buf->p); }
   float *f() const { return static_cast<float *>(buf->p); }
-  int64_t n() const { return meta->total; }
+  int64_t n() const { return meta->total; }                  // elements
+  int64_t nr() const { return meta->total * (cplx ? 2 : 1); }  // stored reals
 };
 using TenP = std::shared_ptr<Ten>;
 
-TenP ten_alloc(Ctx *c, MetaP m, Prec p = kF64);
-TenP ten_zeros(Ctx *c, MetaP m, Prec p = kF64);
-TenP ten_upload(Ctx *c, MetaP m, const double *host);
+TenP ten_alloc(Ctx *c, MetaP m, Prec p = kF64, bool cplx = false);
+TenP ten_zeros(Ctx *c, MetaP m, Prec p = kF64, bool cplx = false);
+// host: n (real) or 2 n (complex, interleaved) doubles
+TenP ten_upload(Ctx *c, MetaP m, const double *host, bool cplx = false);
 void ten_download(const Ten &t, double *host);
 TenP ten_copy(const Ten &t);
-TenP ten_dag(const TenP &t);   // real tensors: flip directions, share data
+// Flipped directions; real tensors share the data, complex ones get the
+// complex conjugate.
+TenP ten_dag(const TenP &t);
+// The complex form of a real FP64 tensor (imaginary parts zero); t itself if
+// it is complex.
+TenP ten_promote(const TenP &t);
 
 // Arithmetic of a matvec, contraction or Lanczos call (the C ABI's precision
 // argument): FP64, FP32 on the FP32 matrix cores, or split-BF16x3 on the BF16
@@ -200,6 +210,24 @@ struct Copy2D {
   double scale;   // multiply by scale (1 for plain copies)
 };
 void k_copy2d(Ctx *c, const double *src, double *dst, const Copy2D *d_desc, int n);
+// Complex element type of the permutation kernel.
+struct alignas(16) Cplx {
+  double re, im;
+};
+// Conjugate n interleaved complex values in place; promote n reals to complex.
+void k_conj(Ctx *c, int64_t n, double *x);
+void k_promote(Ctx *c, int64_t n, const double *re, double *out);
+// Real form of a complex GEMM operand B (K x N complex, row-major, or N x K
+// with trans): the 2K x 2N real matrix [[Re, Im], [-Im, Re]] per element,
+// row-major at dst_off (doubles), so that A (M x K complex read as M x 2K
+// reals) times it is C (M x N complex read as M x 2N reals).  src_off in
+// complex elements.
+struct CplxExpand {
+  int64_t src_off, dst_off;
+  int32_t K, N;
+  int32_t trans, pad;
+};
+void k_expand_cplx(Ctx *c, const double *src, double *dst, const CplxExpand *d_desc, int n, int64_t max_kn);
 // bytes (a multiple of 16) copied by a plain streaming kernel (calibration)
 void k_calib_copy(Ctx *c, const void *src, void *dst, int64_t bytes);
 
@@ -273,6 +301,12 @@ struct DevPlan {
   int tile_m = 64, tile_n = 64;   // heuristic FP32 tile (tiles holds its list)
   BufP tiles64, tiles128;         // the output tiled 64x64 and 128x128
   int ntiles64 = 0, ntiles128 = 0;
+  // complex plans (hp in real terms, complex_view in ops.hip): the expansion
+  // descriptors of B's blocks and the largest K x N among them
+  bool cplx = false;
+  BufP expand;
+  int nexpand = 0;
+  int64_t expand_max_kn = 0;
   // narrow plans (every output block at most kNarrowMaxN wide): the vector-ALU
   // kernel's column chunk (0 = matrix-core kernels) and its tile list
   int narrow_nc = 0;
@@ -330,6 +364,16 @@ struct SvdOut {
 enum OrthSide : int { kOrthLeft = 0, kOrthRight = 1 };
 SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax,
            OrthSide side);
+// Complex tensors (svd_complex.hip; svd() dispatches there).
+SvdOut svd_complex(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax,
+                   OrthSide side);
+// Singular values below kSvdResolve * s_max have no reproducible order in any
+// FP64 SVD; the truncation treats every nonzero one of them as exactly that.
+constexpr double kSvdResolve = 1e-11;
+// The truncation rule (svd.hip) on per-sector singular values (descending
+// within each sector): kept count per sector, discarded weight, total kept.
+void svd_truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err, int64_t dmin, int64_t dmax,
+                       std::vector<int64_t> &kept, double *err, int64_t *D);
 
 // Symmetric tridiagonal lowest eigenpair on the device (tridiag.hip).
 int tridiag_lowest_dev(Ctx *c, int n, const double *a, const double *b, double *eval, double *vec);

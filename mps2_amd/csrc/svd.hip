@@ -58,7 +58,7 @@ constexpr double kDegTol = 1e-6;
 // keeps artifacts of up to eps s_max^2 / s_r ~ 1e-12 s_max (leakage of the
 // values s_r >= 3e-4 s_max just above the refinement threshold into the tail
 // basis).  The truncation treats them as tied.
-constexpr double kResolve = 1e-11;
+constexpr double kResolve = kSvdResolve;
 
 // Jacobi sweeps per 2b x 2b subproblem and per outer round (MPS2_JACOBI_INNER,
 // default 1), and the rotation threshold relative to trace(G_q)
@@ -640,12 +640,18 @@ void truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err,
 
 }  // namespace
 
+void svd_truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err, int64_t dmin, int64_t dmax,
+                       std::vector<int64_t> &kept, double *err, int64_t *D) {
+  truncate_rule(sv, trunc_err, dmin, dmax, kept, err, D);
+}
+
 SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax,
                 OrthSide side);
 
 SvdOut svd(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int64_t dmin, int64_t dmax, OrthSide side) {
   auto t0 = std::chrono::steady_clock::now();
-  SvdOut o = svd_impl(t, ldims, ldiv, trunc_err, dmin, dmax, side);
+  SvdOut o = t->cplx ? svd_complex(t, ldims, ldiv, trunc_err, dmin, dmax, side)
+                     : svd_impl(t, ldims, ldiv, trunc_err, dmin, dmax, side);
   t->ctx->count("svd_calls");
   t->ctx->count("svd_ms", std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   return o;

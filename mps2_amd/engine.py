@@ -36,6 +36,8 @@ SIGNATURES = {
     "mps2_ctx_reset_stats": (C.c_int, [_vp]),
     "mps2_ctx_set_tuning": (C.c_int, [_vp, C.c_int]),
     "mps2_ten_import": (C.c_int, [_vp, _i32p, _dp, C.POINTER(_vp)]),
+    "mps2_ten_import_complex": (C.c_int, [_vp, _i32p, _dp, C.POINTER(_vp)]),
+    "mps2_ten_is_complex": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "mps2_ten_export_sizes": (C.c_int, [_vp, _i64p, _i64p]),
     "mps2_ten_export": (C.c_int, [_vp, _i32p, _dp]),
     "mps2_ten_free": (C.c_int, [_vp]),
@@ -122,12 +124,7 @@ class Context:
 
     # tensors --------------------------------------------------------------
     def put(self, t: BTensor) -> "DTen":
-        meta, data = t.to_flat()
-        if data.size == 0:
-            data = np.zeros(1)
-        h = _vp()
-        check(lib().mps2_ten_import(self.h, meta.ctypes.data_as(_i32p), data.ctypes.data_as(_dp), C.byref(h)))
-        return DTen(h, self)
+        return DTen(_import(self.h, t), self)
 
     def tridiag_lowest(self, a, b, want_vec=True):
         a = np.ascontiguousarray(a, dtype=np.float64)
@@ -139,14 +136,19 @@ class Context:
         return ev.value, vec
 
 
-def put_on(ctx_handle, t: BTensor) -> "DTen":
-    """Import a tensor into the context behind a raw handle (e.g. the DMRG layer's)."""
+def _import(ctx_handle, t: BTensor):
     meta, data = t.to_flat()
     if data.size == 0:
-        data = np.zeros(1)
+        data = np.zeros(2)
     h = _vp()
-    check(lib().mps2_ten_import(ctx_handle, meta.ctypes.data_as(_i32p), data.ctypes.data_as(_dp), C.byref(h)))
-    return DTen(h, None)
+    f = lib().mps2_ten_import_complex if t.is_complex else lib().mps2_ten_import
+    check(f(ctx_handle, meta.ctypes.data_as(_i32p), data.ctypes.data_as(_dp), C.byref(h)))
+    return h
+
+
+def put_on(ctx_handle, t: BTensor) -> "DTen":
+    """Import a tensor into the context behind a raw handle (e.g. the DMRG layer's)."""
+    return DTen(_import(ctx_handle, t), None)
 
 
 class DTen:
@@ -173,7 +175,12 @@ class DTen:
         meta = np.zeros(nm.value, dtype=np.int32)
         data = np.zeros(max(nd.value, 1))
         check(lib().mps2_ten_export(self.h, meta.ctypes.data_as(_i32p), data.ctypes.data_as(_dp)))
-        return BTensor.from_flat(meta, data[:nd.value])
+        return BTensor.from_flat(meta, data[:nd.value], self.is_complex())
+
+    def is_complex(self) -> bool:
+        v = C.c_int()
+        check(lib().mps2_ten_is_complex(self.h, C.byref(v)))
+        return bool(v.value)
 
     def copy(self):
         h = _vp()

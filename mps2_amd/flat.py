@@ -8,7 +8,8 @@ two_site_update_finite_vmps_impl.h:117-129), is exchanged as
   nsect, then per sector qn[qn_width] and dim; then per block the sector
   coordinate of every leg.  Blocks are sorted lexicographically.
 * ``data`` (float64): the blocks in ``meta`` order, each dense row-major
-  over its sector dimensions.
+  over its sector dimensions; complex tensors (GQTEN_Complex) store each
+  element as two float64 (real, imaginary), interleaved.
 
 The divergence of a block is sum(qn of OUT legs) - sum(qn of IN legs).
 This module is plain numpy: it builds inputs, densifies outputs and
@@ -57,6 +58,10 @@ class BTensor:
     def rank(self):
         return len(self.legs)
 
+    @property
+    def is_complex(self) -> bool:
+        return any(np.iscomplexobj(b) for b in self.blocks.values())
+
     def block_shape(self, c):
         return tuple(self.legs[i].sectors[c[i]][1] for i in range(self.rank))
 
@@ -77,12 +82,15 @@ class BTensor:
         coords = sorted(self.blocks)
         for c in coords:
             meta += list(c)
-        data = [np.asarray(self.blocks[c], dtype=np.float64).ravel() for c in coords]
+        if self.is_complex:
+            data = [np.asarray(self.blocks[c], dtype=np.complex128).ravel().view(np.float64) for c in coords]
+        else:
+            data = [np.asarray(self.blocks[c], dtype=np.float64).ravel() for c in coords]
         data = np.concatenate(data) if data else np.zeros(0)
         return np.asarray(meta, dtype=np.int32), np.ascontiguousarray(data, dtype=np.float64)
 
     @staticmethod
-    def from_flat(meta, data) -> "BTensor":
+    def from_flat(meta, data, is_complex=False) -> "BTensor":
         meta = [int(x) for x in meta]
         p = 0
         rank, qw, nblk = meta[0], meta[1], meta[2]
@@ -104,17 +112,21 @@ class BTensor:
             coords.append(tuple(meta[p:p + rank]))
             p += rank
         off = 0
+        w = 2 if is_complex else 1
         for c in coords:
             shp = t.block_shape(c)
             n = int(np.prod(shp)) if shp else 1
-            t.blocks[c] = np.asarray(data[off:off + n], dtype=np.float64).reshape(shp).copy()
-            off += n
+            x = np.asarray(data[off:off + w * n], dtype=np.float64)
+            if is_complex:
+                x = x.view(np.complex128)
+            t.blocks[c] = x.reshape(shp).copy()
+            off += w * n
         return t
 
     # ------------------------------------------------------------ helpers
     def dense(self) -> np.ndarray:
         shp = tuple(l.dim for l in self.legs)
-        out = np.zeros(shp)
+        out = np.zeros(shp, dtype=np.complex128 if self.is_complex else np.float64)
         offs = [l.offsets() for l in self.legs]
         for c, b in self.blocks.items():
             sl = tuple(slice(offs[i][c[i]], offs[i][c[i]] + b.shape[i]) for i in range(self.rank))
@@ -138,10 +150,18 @@ class BTensor:
 
     def dag(self) -> "BTensor":
         return BTensor([l.inverse() for l in self.legs], self.qw,
-                       {c: b.copy() for c, b in self.blocks.items()})
+                       {c: np.conj(b).copy() for c, b in self.blocks.items()})
 
     def norm(self) -> float:
-        return float(np.sqrt(sum(float(np.sum(b * b)) for b in self.blocks.values())))
+        return float(np.sqrt(sum(float(np.sum(np.abs(b) ** 2)) for b in self.blocks.values())))
+
+    @staticmethod
+    def random_complex(legs, div, rng, qw=1, scale=1.0) -> "BTensor":
+        t = BTensor(list(legs), qw)
+        for c in t.allowed_coords(div):
+            shp = t.block_shape(c)
+            t.blocks[c] = scale * (rng.uniform(-1.0, 1.0, size=shp) + 1j * rng.uniform(-1.0, 1.0, size=shp))
+        return t
 
     def nnz(self) -> int:
         return sum(b.size for b in self.blocks.values())

@@ -40,8 +40,12 @@ struct Leg {
   }
 };
 
+// A complex tensor (gqten's GQTEN_Complex) stores every element as two
+// doubles (real, imaginary), interleaved: its block vectors are twice the
+// block size.
 struct Ten {
   int qw = 1;
+  bool cplx = false;
   std::vector<Leg> legs;
   std::map<Coord, std::vector<double>> blk;   // sorted lexicographically
   int rank() const { return (int)legs.size(); }
@@ -130,8 +134,9 @@ void permute(const double *src, const std::vector<int64_t> &dims, const std::vec
 }
 
 // ---------------------------------------------------------------- I/O
-Ten *import_ten(const int32_t *m, const double *data) {
+Ten *import_ten(const int32_t *m, const double *data, bool cplx = false) {
   auto t = std::make_unique<Ten>();
+  t->cplx = cplx;
   int p = 0;
   int rank = m[p++];
   t->qw = m[p++];
@@ -154,7 +159,7 @@ Ten *import_ten(const int32_t *m, const double *data) {
   }
   int64_t off = 0;
   for (int b = 0; b < nblk; ++b) {
-    int64_t sz = t->blk_size(coords[b]);
+    int64_t sz = t->blk_size(coords[b]) * (cplx ? 2 : 1);
     t->blk[coords[b]] = std::vector<double>(data + off, data + off + sz);
     off += sz;
   }
@@ -204,8 +209,112 @@ Leg inverse(const Leg &l) {
 // free legs of b; contracted legs must be mutually inverse indexes.  Output
 // blocks are exactly those that receive a block-pair contribution.  Called at
 // lanczos_solver_impl.h:231-234 and two_site_update_finite_vmps_impl.h:118-317.
+// Complex form of a tensor (a real one gets zero imaginary parts).
+Ten promote(const Ten &a) {
+  if (a.cplx) return a;
+  Ten r;
+  r.qw = a.qw;
+  r.cplx = true;
+  r.legs = a.legs;
+  for (auto &kv : a.blk) {
+    std::vector<double> v(2 * kv.second.size(), 0.0);
+    for (size_t e = 0; e < kv.second.size(); ++e) v[2 * e] = kv.second[e];
+    r.blk[kv.first] = std::move(v);
+  }
+  return r;
+}
+
+// C[M,N] += A[M,K] * B[K,N] in complex arithmetic (interleaved doubles).
+void zgemm_acc(int64_t M, int64_t N, int64_t K, const double *A, const double *B, double *C) {
+#pragma omp parallel for schedule(static) if (M * N * K > 200000)
+  for (int64_t i = 0; i < M; ++i)
+    for (int64_t k = 0; k < K; ++k) {
+      const double ar = A[2 * (i * K + k)], ai = A[2 * (i * K + k) + 1];
+      const double *b = B + 2 * k * N;
+      double *c = C + 2 * i * N;
+      for (int64_t j = 0; j < N; ++j) {
+        c[2 * j] += ar * b[2 * j] - ai * b[2 * j + 1];
+        c[2 * j + 1] += ar * b[2 * j + 1] + ai * b[2 * j];
+      }
+    }
+}
+
+// Permute a dense row-major complex block (the real permute on (re, im)
+// pairs: a trailing axis of extent 2 that stays last).
+void zpermute(const double *src, const std::vector<int64_t> &dims, const std::vector<int> &perm, double *dst) {
+  std::vector<int64_t> d2(dims);
+  d2.push_back(2);
+  std::vector<int> p2(perm);
+  p2.push_back((int)dims.size());
+  permute(src, d2, p2, dst);
+}
+
+Ten contract(const Ten &a, const Ten &b, const std::vector<int> &ax, const std::vector<int> &bx);
+
+// gqten Contract of complex tensors (a real operand enters as complex).
+Ten contract_complex(const Ten &a0, const Ten &b0, const std::vector<int> &ax, const std::vector<int> &bx) {
+  const Ten a = promote(a0), b = promote(b0);
+  int ra = a.rank(), rb = b.rank();
+  std::vector<char> a_c(ra, 0), b_c(rb, 0);
+  for (size_t i = 0; i < ax.size(); ++i) {
+    if (!(a.legs[ax[i]] == inverse(b.legs[bx[i]])))
+      throw std::runtime_error("contract: contracted indexes are not inverse of each other");
+    a_c[ax[i]] = 1;
+    b_c[bx[i]] = 1;
+  }
+  std::vector<int> afree, bfree;
+  for (int i = 0; i < ra; ++i)
+    if (!a_c[i]) afree.push_back(i);
+  for (int i = 0; i < rb; ++i)
+    if (!b_c[i]) bfree.push_back(i);
+  Ten out;
+  out.qw = a.qw;
+  out.cplx = true;
+  for (int i : afree) out.legs.push_back(a.legs[i]);
+  for (int i : bfree) out.legs.push_back(b.legs[i]);
+  std::vector<int> aperm(afree), bperm(bx);
+  aperm.insert(aperm.end(), ax.begin(), ax.end());
+  bperm.insert(bperm.end(), bfree.begin(), bfree.end());
+  std::map<Coord, std::vector<const std::pair<const Coord, std::vector<double>> *>> bgrp;
+  for (auto &kv : b.blk) {
+    Coord k;
+    for (int i : bx) k.push_back(kv.first[i]);
+    bgrp[k].push_back(&kv);
+  }
+  std::vector<double> at, bt;
+  for (auto &ka : a.blk) {
+    Coord k;
+    for (int i : ax) k.push_back(ka.first[i]);
+    auto it = bgrp.find(k);
+    if (it == bgrp.end()) continue;
+    std::vector<int64_t> adims(ra);
+    for (int i = 0; i < ra; ++i) adims[i] = a.legs[i].dim[ka.first[i]];
+    int64_t M = 1, K = 1;
+    for (int i : afree) M *= adims[i];
+    for (int i : ax) K *= adims[i];
+    at.resize(2 * M * K);
+    zpermute(ka.second.data(), adims, aperm, at.data());
+    for (auto *pb : it->second) {
+      std::vector<int64_t> bdims(rb);
+      for (int i = 0; i < rb; ++i) bdims[i] = b.legs[i].dim[pb->first[i]];
+      int64_t N = 1;
+      for (int i : bfree) N *= bdims[i];
+      bt.resize(2 * K * N);
+      zpermute(pb->second.data(), bdims, bperm, bt.data());
+      Coord oc;
+      for (int i : afree) oc.push_back(ka.first[i]);
+      for (int i : bfree) oc.push_back(pb->first[i]);
+      auto &ob = out.blk[oc];
+      if (ob.empty()) ob.assign(2 * M * N, 0.0);
+      zgemm_acc(M, N, K, at.data(), bt.data(), ob.data());
+    }
+  }
+  return out;
+}
+
 Ten contract(const Ten &a, const Ten &b, const std::vector<int> &ax, const std::vector<int> &bx) {
   if (ax.size() != bx.size()) throw std::runtime_error("contract: axes size mismatch");
+  if (a.cplx || b.cplx) return contract_complex(a, b, ax, bx);
   int ra = a.rank(), rb = b.rank();
   std::vector<char> a_c(ra, 0), b_c(rb, 0);
   for (size_t i = 0; i < ax.size(); ++i) {
@@ -266,9 +375,14 @@ Ten contract(const Ten &a, const Ten &b, const std::vector<int> &ax, const std::
   return out;
 }
 
-Ten dag(const Ten &a) {   // real tensors: conjugation is the identity
+// gqten Dag: inverse indexes and complex conjugate data (the identity on
+// real data), lanczos_solver_impl.h:135.
+Ten dag(const Ten &a) {
   Ten r = a;
   for (auto &l : r.legs) l.dir = 1 - l.dir;
+  if (r.cplx)
+    for (auto &kv : r.blk)
+      for (size_t e = 1; e < kv.second.size(); e += 2) kv.second[e] = -kv.second[e];
   return r;
 }
 
@@ -292,6 +406,8 @@ double normalize(Ten &a) {
 // out = beta*out + sum_i c_i t_i over the union of block sets
 // (gqten LinearCombine, lanczos_solver_impl.h:151-158,173,215).
 void lincomb(const std::vector<double> &c, const std::vector<const Ten *> &ts, double beta, Ten &out) {
+  for (auto *t : ts)
+    if (t->cplx != out.cplx) throw std::runtime_error("lincomb: mixed real and complex tensors");
   for (auto &kv : out.blk)
     for (double &v : kv.second) v *= beta;
   for (size_t i = 0; i < ts.size(); ++i) {
@@ -303,7 +419,10 @@ void lincomb(const std::vector<double> &c, const std::vector<const Ten *> &ts, d
   }
 }
 
+// Re <a|b> = Re sum conj(a) b: for complex tensors the dot product of the
+// interleaved doubles.
 double dot(const Ten &a, const Ten &b) {
+  if (a.cplx != b.cplx) return dot(promote(a), promote(b));
   double s = 0;
   for (auto &kv : a.blk) {
     auto it = b.blk.find(kv.first);
@@ -490,6 +609,102 @@ void svd_jacobi(int64_t m, int64_t n, const double *A, std::vector<double> &U, s
   }
 }
 
+// Complex thin SVD A (m x n, row-major, interleaved) = U diag(s) V^H by
+// one-sided cyclic Jacobi on the columns of the tall orientation W (A, or A^H
+// when m < n): for a column pair with g = sum conj(w_p) w_q = |g| e^{i phi},
+// the real rotation of (w_p, e^{-i phi} w_q) orthogonalises them (the
+// Hermitian Jacobi step); the rotations accumulate X with W X = Y diag(s).
+// U = Y, V = X for A; U = X, V = Y for A^H.  r = min(m, n) values, descending.
+void svd_jacobi_c(int64_t m, int64_t n, const double *A, std::vector<double> &U, std::vector<double> &s,
+                  std::vector<double> &V) {
+  bool tr = m < n;
+  int64_t rows = tr ? n : m, cols = tr ? m : n;
+  std::vector<double> W(2 * (size_t)cols * rows), X(2 * (size_t)cols * cols, 0.0);
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t j = 0; j < n; ++j) {
+      const double re = A[2 * (i * n + j)], im = A[2 * (i * n + j) + 1];
+      if (!tr) {
+        W[2 * ((size_t)j * rows + i)] = re;
+        W[2 * ((size_t)j * rows + i) + 1] = im;
+      } else {   // W = A^H: column i holds conj(row i of A)
+        W[2 * ((size_t)i * rows + j)] = re;
+        W[2 * ((size_t)i * rows + j) + 1] = -im;
+      }
+    }
+  for (int64_t j = 0; j < cols; ++j) X[2 * ((size_t)j * cols + j)] = 1.0;
+  const double tol = 1e-15;
+  auto rot = [](double *u, double *v, int64_t len, double c, double sn, double er, double ei) {
+    // u' = c u - sn e^{-i phi} v ;  v' = sn e^{i phi} u + c v   (e = er + i ei)
+    for (int64_t i = 0; i < len; ++i) {
+      const double ur = u[2 * i], ui = u[2 * i + 1], vr = v[2 * i], vi = v[2 * i + 1];
+      const double evr = er * vr + ei * vi, evi = er * vi - ei * vr;   // conj(e) v
+      const double eur = er * ur - ei * ui, eui = er * ui + ei * ur;   // e u
+      u[2 * i] = c * ur - sn * evr;
+      u[2 * i + 1] = c * ui - sn * evi;
+      v[2 * i] = sn * eur + c * vr;
+      v[2 * i + 1] = sn * eui + c * vi;
+    }
+  };
+  for (int sweep = 0; sweep < 80; ++sweep) {
+    int64_t nrot = 0;
+    for (int64_t p = 0; p < cols - 1; ++p) {
+      for (int64_t q = p + 1; q < cols; ++q) {
+        double *wp = &W[2 * (size_t)p * rows], *wq = &W[2 * (size_t)q * rows];
+        double al = 0, be = 0, gr = 0, gi = 0;
+        for (int64_t i = 0; i < rows; ++i) {
+          const double pr = wp[2 * i], pi = wp[2 * i + 1], qr = wq[2 * i], qi = wq[2 * i + 1];
+          al += pr * pr + pi * pi;
+          be += qr * qr + qi * qi;
+          gr += pr * qr + pi * qi;   // g = sum conj(w_p) w_q
+          gi += pr * qi - pi * qr;
+        }
+        const double g = std::sqrt(gr * gr + gi * gi);
+        if (al == 0.0 || be == 0.0) continue;
+        if (g <= tol * std::sqrt(al * be)) continue;
+        ++nrot;
+        const double zeta = (be - al) / (2.0 * g);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double c = 1.0 / std::sqrt(1.0 + t * t), sn = c * t;
+        rot(wp, wq, rows, c, sn, gr / g, gi / g);
+        rot(&X[2 * (size_t)p * cols], &X[2 * (size_t)q * cols], cols, c, sn, gr / g, gi / g);
+      }
+    }
+    if (nrot == 0) break;
+  }
+  std::vector<double> nrm(cols);
+  for (int64_t j = 0; j < cols; ++j) {
+    double a = 0;
+    for (int64_t i = 0; i < rows; ++i) {
+      const double re = W[2 * ((size_t)j * rows + i)], im = W[2 * ((size_t)j * rows + i) + 1];
+      a += re * re + im * im;
+    }
+    nrm[j] = std::sqrt(a);
+  }
+  std::vector<int64_t> ord(cols);
+  for (int64_t j = 0; j < cols; ++j) ord[j] = j;
+  std::stable_sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) { return nrm[x] > nrm[y]; });
+  const int64_t r = cols;
+  s.assign(r, 0.0);
+  U.assign(2 * (size_t)m * r, 0.0);
+  V.assign(2 * (size_t)n * r, 0.0);
+  for (int64_t k = 0; k < r; ++k) {
+    const int64_t j = ord[k];
+    s[k] = nrm[j];
+    const double inv = nrm[j] > 1e-280 ? 1.0 / nrm[j] : 0.0;
+    // Y column = W column / s (tall side), X column (other side)
+    std::vector<double> &T = tr ? V : U;   // tall side: length rows
+    std::vector<double> &O = tr ? U : V;   // other side: length cols
+    for (int64_t i = 0; i < rows; ++i) {
+      T[2 * ((size_t)i * r + k)] = W[2 * ((size_t)j * rows + i)] * inv;
+      T[2 * ((size_t)i * r + k) + 1] = W[2 * ((size_t)j * rows + i) + 1] * inv;
+    }
+    for (int64_t i = 0; i < cols; ++i) {
+      O[2 * ((size_t)i * r + k)] = X[2 * ((size_t)j * cols + i)];
+      O[2 * ((size_t)i * r + k) + 1] = X[2 * ((size_t)j * cols + i) + 1];
+    }
+  }
+}
+
 // Truncation rule shared (as a specification, not as code) with the product:
 //  1. all singular values of all sectors, sorted by value descending, ties by
 //     (sector position, index in sector) ascending;
@@ -635,10 +850,11 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
     }
   const int sexp = gmax > 0.0 ? std::max(-1000, std::min(1000, std::ilogb(gmax))) : 0;
   const double pre = std::ldexp(1.0, -sexp), post = std::ldexp(1.0, sexp);
+  const int w = t.cplx ? 2 : 1;   // doubles per element
   std::vector<std::vector<double>> Us(ns), Ss(ns), Vs(ns);
   for (int q = 0; q < ns; ++q) {
     auto &s = secs[q];
-    std::vector<double> M((size_t)s.m * s.n, 0.0);
+    std::vector<double> M((size_t)w * s.m * s.n, 0.0);
     for (auto &kv : t.blk) {
       Coord rc(kv.first.begin(), kv.first.begin() + ldims), cc(kv.first.begin() + ldims, kv.first.end());
       auto ir = s.rows.find(rc);
@@ -649,7 +865,9 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
       for (int i = 0; i < ldims; ++i) bm *= t.legs[i].dim[kv.first[i]];
       for (int i = ldims; i < r; ++i) bn *= t.legs[i].dim[kv.first[i]];
       for (int64_t i = 0; i < bm; ++i)
-        for (int64_t j = 0; j < bn; ++j) M[(size_t)(ir->second + i) * s.n + ic->second + j] = kv.second[i * bn + j] * pre;
+        for (int64_t j = 0; j < bn; ++j)
+          for (int h = 0; h < w; ++h)
+            M[w * ((size_t)(ir->second + i) * s.n + ic->second + j) + h] = kv.second[w * (i * bn + j) + h] * pre;
     }
     // each sector additionally at its own scale 2^-e_q (exact): sectors of
     // negligible weight would otherwise underflow in the sums of squares
@@ -657,7 +875,8 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
     for (double v : M) mq = std::max(mq, std::fabs(v));
     const int eq = mq > 0.0 ? std::max(-1000, std::min(1000, std::ilogb(mq))) : 0;
     for (double &v : M) v = std::ldexp(v, -eq);
-    svd_jacobi(s.m, s.n, M.data(), Us[q], Ss[q], Vs[q]);
+    if (t.cplx) svd_jacobi_c(s.m, s.n, M.data(), Us[q], Ss[q], Vs[q]);
+    else svd_jacobi(s.m, s.n, M.data(), Us[q], Ss[q], Vs[q]);
     for (double &v : Ss[q]) v = std::ldexp(v, eq);
   }
   // Below 1e-11 s_max (s_max over all sectors; weights below 1e-22) FP64
@@ -689,6 +908,7 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
   S = Ten();
   Vt = Ten();
   U.qw = S.qw = Vt.qw = t.qw;
+  U.cplx = Vt.cplx = t.cplx;
   for (int i = 0; i < ldims; ++i) U.legs.push_back(t.legs[i]);
   U.legs.push_back(knew);
   S.legs.push_back(inverse(knew));
@@ -704,9 +924,10 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
       c.push_back(secpos[q]);
       int64_t bm = 1;
       for (int i = 0; i < ldims; ++i) bm *= t.legs[i].dim[kv.first[i]];
-      std::vector<double> b((size_t)bm * kq);
+      std::vector<double> b((size_t)w * bm * kq);
       for (int64_t i = 0; i < bm; ++i)
-        for (int64_t k = 0; k < kq; ++k) b[i * kq + k] = Us[q][(size_t)(kv.second + i) * rq + k];
+        for (int64_t k = 0; k < kq; ++k)
+          for (int h = 0; h < w; ++h) b[w * (i * kq + k) + h] = Us[q][w * ((size_t)(kv.second + i) * rq + k) + h];
       U.blk[c] = std::move(b);
     }
     for (auto &kv : s.cols) {
@@ -715,9 +936,11 @@ void svd(const Ten &t, int ldims, const QNv &ldiv, double trunc_err, int64_t dmi
       c.insert(c.end(), kv.first.begin(), kv.first.end());
       int64_t bn = 1;
       for (int i = ldims; i < r; ++i) bn *= t.legs[i].dim[kv.first[i - ldims]];
-      std::vector<double> b((size_t)kq * bn);
+      std::vector<double> b((size_t)w * kq * bn);
       for (int64_t k = 0; k < kq; ++k)
-        for (int64_t j = 0; j < bn; ++j) b[k * bn + j] = Vs[q][(size_t)(kv.second + j) * rq + k];
+        for (int64_t j = 0; j < bn; ++j)
+          for (int h = 0; h < w; ++h)   // Vt = V^H: conjugate
+            b[w * (k * bn + j) + h] = (h ? -1.0 : 1.0) * Vs[q][w * ((size_t)(kv.second + j) * rq + k) + h];
       Vt.blk[c] = std::move(b);
     }
     std::vector<double> sd((size_t)kq * kq, 0.0);
@@ -764,7 +987,8 @@ LanczosOut lanczos(const Ten *L, const Ten *W1, const Ten *W2, const Ten *R, con
   std::vector<Ten> bases;
   bases.reserve(max_iter);
   std::vector<double> a(max_iter, 0.0), b(max_iter, 0.0), N(max_iter, 0.0);
-  bases.push_back(init);
+  const bool cz = init.cplx || (L && L->cplx) || W1->cplx || W2->cplx || (R && R->cplx);
+  bases.push_back(cz ? promote(init) : init);
   normalize(bases[0]);
   Ten last = heff_mul(L, W1, W2, R, bases[0], where);
   a[0] = dot(last, bases[0]);
@@ -789,6 +1013,7 @@ LanczosOut lanczos(const Ten *L, const Ten *W1, const Ten *W2, const Ten *R, con
       if (tridiag_lowest((int)m, a.data(), b.data(), &ev, &z)) throw std::runtime_error("?stev error.");
       Ten gs;
       gs.qw = init.qw;
+      gs.cplx = bases[0].cplx;
       gs.legs = bases[0].legs;
       std::vector<const Ten *> ps;
       for (int64_t i = 0; i < m; ++i) ps.push_back(&bases[i]);
@@ -811,6 +1036,7 @@ LanczosOut lanczos(const Ten *L, const Ten *W1, const Ten *W2, const Ten *R, con
       tridiag_lowest((int)m + 1, a.data(), b.data(), &ev, &z);
       Ten gs;
       gs.qw = init.qw;
+      gs.cplx = bases[0].cplx;
       gs.legs = bases[0].legs;
       std::vector<const Ten *> ps;
       for (int64_t i = 0; i <= m; ++i) ps.push_back(&bases[i]);
@@ -1010,6 +1236,14 @@ oracle_ten oracle_ten_import(const int32_t *meta, const double *data) {
   guarded([&] { t = import_ten(meta, data); });
   return t;
 }
+
+oracle_ten oracle_ten_import_complex(const int32_t *meta, const double *data) {
+  Ten *t = nullptr;
+  guarded([&] { t = import_ten(meta, data, true); });
+  return t;
+}
+
+int oracle_ten_is_complex(oracle_ten t) { return T(t)->cplx ? 1 : 0; }
 
 int oracle_ten_export_sizes(oracle_ten t, int64_t *nmeta, int64_t *ndata) {
   export_sizes(*T(t), nmeta, ndata);

@@ -39,6 +39,10 @@ const char *oracle_last_error(void);
 
 /* Tensor import / export / free. */
 oracle_ten oracle_ten_import(const int32_t *meta, const double *data);
+/* Complex tensor: data holds (real, imaginary) pairs, interleaved; the
+ * operations below accept complex tensors (dot returns Re <a|b>). */
+oracle_ten oracle_ten_import_complex(const int32_t *meta, const double *data);
+int oracle_ten_is_complex(oracle_ten t);
 int oracle_ten_export_sizes(oracle_ten t, int64_t *nmeta, int64_t *ndata);
 int oracle_ten_export(oracle_ten t, int32_t *meta, double *data);
 void oracle_ten_free(oracle_ten t);

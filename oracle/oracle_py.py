@@ -27,6 +27,10 @@ def lib():
         vp = C.c_void_p
         L.oracle_ten_import.restype = vp
         L.oracle_ten_import.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.oracle_ten_import_complex.restype = vp
+        L.oracle_ten_import_complex.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+        L.oracle_ten_is_complex.restype = C.c_int
+        L.oracle_ten_is_complex.argtypes = [vp]
         L.oracle_ten_export_sizes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.oracle_ten_export.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_double)]
         L.oracle_ten_free.argtypes = [vp]
@@ -76,9 +80,9 @@ class OTen:
     def from_bt(t: BTensor) -> "OTen":
         meta, data = t.to_flat()
         if data.size == 0:
-            data = np.zeros(1)
-        return OTen(lib().oracle_ten_import(meta.ctypes.data_as(C.POINTER(C.c_int32)),
-                                            data.ctypes.data_as(C.POINTER(C.c_double))))
+            data = np.zeros(2)
+        f = lib().oracle_ten_import_complex if t.is_complex else lib().oracle_ten_import
+        return OTen(f(meta.ctypes.data_as(C.POINTER(C.c_int32)), data.ctypes.data_as(C.POINTER(C.c_double))))
 
     def to_bt(self) -> BTensor:
         nm, nd = C.c_int64(), C.c_int64()
@@ -87,7 +91,7 @@ class OTen:
         data = np.zeros(max(nd.value, 1))
         lib().oracle_ten_export(self.h, meta.ctypes.data_as(C.POINTER(C.c_int32)),
                                 data.ctypes.data_as(C.POINTER(C.c_double)))
-        return BTensor.from_flat(meta, data[:nd.value])
+        return BTensor.from_flat(meta, data[:nd.value], bool(lib().oracle_ten_is_complex(self.h)))
 
     def __del__(self):
         try:

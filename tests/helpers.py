@@ -252,3 +252,47 @@ def run_pair(model, sweeps, dmin, dmax, trunc, lerr, lmax=200, labs=None, enlarg
     g = model.dmrg(sweeps, dmin, dmax, trunc, lerr, lmax)
     o = O.dmrg(mps0, mpo, sweeps, dmin, dmax, trunc, lerr, lmax)
     return g, o
+
+
+# ------------------------------------------------------------------ complex (GQTEN_Complex) problems
+
+def herm_env_c(rng, ket: Leg, mpo: Leg, bra: Leg, scale=1.0):
+    """Complex environment, Hermitian in (ket, bra) for every MPO index."""
+    t = BTensor([ket, mpo, bra], 1)
+    for c in t.allowed_coords((0,)):
+        if c[0] != c[2]:
+            continue
+        shp = t.block_shape(c)
+        b = (rng.uniform(-1, 1, size=shp) + 1j * rng.uniform(-1, 1, size=shp)) * scale
+        t.blocks[c] = 0.5 * (b + np.conj(b.transpose(2, 1, 0)))
+    return t
+
+
+def herm_w_c(rng, bl: Leg, pin: Leg, pout: Leg, br: Leg):
+    """Complex MPO tensor, Hermitian in (p_in, p_out) for every bond pair."""
+    t = BTensor([bl, pin, pout, br], 1)
+    for c in t.allowed_coords((0,)):
+        if c[1] != c[2]:
+            continue
+        shp = t.block_shape(c)
+        b = rng.uniform(-1, 1, size=shp) + 1j * rng.uniform(-1, 1, size=shp)
+        t.blocks[c] = 0.5 * (b + np.conj(b.transpose(0, 2, 1, 3)))
+    return t
+
+
+def hermitian_cent_problem_c(rng, dl=(3, 4, 2), dr=(2, 5, 3), qns=(-2, 0, 2), wdims=(2, 3)):
+    """The complex twin of hermitian_cent_problem: H_eff = sum L_b (x) W1 (x)
+    W2 (x) R_b with every factor Hermitian in its (in, out) pair, so H_eff is
+    Hermitian; psi is complex with divergence 0."""
+    l_leg = bond(IN, dl, qns)
+    r_leg = bond(OUT, dr, qns)
+    p = spin_phys(OUT)
+    m1 = bond(OUT, wdims, (0,) * len(wdims))
+    m2 = bond(OUT, wdims, (0,) * len(wdims))
+    m3 = bond(OUT, wdims, (0,) * len(wdims))
+    L = herm_env_c(rng, l_leg.inverse(), m1, l_leg)
+    W1 = herm_w_c(rng, m1.inverse(), p.inverse(), p, m2)
+    W2 = herm_w_c(rng, m2.inverse(), p.inverse(), p, m3)
+    R = herm_env_c(rng, r_leg.inverse(), m3.inverse(), r_leg)
+    psi = BTensor.random_complex([l_leg, p, p, r_leg], (0,), rng)
+    return L, W1, W2, R, psi

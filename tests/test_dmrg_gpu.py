@@ -276,3 +276,20 @@ def test_cpp_spin_2d_cases(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=180)
     print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0 and "PASSED" in r.stdout
+
+
+def test_cpp_complex_cases(tmp_path):
+    """The reference's GQTEN_Complex sweep cases (test_two_site_update_finite_vmps.cc:168-182,
+    225-247, 285-303, 347-369, 374-484, 640-659, 704-786) as a C++ program over
+    the drop-in headers: complex halves of 1D Ising and Heisenberg, 2D
+    Heisenberg, Kitaev simple case, the complex t-J plaquette, the 2D Kitaev
+    complex case (3x4, D = 60, E0 = -4.57509167674 to 2e-10) and the Rashba
+    t-J case with imaginary couplings (E0 = -11.018692166942165 to 1e-10), each
+    at the reference's tolerance."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests", "cpp",
+                       "test_complex_sweeps")
+    assert os.path.exists(exe), "tests/cpp is not built (run __graft_entry__.build())"
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0 and "PASSED" in r.stdout

@@ -659,3 +659,124 @@ def test_narrow_outputs_vs_numpy(ctx, prec):
                 assert np.abs(g - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (nd, case)
             else:
                 assert (np.abs(g - ref) <= low_prec_bound(prec, 120) * bound + 1e-30).all(), (nd, case)
+
+
+# ------------------------------------------------------------------ complex (GQTEN_Complex)
+
+def _complex_cases(rng):
+    """Complex contractions covering every operand layout of the device
+    plan: contracted legs trailing or leading on A (A read in place or
+    permuted to rows), on B (B expanded from either layout), in the middle
+    (permutations), output blocks narrow (vector-ALU kernel) and wide
+    (matrix cores), and a real operand with a complex one."""
+    from tests.test_oracle import contract_case
+    out = []
+    for axes in ([[2], [0]], [[0], [0]], [[2, 1], [0, 1]], [[0, 2], [2, 0]]):
+        A, B = contract_case(rng, axes, BTensor.random_complex)
+        out.append((A, B, axes))
+    # wide outputs (matrix-core tiles) with several pairs per block
+    from mps2_amd.flat import Leg
+    m = Leg(IN, [((0,), 70), ((1,), 130)])
+    k1 = Leg(OUT, [((0,), 9), ((1,), 20)])
+    k2 = Leg(OUT, [((0,), 5), ((1,), 7)])
+    n = Leg(OUT, [((q,), 40 + 30 * q) for q in range(3)])
+    A = BTensor.random_complex([m, k1, k2], (1,), rng)
+    B = BTensor.random_complex([k1.inverse(), k2.inverse(), n], (0,), rng)
+    out.append((A, B, [[1, 2], [0, 1]]))
+    At = BTensor([k1, k2, m], 1, {(c[1], c[2], c[0]): b.transpose(1, 2, 0).copy() for c, b in A.blocks.items()})
+    out.append((At, B, [[0, 1], [0, 1]]))
+    Bt = BTensor([n, k1.inverse(), k2.inverse()], 1,
+                 {(c[2], c[0], c[1]): b.transpose(2, 0, 1).copy() for c, b in B.blocks.items()})
+    out.append((A, Bt, [[1, 2], [1, 2]]))
+    # real times complex (the MPS update's S Vt) and complex times real
+    out.append((BTensor.random(A.legs, (1,), rng), B, [[1, 2], [0, 1]]))
+    out.append((A, BTensor.random(B.legs, (0,), rng), [[1, 2], [0, 1]]))
+    return out
+
+
+def test_complex_contract_parity(ctx):
+    """Complex Contract on the device against the oracle's complex Contract
+    (itself pinned by numpy einsum): identical output block sets, values to
+    1e-12 relative, for every operand layout of the device plan."""
+    rng = np.random.default_rng(41)
+    for i, (A, B, axes) in enumerate(_complex_cases(rng)):
+        ref = O.contract(A, B, axes)
+        got = E.contract(ctx.put(A), ctx.put(B), axes).get()
+        assert got.is_complex, i
+        assert sorted(got.blocks) == sorted(ref.blocks), i
+        assert H.max_rel_diff(got, ref) < 1e-12, (i, H.max_rel_diff(got, ref))
+
+
+def test_complex_heff_and_lanczos_parity(ctx):
+    """Complex Hermitian effective Hamiltonian (every factor Hermitian): the
+    device matvec against the oracle's to 1e-12, and the device Lanczos
+    (MPS2_PREC_FP64) against the oracle's: same iteration count, energy to
+    1e-12 relative, ground states overlapping to 1 - 1e-10 in modulus; the
+    mixed precisions reject complex problems."""
+    rng = np.random.default_rng(43)
+    L, W1, W2, R, psi = H.hermitian_cent_problem_c(rng, dl=(5, 7, 4), dr=(4, 8, 5))
+    d = [ctx.put(x) for x in (L, W1, W2, R, psi)]
+    hp = E.heff_mul(*d, "cent").get()
+    ho = O.heff_mul(L, W1, W2, R, psi, "cent")
+    assert hp.is_complex and H.max_rel_diff(hp, ho) < 1e-12
+    it_o, e_o, gs_o = O.lanczos(L, W1, W2, R, psi, "cent", 1e-12, 200)
+    it_g, e_g, gs_g = E.lanczos(*d, "cent", 1e-12, 200, E.PREC_FP64)
+    assert it_g == it_o, (it_g, it_o)
+    assert abs(e_g - e_o) <= 1e-12 * abs(e_o), (e_g, e_o)
+    ov = np.vdot(gs_o.dense(), gs_g.get().dense())
+    assert abs(abs(ov) - 1.0) < 1e-10, ov
+    for prec in (E.PREC_MIXED, E.PREC_MIXED_BF16X3):
+        with pytest.raises(E.EngineError):
+            E.lanczos(*d, "cent", 1e-12, 200, prec)
+
+
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+def test_complex_svd_parity(ctx, orth):
+    """Complex truncated SVD on the device (one-sided Jacobi per sector)
+    against the oracle's complex SVD: identical new-bond sectors and kept
+    counts, singular values to 1e-12 of the largest, truncation error to
+    1e-12; the factors reconstruct the kept part, the orthonormal side is
+    unitary to 1e-12; untruncated, U S Vt = psi."""
+    rng = np.random.default_rng(47)
+    l0 = H.bond(IN, (7, 12, 9), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (8, 11, 6, 5), (0, 2, -2, 4))
+    psi = BTensor.random_complex([l0, p, p, r], (0,), rng)
+    dpsi = ctx.put(psi)
+    for dmax in (1000, 25, 9):
+        Uo, So, Vo, eo, Do = O.svd(psi, 2, (0,), 0.0, 1, dmax)
+        u, s, vt, eg, Dg = E.svd(dpsi, 2, (0,), 0.0, 1, dmax, orth)
+        Ug, Sg, Vg = u.get(), s.get(), vt.get()
+        assert Dg == Do and Ug.legs[-1].key() == Uo.legs[-1].key(), (dmax, Dg, Do)
+        so, sg = np.diag(So.dense()), np.diag(Sg.dense())
+        assert np.abs(so - sg).max() < 1e-12 * so.max(), dmax
+        assert abs(eg - eo) < 1e-12, dmax
+        ud = Ug.dense().reshape(-1, Dg)
+        vd = Vg.dense().reshape(Dg, -1)
+        if orth == E.ORTH_LEFT:
+            assert np.abs(ud.conj().T @ ud - np.eye(Dg)).max() < 1e-12
+        else:
+            assert np.abs(vd @ vd.conj().T - np.eye(Dg)).max() < 1e-12
+        rec = np.einsum("abk,kl,lcd->abcd", Ug.dense(), Sg.dense(), Vg.dense())
+        ref = np.einsum("abk,kl,lcd->abcd", Uo.dense(), So.dense(), Vo.dense())
+        assert np.abs(rec - ref).max() < 1e-11 * np.abs(ref).max(), dmax
+        if dmax == 1000:
+            assert np.abs(rec - psi.dense()).max() < 1e-12 * np.abs(psi.dense()).max()
+
+
+def test_complex_dag_and_dot(ctx):
+    """Dag of a complex tensor conjugates its data and inverts its indexes;
+    mps2_dot returns Re <a|b>; Normalize uses the complex 2-norm."""
+    rng = np.random.default_rng(53)
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (3, 4), (1, -1))
+    a = BTensor.random_complex([p, r], (0,), rng)
+    b = BTensor.random_complex([p, r], (0,), rng)
+    da, db = ctx.put(a), ctx.put(b)
+    g = da.dag().get()
+    assert np.array_equal(g.dense(), np.conj(a.dense()))
+    assert [l.dir for l in g.legs] == [1 - l.dir for l in a.legs]
+    assert abs(E.dot(da, db) - np.vdot(a.dense(), b.dense()).real) < 1e-13
+    n = da.normalize()
+    assert abs(n - np.linalg.norm(a.dense())) < 1e-13
+    assert abs(np.linalg.norm(da.get().dense()) - 1.0) < 1e-14

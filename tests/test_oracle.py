@@ -12,6 +12,26 @@ from oracle import oracle_py as O
 from tests import helpers as H
 
 
+def contract_case(rng, axes, make):
+    """Operands of one of the four axis patterns with charges that admit
+    blocks on both sides (odd charges on l, even on r, spin-1/2 p)."""
+    l0 = H.bond(IN, (3, 2, 4), (1, -1, 3))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (3, 2, 1), (0, 2, -2))
+    A = make([l0, p, r], (0,), rng)
+    if axes == [[2], [0]]:
+        legs = [r.inverse(), p, H.bond(OUT, (2, 3), (1, -1))]
+    elif axes == [[0], [0]]:
+        legs = [l0.inverse(), p, H.bond(OUT, (2, 3, 1), (0, 2, -2))]
+    elif axes == [[2, 1], [0, 1]]:
+        legs = [r.inverse(), p.inverse(), H.bond(OUT, (2, 3, 2), (1, -1, 3))]
+    else:
+        legs = [r.inverse(), H.bond(OUT, (2, 3), (1, 3)), l0.inverse()]
+    B = make(legs, (0,), rng)
+    assert len(A.blocks) >= 4 and len(B.blocks) >= 2
+    return A, B
+
+
 @pytest.mark.parametrize("axes,spec", [
     ([[2], [0]], "abc,cde->abde"),
     ([[0], [0]], "abc,ade->bcde"),
@@ -20,20 +40,10 @@ from tests import helpers as H
 ])
 def test_contract_matches_dense_einsum(axes, spec):
     rng = np.random.default_rng(3)
-    l0 = H.bond(IN, (3, 2, 4), (0, 2, -2))
-    p = H.spin_phys(OUT)
-    r = H.bond(OUT, (3, 2, 1), (1, -1, 3))
-    A = BTensor.random([l0, p, r], (1,), rng)
-    if axes == [[2], [0]]:
-        B = BTensor.random([r.inverse(), p, H.bond(OUT, (2, 3), (0, 2))], (0,), rng)
-    elif axes == [[0], [0]]:
-        B = BTensor.random([l0.inverse(), p, H.bond(OUT, (2, 3), (0, 2))], (0,), rng)
-    elif axes == [[2, 1], [0, 1]]:
-        B = BTensor.random([r.inverse(), p.inverse(), H.bond(OUT, (2, 3, 2), (0, 2, 4))], (0,), rng)
-    else:
-        B = BTensor.random([r.inverse(), H.bond(OUT, (2, 3), (1, 3)), l0.inverse()], (0,), rng)
+    A, B = contract_case(rng, axes, BTensor.random)
     C = O.contract(A, B, axes)
     ref = np.einsum(spec, A.dense(), B.dense())
+    assert len(C.blocks) >= 1 and np.abs(ref).max() > 0.1
     assert np.abs(C.dense() - ref).max() < 1e-13
     # output blocks: exactly the coordinates that receive a block-pair product
     for c, blk in C.blocks.items():
@@ -339,3 +349,96 @@ def test_u1xu1_contract_and_svd_against_dense():
     dense_s = np.linalg.svd(psi.dense().reshape(l0.dim * 4, -1), compute_uv=False)
     ours = np.sort(np.diag(S.dense()))[::-1]
     assert np.abs(ours - dense_s[:len(ours)]).max() < 1e-13
+
+
+# ------------------------------------------------------------------ complex (GQTEN_Complex)
+
+@pytest.mark.parametrize("axes,spec", [
+    ([[2], [0]], "abc,cde->abde"),
+    ([[0], [0]], "abc,ade->bcde"),
+    ([[2, 1], [0, 1]], "abc,cbe->ae"),
+    ([[0, 2], [2, 0]], "abc,cda->bd"),
+])
+def test_complex_contract_matches_dense_einsum(axes, spec):
+    """Complex Contract against numpy einsum on the dense complex operands
+    (the same four axis patterns as the real test), and a real operand with
+    a complex one (promoted, as the MPS update's real S times a complex Vt)."""
+    rng = np.random.default_rng(13)
+    A, B = contract_case(rng, axes, BTensor.random_complex)
+    legs = B.legs
+    C = O.contract(A, B, axes)
+    ref = np.einsum(spec, A.dense(), B.dense())
+    assert len(C.blocks) >= 1 and np.abs(ref.imag).max() > 0.1
+    assert C.is_complex and np.abs(C.dense() - ref).max() < 1e-13
+    Br = BTensor.random(legs, (0,), rng)
+    Cr = O.contract(A, Br, axes)
+    assert np.abs(Cr.dense() - np.einsum(spec, A.dense(), Br.dense())).max() < 1e-13
+
+
+def test_complex_svd_reconstructs_and_truncates():
+    """Complex truncated SVD: U S Vt = psi, U^H U = 1, Vt Vt^H = 1, singular
+    values of the QN-blocked matrix equal numpy's of the dense matrix, the
+    discarded weight is the truncation error, trunc_err drives D."""
+    rng = np.random.default_rng(15)
+    l0 = H.bond(IN, (3, 2, 4), (0, 2, -2))
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (3, 2, 1, 2), (0, 2, -2, 4))
+    psi = BTensor.random_complex([l0, p, p, r], (0,), rng)
+    U, S, Vt, err, D = O.svd(psi, 2, (0,), 0.0, 1000, 1000)
+    assert U.is_complex and Vt.is_complex and not S.is_complex
+    rec = np.einsum("abk,kl,lcd->abcd", U.dense(), S.dense(), Vt.dense())
+    assert np.abs(rec - psi.dense()).max() < 1e-13
+    u = U.dense().reshape(-1, U.dense().shape[-1])
+    assert np.abs(u.conj().T @ u - np.eye(u.shape[1])).max() < 1e-13
+    v = Vt.dense().reshape(Vt.dense().shape[0], -1)
+    assert np.abs(v @ v.conj().T - np.eye(v.shape[0])).max() < 1e-13
+    dense_s = np.linalg.svd(psi.dense().reshape(psi.dense().shape[0] * 2, -1), compute_uv=False)
+    ours = np.sort(np.diag(S.dense()))[::-1]
+    assert np.abs(ours - dense_s[:len(ours)]).max() < 1e-13
+    U5, S5, V5, err5, D5 = O.svd(psi, 2, (0,), 0.0, 5, 5)
+    assert D5 == 5
+    assert abs(err5 - (dense_s[5:] ** 2).sum() / (dense_s ** 2).sum()) < 1e-13
+    target = (dense_s[7:] ** 2).sum() / (dense_s ** 2).sum() * 1.0000001
+    _, _, _, e7, D7 = O.svd(psi, 2, (0,), target, 1, 1000)
+    assert D7 == 7 and e7 <= target
+    # a wide sector (fewer rows than columns) takes the conjugate-transpose route
+    Uw, Sw, Vw, _, _ = O.svd(psi, 1, (0,), 0.0, 1000, 1000)
+    recw = np.einsum("ak,kl,lbcd->abcd", Uw.dense(), Sw.dense(), Vw.dense())
+    assert np.abs(recw - psi.dense()).max() < 1e-13
+
+
+def test_complex_lanczos_matches_dense_eigh():
+    """Complex Hermitian H_eff (every factor Hermitian): H is Hermitian, the
+    oracle Lanczos ground energy is real, not below the dense lowest
+    eigenvalue, and equals the Rayleigh quotient of its ground state; the
+    Hermiticity invariant <y|Hx> = conj <x|Hy> holds for the matvec."""
+    rng = np.random.default_rng(17)
+    L, W1, W2, R, psi = H.hermitian_cent_problem_c(rng)
+    Hd = H.dense_heff_cent(L, W1, W2, R)
+    assert np.abs(Hd - Hd.conj().T).max() < 1e-12 and np.abs(Hd.imag).max() > 0.1
+    it, e, gs = O.lanczos(L, W1, W2, R, psi, "cent", 1e-12, 200)
+    x = gs.dense().ravel()
+    assert gs.is_complex and abs(np.linalg.norm(x) - 1) < 1e-10
+    assert abs((x.conj() @ Hd @ x).real - e) < 1e-8
+    # the ground state of the symmetry sector the start vector lies in
+    w = np.linalg.eigvalsh(Hd)
+    assert e >= w[0] - 1e-8
+    y = BTensor.random_complex(psi.legs, (0,), rng)
+    hx = O.heff_mul(L, W1, W2, R, psi, "cent").dense()
+    hy = O.heff_mul(L, W1, W2, R, y, "cent").dense()
+    lhs = np.vdot(y.dense(), hx)
+    rhs = np.conj(np.vdot(psi.dense(), hy))
+    assert abs(lhs - rhs) < 1e-11 * max(1, abs(lhs))
+
+
+def test_complex_dag_conjugates_and_dot_is_real_part():
+    rng = np.random.default_rng(19)
+    p = H.spin_phys(OUT)
+    r = H.bond(OUT, (2, 3), (1, -1))
+    a = BTensor.random_complex([p, r], (0,), rng)
+    b = BTensor.random_complex([p, r], (0,), rng)
+    A, B = O.OTen.from_bt(a), O.OTen.from_bt(b)
+    d = O.OTen(O.lib().oracle_dag(A.h)).to_bt()
+    assert np.abs(d.dense() - np.conj(a.dense())).max() == 0.0
+    assert [l.dir for l in d.legs] == [1 - l.dir for l in a.legs]
+    assert abs(O.lib().oracle_dot(A.h, B.h) - np.vdot(a.dense(), b.dense()).real) < 1e-14
