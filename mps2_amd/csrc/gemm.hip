@@ -505,10 +505,81 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 constexpr int kBf3BK = 32;
 constexpr int kBf3Row = 2 * kBf3BK + 8;   // bf16 per LDS row
 
-__device__ __forceinline__ void split_bf16(float x, __bf16 &hi, __bf16 &lo) {
-  hi = (__bf16)x;
-  lo = (__bf16)(x - (float)hi);
-}
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf4v_t __attribute__((ext_vector_type(4)));
+
+// Per-thread share of one 32-deep slab, as quads of four consecutive k of
+// one row of A (BM x 32) or one column of B (32 x BN): QA and QB quads per
+// thread, loaded from global memory as FP32 (out-of-range elements read as
+// zero) and stored to LDS split into BF16 hi and lo parts, one 8-byte write
+// per part and quad.  Operands whose k is contiguous in memory (A row-major,
+// B stored N x K) give 8 threads per row (a row's 32 k in one 128-byte run);
+// operands stored k-major (A as K x M, B as K x N) give consecutive threads
+// consecutive rows (each of the four loads coalesced along the row index).
+template <int BM, int BN, bool TA, bool TB>
+struct Bf3Slab {
+  static constexpr int BK = kBf3BK;
+  static constexpr int QA = BM * BK / 4 / kThreads, QB = BN * BK / 4 / kThreads;
+  f4v_t ra[QA], rb[QB];
+
+  // (row, quad) of this thread's j-th quad
+  template <int BR, bool KCONTIG>
+  static __device__ __forceinline__ void pos(int tid, int j, int &row, int &quad) {
+    if (KCONTIG) {
+      row = tid / 8 + j * (kThreads / 8);
+      quad = tid % 8;
+    } else {
+      row = tid % BR;
+      quad = tid / BR + j * (kThreads / BR);
+    }
+  }
+
+  __device__ __forceinline__ void load(const float *__restrict__ Ab, const float *__restrict__ Bb, int M, int N,
+                                       int K, int m0, int n0, int k0, int tid) {
+#pragma unroll
+    for (int j = 0; j < QA; ++j) {
+      int r, q;
+      pos<BM, !TA>(tid, j, r, q);
+      const int gm = m0 + r, gk = k0 + 4 * q;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        ra[j][e] = (gm < M && gk + e < K) ? (TA ? Ab[(int64_t)(gk + e) * M + gm] : Ab[(int64_t)gm * K + gk + e])
+                                          : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < QB; ++j) {
+      int r, q;
+      pos<BN, TB>(tid, j, r, q);
+      const int gn = n0 + r, gk = k0 + 4 * q;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        rb[j][e] = (gn < N && gk + e < K) ? (TB ? Bb[(int64_t)gn * K + gk + e] : Bb[(int64_t)(gk + e) * N + gn])
+                                          : 0.0f;
+    }
+  }
+
+  static __device__ __forceinline__ void put(__bf16 *row, int q, f4v_t x) {
+    const bf4v_t hi = __builtin_convertvector(x, bf4v_t);
+    const bf4v_t lo = __builtin_convertvector(x - __builtin_convertvector(hi, f4v_t), bf4v_t);
+    *reinterpret_cast<bf4v_t *>(row + 4 * q) = hi;
+    *reinterpret_cast<bf4v_t *>(row + BK + 4 * q) = lo;
+  }
+
+  __device__ __forceinline__ void store(__bf16 *as, __bf16 *bs, int tid) const {
+#pragma unroll
+    for (int j = 0; j < QA; ++j) {
+      int r, q;
+      pos<BM, !TA>(tid, j, r, q);
+      put(as + r * kBf3Row, q, ra[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < QB; ++j) {
+      int r, q;
+      pos<BN, TB>(tid, j, r, q);
+      put(bs + r * kBf3Row, q, rb[j]);
+    }
+  }
+};
 
 template <int BM, int BN, bool TA, bool TB, int TAG>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float *__restrict__ A,
@@ -519,7 +590,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
                                                                      const PairD *__restrict__ pairs, int ntiles,
                                                                      int sub) {
   using V4 = Mfma<float>::v4;
-  using SlabT = Slab<float, BM, BN, TA, TB, kBf3BK>;
+  using SlabT = Bf3Slab<BM, BN, TA, TB>;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int FM = WM / 16, FN = WN / 16;
   constexpr int RS = kBf3Row, BK = kBf3BK;
@@ -539,58 +610,6 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
   const int m0 = tile.m0, n0 = tile.n0;
   if (m0 >= M || n0 >= N || blk.pb >= blk.pe) return;
 
-  // registers -> LDS, element (m, k) of A at As[m RS + k] (hi) and
-  // As[m RS + BK + k] (lo); B (k, n) at Bs[n RS + k] likewise.  The thread's
-  // (row, k) pattern is the one Slab::load used.
-  auto store = [&](const SlabT &s, __bf16 *as, __bf16 *bs) {
-    if (!TA) {
-      constexpr int RSTEP = kThreads / BK;
-      const int kk = tid % BK;
-#pragma unroll
-      for (int i = 0; i < SlabT::NA; ++i) {
-        __bf16 h, l;
-        split_bf16(s.ra[i], h, l);
-        const int m = tid / BK + i * RSTEP;
-        as[m * RS + kk] = h;
-        as[m * RS + BK + kk] = l;
-      }
-    } else {
-      constexpr int KSTEP = kThreads / BM;
-      const int mm = tid % BM;
-#pragma unroll
-      for (int i = 0; i < SlabT::NA; ++i) {
-        __bf16 h, l;
-        split_bf16(s.ra[i], h, l);
-        const int k = tid / BM + i * KSTEP;
-        as[mm * RS + k] = h;
-        as[mm * RS + BK + k] = l;
-      }
-    }
-    if (!TB) {
-      constexpr int KSTEP = kThreads / BN;
-      const int nn = tid % BN;
-#pragma unroll
-      for (int i = 0; i < SlabT::NB; ++i) {
-        __bf16 h, l;
-        split_bf16(s.rb[i], h, l);
-        const int k = tid / BN + i * KSTEP;
-        bs[nn * RS + k] = h;
-        bs[nn * RS + BK + k] = l;
-      }
-    } else {
-      constexpr int RSTEP = kThreads / BK;
-      const int kk = tid % BK;
-#pragma unroll
-      for (int i = 0; i < SlabT::NB; ++i) {
-        __bf16 h, l;
-        split_bf16(s.rb[i], h, l);
-        const int n = tid / BK + i * RSTEP;
-        bs[n * RS + kk] = h;
-        bs[n * RS + BK + kk] = l;
-      }
-    }
-  };
-
   V4 acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -601,7 +620,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
   int p = blk.pb, k0 = 0;
   PairD pr = pairs[p];
   slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, 0, tid);
-  store(slab, As[0], Bs[0]);
+  slab.store(As[0], Bs[0], tid);
   __syncthreads();
   int cur = 0;
   const int frag_row = (lane & 15) * RS + 8 * (lane >> 4);
@@ -644,7 +663,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
     }
-    if (has_next) store(slab, As[cur ^ 1], Bs[cur ^ 1]);
+    if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
     __syncthreads();
     if (!has_next) break;
     cur ^= 1;
