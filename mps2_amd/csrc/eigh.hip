@@ -919,11 +919,7 @@ __global__ void gather_vec_kernel(const double *__restrict__ Z, int64_t zoff, in
 
 inline int nblk(int64_t n, int t = 256) { return (int)std::max<int64_t>((n + t - 1) / t, 1); }
 
-BufP upload_(Ctx *c, const void *h, size_t bytes) {
-  auto b = c->alloc(std::max<size_t>(bytes, 1));
-  if (bytes) MPS2_HIP(hipMemcpyAsync(b->p, h, bytes, hipMemcpyHostToDevice, c->stream));
-  return b;
-}
+BufP upload_(Ctx *c, const void *h, size_t bytes) { return c->upload(h, bytes); }
 
 template <typename T>
 T *dp(const BufP &b) {

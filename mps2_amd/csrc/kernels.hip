@@ -36,8 +36,55 @@ Ctx::Ctx(int dev) : device(dev) {
   dev_scalars = alloc(256 * sizeof(double));
 }
 
+namespace {
+constexpr size_t kPinMax = 256u << 20;   // pinned staging cap; larger uploads sync
+}
+
+BufP Ctx::upload(const void *h, size_t bytes) {
+  auto b = alloc(std::max<size_t>(bytes, 16));
+  if (!bytes) return b;
+  PinBlk *blk = nullptr;
+  for (auto &pb : pins) {
+    if (pb.cap < bytes) continue;
+    if (pb.pending) {
+      const hipError_t e = hipEventQuery(pb.ev);
+      if (e == hipErrorNotReady) continue;
+      if (e != hipSuccess) MPS2_HIP(e);
+      pb.pending = false;
+    }
+    if (!blk || pb.cap < blk->cap) blk = &pb;
+  }
+  if (!blk) {
+    size_t cap = 64u << 10;
+    while (cap < bytes) cap <<= 1;
+    if (pinned_bytes + cap > kPinMax) {
+      MPS2_HIP(hipMemcpyAsync(b->p, h, bytes, hipMemcpyHostToDevice, stream));
+      sync();
+      return b;
+    }
+    PinBlk nb;
+    nb.cap = cap;
+    MPS2_HIP(hipHostMalloc((void **)&nb.p, cap, hipHostMallocDefault));
+    MPS2_HIP(hipEventCreateWithFlags(&nb.ev, hipEventDisableTiming));
+    pinned_bytes += cap;
+    count("pinned_blocks");
+    pins.push_back(nb);
+    blk = &pins.back();
+  }
+  std::memcpy(blk->p, h, bytes);
+  MPS2_HIP(hipMemcpyAsync(b->p, blk->p, bytes, hipMemcpyHostToDevice, stream));
+  MPS2_HIP(hipEventRecord(blk->ev, stream));
+  blk->pending = true;
+  return b;
+}
+
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (auto &pb : pins) {
+    if (pb.ev) (void)hipEventDestroy(pb.ev);
+    if (pb.p) (void)hipHostFree(pb.p);
+  }
+  pins.clear();
   red_partial.reset();
   dev_scalars.reset();
   for (auto &kv : free_) (void)hipFree(kv.second);

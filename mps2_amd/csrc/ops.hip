@@ -18,14 +18,7 @@ namespace mps2 {
 
 namespace {
 
-BufP upload_bytes(Ctx *c, const void *host, size_t bytes) {
-  auto b = c->alloc(std::max<size_t>(bytes, 16));
-  if (bytes) {
-    MPS2_HIP(hipMemcpyAsync(b->p, host, bytes, hipMemcpyHostToDevice, c->stream));
-    c->sync();   // the host vector may be freed right after
-  }
-  return b;
-}
+BufP upload_bytes(Ctx *c, const void *host, size_t bytes) { return c->upload(host, bytes); }
 
 using PlanKey = std::tuple<Ctx *, const Meta *, const Meta *, std::vector<int>, std::vector<int>, const Meta *, bool>;
 

@@ -108,6 +108,22 @@ struct Ctx {
   // FP32 accumulation).  Set for the duration of a call by SplitScope.
   int f32_split = 0;
 
+  // Host-to-device uploads of small host arrays (descriptor lists, tile
+  // lists, scalars) go through pinned staging blocks, each reused once the
+  // copy recorded after its last use has completed: no stream sync per
+  // upload, so the device keeps running queued work while the host plans.
+  struct PinBlk {
+    char *p = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  std::vector<PinBlk> pins;
+  size_t pinned_bytes = 0;
+  // A device buffer holding `bytes` of host data h (copied to staging at once;
+  // the caller may free h on return).
+  BufP upload(const void *h, size_t bytes);
+
   explicit Ctx(int dev);
   ~Ctx();
   // Destroy on behalf of the owner: frees the plans and the cache now, and

@@ -86,14 +86,7 @@ double jacobi_rel_tol() {
   return v;
 }
 
-BufP upload(Ctx *c, const void *h, size_t bytes) {
-  auto b = c->alloc(std::max<size_t>(bytes, 16));
-  if (bytes) {
-    MPS2_HIP(hipMemcpyAsync(b->p, h, bytes, hipMemcpyHostToDevice, c->stream));
-    c->sync();
-  }
-  return b;
-}
+BufP upload(Ctx *c, const void *h, size_t bytes) { return c->upload(h, bytes); }
 
 // Grouped GEMM from host-side block / pair lists (not cached).
 void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *C, const std::vector<BlkD> &blks,

@@ -165,14 +165,7 @@ __global__ void cfac_kernel(const double *src, double *dst, const CFacD *desc, c
   }
 }
 
-BufP upload_c(Ctx *c, const void *h, size_t bytes) {
-  auto b = c->alloc(std::max<size_t>(bytes, 16));
-  if (bytes) {
-    MPS2_HIP(hipMemcpyAsync(b->p, h, bytes, hipMemcpyHostToDevice, c->stream));
-    c->sync();
-  }
-  return b;
-}
+BufP upload_c(Ctx *c, const void *h, size_t bytes) { return c->upload(h, bytes); }
 
 void run_cfac(Ctx *c, const double *src, double *dst, const std::vector<CFacD> &d, const std::vector<int32_t> &sel,
               const std::vector<double> &scl, int64_t max_elems) {
@@ -185,7 +178,6 @@ void run_cfac(Ctx *c, const double *src, double *dst, const std::vector<CFacD> &
   cfac_kernel<<<dim3(gx, (unsigned)d.size()), 256, 0, c->stream>>>(src, dst, static_cast<const CFacD *>(dd->p),
                                                                     static_cast<const int32_t *>(ds->p),
                                                                     static_cast<const double *>(dscl->p));
-  c->sync();   // the descriptor buffers above are released on return
 }
 
 }  // namespace
