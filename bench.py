@@ -144,12 +144,13 @@ def tune_summary():
                                 f"variant {r[3]}" for r in rows}))
 
 
-def measured_traffic(args):
-    """HBM bytes per launch of the roofline kernel, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled
-    by the same counter's reading for the calibration copy); None when no
-    summary exists for this configuration."""
-    path = os.path.join(ROOT, "profiles", "matvec_stage4_f32_j1j2_4x24_D4096_traffic.json")
+def measured_traffic(args, form="f32"):
+    """HBM bytes per launch of the roofline kernel of one arithmetic form
+    ("f32" or "bf16x3"), from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by the same counter's
+    reading for the calibration copy); None when no summary exists for this
+    configuration."""
+    path = os.path.join(ROOT, "profiles", f"matvec_stage4_{form}_j1j2_4x24_D4096_traffic.json")
     if args.dmax != 4096 or args.lx != 24 or args.width != 4 or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -343,6 +344,7 @@ def main():
     roofline_bf = dict(bound="mfma", achieved=round(dbf_ach, 3), peak=PEAK["bf16"], unit="TFLOP/s",
                        frac=round(dbf_ach / PEAK["bf16"], 4),
                        executed_frac=round(3 * dbf_ach / PEAK["bf16"], 4),
+                       traffic=measured_traffic(args, "bf16x3"),
                        kernel=f"grouped_gemm_bf3_kernel<..., 1> (stage {dbf['dom_stage'] + 1} of the centre-bond "
                               f"matvec, run as the tagged instance)",
                        flops_per_launch=dbf["dom_flops"], ms_per_launch=dbf["dom_ms"],
