@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Two-site DMRG sweep wall-time and H*psi TFLOP/s at D=4096, FP64 vs mixed.
+"""Two-site DMRG sweep wall-time and H*psi TFLOP/s at D=4096, FP64 vs mixed
+(FP32 MFMA, the headline) and mixed split-BF16x3.
 
 Workload: J1-J2 Heisenberg model on a width-4 cylinder (4x24 = 96 sites,
 periodic in y, snake ordering, J1 = 1, J2 = 0.5), U(1) Sz = 0, MPO from the
@@ -10,15 +11,16 @@ the mixed form (FP32 MFMA operands and accumulation): the operation whose
 TFLOP/s the metric quotes.  Flow (one GPU):
   1. ramp sweeps (untimed, FP64) with Dmax 256 -> 512 -> 1024 -> 2048 -> 4096,
      then a device checkpoint of the D = 4096 state;
-  2. one full FP64 sweep and one full mixed sweep from that checkpoint
-     (sweep wall-time, phases, energies and kept dimensions of both);
+  2. one full FP64, one full mixed and one full split-BF16x3 sweep from that
+     checkpoint (sweep wall-time, phases, energies and kept dimensions);
   3. the centre-bond operator bound once per precision; `--warmup` untimed
      steps, then exactly `--steps` timed steps (host clock, the stream
-     synchronised on both sides), FP64 and mixed;
+     synchronised on both sides), for each of the three forms;
   4. per-stage HIP-event timing of the matvec; the dominant grouped-GEMM
      launch as the roofline kernel;
-  5. FP64 vs mixed Lanczos from one random start at the centre bond, and one
-     GPU's share of an 8-way owner-computes split of the dominant stage;
+  5. FP64 vs both mixed Lanczos forms from one random start at the centre
+     bond, and one GPU's share of an 8-way owner-computes split of the
+     dominant stage;
   6. the CPU FP64 oracle on the host cores: the same centre-bond matvec and
      the full L=16 Heisenberg run (D=64, 4 sweeps) on all threads and on one
      thread.
