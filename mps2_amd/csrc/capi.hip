@@ -123,6 +123,7 @@ int mps2_ctx_reset_stats(mps2_ctx ctx) {
     ctx->c->gemm_flops = 0;
     ctx->c->gemm_launches = 0;
     ctx->c->peak_bytes = ctx->c->live_bytes;
+    ctx->c->flush_phases();
     ctx->c->counters.clear();
   });
 }
@@ -140,6 +141,7 @@ int mps2_ctx_counter(mps2_ctx ctx, const char *name, double *value) {
     need(ctx, "ctx");
     need(name, "name");
     need(value, "value");
+    ctx->c->flush_phases();
     auto it = ctx->c->counters.find(name);
     *value = it == ctx->c->counters.end() ? 0.0 : it->second;
   });

@@ -30,7 +30,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 
 #include "eigh.h"
@@ -979,10 +978,7 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
   k_fill(c, tauv, nv, 0.0);
   k_fill(c, dp<double>(bw0), nv * kPB, 0.0);  // read with zero weight before written
   k_fill(c, dp<double>(bw1), nv * kPB, 0.0);
-  auto clock_ms = [](std::chrono::steady_clock::time_point t0) {
-    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  };
-  const auto t_trd = std::chrono::steady_clock::now();
+  const hipEvent_t t_trd = c->mark();
   {
     double n3 = 0;
     for (int q = 0; q < nsec; ++q) n3 += (double)ns[q] * ns[q] * ns[q];
@@ -1040,11 +1036,10 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
                                 dp<PairD>(dgp), 64, 64);
     trd_tblock_kernel<<<(int)prs.size(), kNB, 0, c->stream>>>(SD, dp<PanelRef>(dpr), dp<double>(bg), tauv, Tb);
     trd_wt_kernel<<<nblk(nn), 256, 0, c->stream>>>(SD, nsec, VH, Tb, dp<double>(st->WT), nn);
-    c->sync();
   }
   c->count("eigh_trd_columns", maxn);
-  c->count("eigh_trd_ms", clock_ms(t_trd));
-  const auto t_dc = std::chrono::steady_clock::now();
+  const hipEvent_t t_dc = c->mark();
+  c->phase("eigh_trd_ms", t_trd, t_dc);
   // ---- 2. divide and conquer on (d, e)
   auto bl = c->alloc(nv * sizeof(double));
   DcBufs B{};
@@ -1134,16 +1129,17 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
                                 dp<PairD>(dgp), 64, 64);
     dc_rank_kernel<<<g256, 256, 0, c->stream>>>(SD, MS, B);
     dc_write_kernel<<<g4, 256, 0, c->stream>>>(SD, MS, B);
-    c->sync();  // keeps this level's descriptors alive until its kernels ran
+    // This level's descriptor buffers go back to the pool here; a later
+    // allocation reusing them is written in stream order, after these kernels.
   }
   c->count("eigh_dc_levels", levels);
+  c->phase("eigh_dc_ms", t_dc, c->mark());
   for (int q = 0; q < nsec; ++q) {
     res.evals[q].resize(ns[q]);
     MPS2_HIP(hipMemcpyAsync(res.evals[q].data(), B.lam + st->secs[q].nv, ns[q] * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
   }
   c->sync();
-  c->count("eigh_dc_ms", clock_ms(t_dc));
   for (int q = 0; q < nsec; ++q)
     for (double x : res.evals[q])
       if (!std::isfinite(x)) throw Error(kErrNumeric, "eigh: non-finite eigenvalue (input holds NaN or Inf)");
@@ -1154,7 +1150,7 @@ void eigh_vectors_all(Ctx *c, const EighResult &r, const std::vector<std::vector
                       const std::vector<int64_t> &eoff) {
   const EighState &st = *r.st;
   const int nsec = st.nsec;
-  const auto t_bt = std::chrono::steady_clock::now();
+  const hipEvent_t t_bt = c->mark();
   // E_q = Z_q[:, cols_q]
   std::vector<BufP> keep;
   std::vector<int64_t> xoff(nsec, 0);
@@ -1172,7 +1168,7 @@ void eigh_vectors_all(Ctx *c, const EighResult &r, const std::vector<std::vector
     maxpan = std::max(maxpan, (n + kNB - 1) / kNB);
   }
   if (maxpan == 0) {
-    c->sync();
+    c->phase("eigh_back_ms", t_bt, c->mark());
     return;
   }
   // E <- Q E, Q = prod_p (I - V_p T_p V_p^T), panels from the last: per panel
@@ -1218,9 +1214,7 @@ void eigh_vectors_all(Ctx *c, const EighResult &r, const std::vector<std::vector
     launch_grouped_gemm<double>(c, 1, 0, dp<double>(st.WT), X, E, dp<TileD>(dt) + t2b[p], t2e[p] - t2b[p],
                                 dp<BlkD>(db), dp<PairD>(dpp), 64, 64, 1);
   }
-  c->sync();
-  c->count("eigh_back_ms",
-           std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_bt).count());
+  c->phase("eigh_back_ms", t_bt, c->mark());
 }
 
 }  // namespace mps2

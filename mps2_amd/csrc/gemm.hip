@@ -136,6 +136,17 @@ struct Slab {
   }
 };
 
+// Number of a wavefront's 16-row (or 16-column) fragments that hold any row
+// (column) of the block, out of F, given the rows left from the wavefront's
+// first one (<= 0: none); read through readfirstlane so the count is a scalar
+// and the branches on it are uniform.  Fragments past it only multiply the
+// zero padding of an edge tile and are skipped.
+template <int F>
+__device__ __forceinline__ int live_frags(int rows_left) {
+  const int f = rows_left <= 0 ? 0 : min(F, (rows_left + 15) / 16);
+  return __builtin_amdgcn_readfirstlane(f);
+}
+
 // Four consecutive k values of one FP32 LDS row (16-byte aligned).
 struct Quad {
   float v[4];
@@ -185,6 +196,12 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+  // FP64: fragments beyond the block are skipped (wave-uniform scalar
+  // branches; measured faster).  FP32 multiplies them (the branches cost as
+  // much as the skipped products save there).
+  constexpr bool kSkip = sizeof(T) == 8;
+  const int fmv = kSkip ? live_frags<FM>(M - m0 - wm * WM) : FM;
+  const int fnv = kSkip ? live_frags<FN>(N - n0 - wn * WN) : FN;
 
   SlabT slab;
   int p = blk.pb, k0 = 0;
@@ -193,16 +210,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   slab.store(As[0], Bs[0], tid);
   __syncthreads();
   int cur = 0;
-  while (true) {
-    int np = p, nk0 = k0 + BK;
-    PairD npr = pr;
-    if (nk0 >= pr.K) {
-      np = p + 1;
-      nk0 = 0;
-      if (np < blk.pe) npr = pairs[np];
-    }
-    const bool has_next = np < blk.pe;
-    if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
+  // One slab's products.
+  auto multiply = [&]() {
     if constexpr (SlabT::kQ) {
       constexpr int SK = SlabT::SK;
       constexpr int KPL = BK / 4;   // k values per lane group and slab
@@ -220,7 +229,8 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
 #pragma unroll
           for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i].v[st], bf[j].v[st], acc[i][j]);
+            for (int j = 0; j < FN; ++j)
+              if (!kSkip || (i < fmv && j < fnv)) acc[i][j] = Mfma<T>::op(af[i].v[st], bf[j].v[st], acc[i][j]);
       }
     } else {
       constexpr int PA = SlabT::PA, PB = SlabT::PB;
@@ -237,9 +247,22 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
 #pragma unroll
         for (int i = 0; i < FM; ++i)
 #pragma unroll
-          for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
+          for (int j = 0; j < FN; ++j)
+            if (!kSkip || (i < fmv && j < fnv)) acc[i][j] = Mfma<T>::op(af[i], bf[j], acc[i][j]);
       }
     }
+  };
+  while (true) {
+    int np = p, nk0 = k0 + BK;
+    PairD npr = pr;
+    if (nk0 >= pr.K) {
+      np = p + 1;
+      nk0 = 0;
+      if (np < blk.pe) npr = pairs[np];
+    }
+    const bool has_next = np < blk.pe;
+    if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
+    multiply();
     if constexpr (NBUF == 2) {
       if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
       __syncthreads();

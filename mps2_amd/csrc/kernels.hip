@@ -80,6 +80,12 @@ BufP Ctx::upload(const void *h, size_t bytes) {
 
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
+  for (auto &ph : phases_pending)
+    for (hipEvent_t e : {ph.a, ph.b})
+      if (std::find(ev_free.begin(), ev_free.end(), e) == ev_free.end()) ev_free.push_back(e);
+  phases_pending.clear();
+  for (hipEvent_t e : ev_free) (void)hipEventDestroy(e);
+  ev_free.clear();
   for (auto &pb : pins) {
     if (pb.ev) (void)hipEventDestroy(pb.ev);
     if (pb.p) (void)hipHostFree(pb.p);
@@ -178,6 +184,38 @@ void Ctx::trim() {
 void Ctx::sync() {
   counters["syncs"] += 1;
   MPS2_HIP(hipStreamSynchronize(stream));
+  resolve_phases();
+}
+
+hipEvent_t Ctx::mark() {
+  hipEvent_t e;
+  if (!ev_free.empty()) {
+    e = ev_free.back();
+    ev_free.pop_back();
+  } else {
+    MPS2_HIP(hipEventCreate(&e));
+  }
+  MPS2_HIP(hipEventRecord(e, stream));
+  return e;
+}
+
+void Ctx::phase(const char *name, hipEvent_t a, hipEvent_t b) { phases_pending.push_back(Phase{name, a, b}); }
+
+void Ctx::resolve_phases() {
+  // every event recorded before the stream sync that precedes this call has
+  // completed
+  for (auto &ph : phases_pending) {
+    float ms = 0;
+    MPS2_HIP(hipEventElapsedTime(&ms, ph.a, ph.b));
+    counters[ph.name] += ms;
+    for (hipEvent_t e : {ph.a, ph.b})
+      if (std::find(ev_free.begin(), ev_free.end(), e) == ev_free.end()) ev_free.push_back(e);
+  }
+  phases_pending.clear();
+}
+
+void Ctx::flush_phases() {
+  if (!phases_pending.empty()) sync();
 }
 
 TenP ten_alloc(Ctx *c, MetaP m, Prec p, bool cplx) {

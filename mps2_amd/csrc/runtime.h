@@ -120,6 +120,21 @@ struct Ctx {
   };
   std::vector<PinBlk> pins;
   size_t pinned_bytes = 0;
+
+  // Phase clocks on the device timeline: mark() records a timing event on
+  // the stream, phase(name, a, b) adds the time between two marks to
+  // counters[name] once both have completed (resolved at the next sync(), so
+  // timing a phase costs no sync of its own; flush_phases() forces it).
+  hipEvent_t mark();
+  void phase(const char *name, hipEvent_t a, hipEvent_t b);
+  void flush_phases();
+  struct Phase {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Phase> phases_pending;
+  std::vector<hipEvent_t> ev_free;
+  void resolve_phases();
   // A device buffer holding `bytes` of host data h (copied to staging at once;
   // the caller may free h on return).
   BufP upload(const void *h, size_t bytes);

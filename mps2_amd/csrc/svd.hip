@@ -101,7 +101,6 @@ void raw_gemm(Ctx *c, int la, int lb, const double *A, const double *B, double *
   auto dp = upload(c, pairs.data(), pairs.size() * sizeof(PairD));
   launch_grouped_gemm<double>(c, la, lb, A, B, C, static_cast<const TileD *>(dt->p), (int)tiles.size(),
                               static_cast<const BlkD *>(db->p), static_cast<const PairD *>(dp->p), 64, 64, sub);
-  c->sync();
 }
 
 // Largest |x| over n device doubles (NaN propagates); host value.
@@ -896,16 +895,9 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     // error theta^2 in the refined values; the tail's own maximum, ~1e-10 top,
     // then resolves a cut at 1e-8 s_max in one round.
     constexpr double kAccept = 1e-7, kTail = 1e-10, kMargin = 1e3;
-    // phase clocks of the refinement (engine counters svd_refine_{vec,eigh}_ms)
-    double vec_ms = 0, eigh_ms = 0;
-    std::chrono::steady_clock::time_point tv0;
-    auto synced_now = [&] {
-      c->sync();
-      return std::chrono::steady_clock::now();
-    };
-    auto ms_since = [](std::chrono::steady_clock::time_point t0) {
-      return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    };
+    // phase clocks of the refinement (engine counters svd_refine_{vec,eigh}_ms),
+    // device-timeline events
+    hipEvent_t tv0 = nullptr;
     std::vector<double> lam_top(ns, 0.0);  // scale of each sector's current problem
     for (int q = 0; q < ns; ++q)
       for (double x : evals[q]) lam_top[q] = std::max(lam_top[q], x);
@@ -956,7 +948,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
           tot += (int64_t)gn[q] * rn[i];
         }
         Esub = c->alloc(std::max<int64_t>(tot, 1) * sizeof(double));
-        tv0 = synced_now();
+        tv0 = c->mark();
         eigh_vectors_all(c, er, cols, static_cast<double *>(Esub->p), esoff);
         ES0 = Esub;
         es0off = esoff;
@@ -988,7 +980,6 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         if (totE > 0) {
           auto ELb = c->alloc(totE * sizeof(double));
           eigh_vectors_all(c, er, colsL, static_cast<double *>(ELb->p), eLoff);
-          c->sync();
           EL0 = ELb;
           el0off = eLoff;
           FLb = c->alloc(totF * sizeof(double));
@@ -1032,7 +1023,7 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
             for (int j = 0; j < nL[q]; ++j) evals[q][gn[q] - nL[q] + j] = std::ldexp(lv[iLoff[q] + j], 2 * dq[q]);
         }
       }
-      if (round == 0) vec_ms += ms_since(tv0);
+      if (round == 0) c->phase("svd_refine_vec_ms", tv0, c->mark());
       const double *Es = static_cast<const double *>(Esub->p);
       // W_S and the Gram of the projected tail
       std::vector<int64_t> woff(nr), g2off(nr);
@@ -1113,9 +1104,9 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
       else raw_gemm(c, kTrans, kNormal, W_, W_, G_, gb2, gp2);
       if (!std::isfinite(dev_maxabs(c, G_, gtot2)))
         throw Error(kErrNumeric, "SVD refinement round " + std::to_string(round) + ": non-finite Gram of the projected tail");
-      const auto te0 = synced_now();
+      const hipEvent_t te0 = c->mark();
       EighResult er2 = eigh_named(c, G_, g2off, rn, "refinement round " + std::to_string(round));
-      eigh_ms += ms_since(te0);
+      c->phase("svd_refine_eigh_ms", te0, c->mark());
       for (int i = 0; i < nr; ++i) {
         const int q = rs[i], r = rn[i];
         for (double &x : er2.evals[i]) x = std::ldexp(x, 2 * dq[q]);
@@ -1183,8 +1174,6 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     }
     c->count("svd_refine_ms",
              std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tr0).count());
-    c->count("svd_refine_vec_ms", vec_ms);
-    c->count("svd_refine_eigh_ms", eigh_ms);
   }
   // 6. new bond index
   Leg knew;
