@@ -10,6 +10,9 @@
 // of the two-site state), so the BLAS-1 steps are flat vector kernels.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include <cmath>
 
 #include "runtime.h"
@@ -188,47 +191,140 @@ TenP cast_to(const TenP &t, Prec p) {
   return r;
 }
 
-// Lowest eigenpair of a small dense symmetric matrix (row-major n x n) by
-// cyclic Jacobi; returns the eigenvalue and fills vec.
+// Lowest eigenpair of a small dense symmetric matrix (row-major n x n, n of
+// the order of the restart length): Householder reduction to tridiagonal
+// form (A = Q T Q^T), the lowest eigenvalue of T by bisection on Sturm
+// counts, its vector by two steps of inverse iteration (Gaussian elimination
+// with partial pivoting on T - lambda I), then vec = Q z.  O(n^3) once
+// instead of per Jacobi sweep; the eigenvalue is accurate to a few eps ||A||.
 double small_sym_lowest(int n, std::vector<double> A, std::vector<double> &vec) {
-  std::vector<double> V(n * n, 0.0);
-  for (int i = 0; i < n; ++i) V[i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 60; ++sweep) {
-    double off = 0, tot = 0;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) {
-        tot += A[i * n + j] * A[i * n + j];
-        if (i != j) off += A[i * n + j] * A[i * n + j];
-      }
-    if (off <= 1e-32 * tot) break;
-    for (int p = 0; p < n - 1; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        double apq = A[p * n + q];
-        if (apq == 0.0) continue;
-        double zeta = (A[q * n + q] - A[p * n + p]) / (2 * apq);
-        double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));
-        double c = 1 / std::sqrt(1 + t * t), s = t * c;
-        for (int k = 0; k < n; ++k) {
-          double x = A[p * n + k], y = A[q * n + k];
-          A[p * n + k] = c * x - s * y;
-          A[q * n + k] = s * x + c * y;
-        }
-        for (int k = 0; k < n; ++k) {
-          double x = A[k * n + p], y = A[k * n + q];
-          A[k * n + p] = c * x - s * y;
-          A[k * n + q] = s * x + c * y;
-          double u = V[k * n + p], v = V[k * n + q];
-          V[k * n + p] = c * u - s * v;
-          V[k * n + q] = s * u + c * v;
-        }
-      }
+  vec.assign(n, 0.0);
+  if (n == 1) {
+    vec[0] = 1.0;
+    return A[0];
   }
-  int im = 0;
-  for (int i = 1; i < n; ++i)
-    if (A[i * n + i] < A[im * n + im]) im = i;
-  vec.resize(n);
-  for (int k = 0; k < n; ++k) vec[k] = V[k * n + im];
-  return A[im * n + im];
+  auto at = [&](int r, int c) -> double & { return A[(size_t)r * n + c]; };
+  std::vector<std::vector<double>> refl;  // v_i (length n - i - 1, v[0] = 1), reflector i
+  std::vector<double> taus;
+  std::vector<double> p(n), w(n);
+  for (int i = 0; i + 2 < n; ++i) {
+    const int m = n - i - 1;
+    const double alpha = at(i + 1, i);
+    double xn2 = 0;
+    for (int r = i + 2; r < n; ++r) xn2 += at(r, i) * at(r, i);
+    std::vector<double> v(m, 0.0);
+    v[0] = 1.0;
+    if (xn2 == 0.0) {
+      refl.push_back(v);
+      taus.push_back(0.0);
+      continue;
+    }
+    const double beta = -std::copysign(std::sqrt(alpha * alpha + xn2), alpha);
+    const double tau = (beta - alpha) / beta, sc = 1.0 / (alpha - beta);
+    for (int r = 1; r < m; ++r) v[r] = at(i + 1 + r, i) * sc;
+    // A22 <- H A22 H, H = I - tau v v^T:  p = tau A22 v, w = p - (tau/2)(p.v) v
+    double pv = 0;
+    for (int r = 0; r < m; ++r) {
+      double t = 0;
+      for (int c = 0; c < m; ++c) t += at(i + 1 + r, i + 1 + c) * v[c];
+      p[r] = tau * t;
+      pv += p[r] * v[r];
+    }
+    const double K = 0.5 * tau * pv;
+    for (int r = 0; r < m; ++r) w[r] = p[r] - K * v[r];
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < m; ++c) at(i + 1 + r, i + 1 + c) -= v[r] * w[c] + w[r] * v[c];
+    at(i + 1, i) = at(i, i + 1) = beta;
+    for (int r = i + 2; r < n; ++r) at(r, i) = at(i, r) = 0.0;
+    refl.push_back(v);
+    taus.push_back(tau);
+  }
+  std::vector<double> d(n), e(n, 0.0);
+  double tnorm = 0;
+  for (int i = 0; i < n; ++i) {
+    d[i] = at(i, i);
+    if (i + 1 < n) e[i] = at(i + 1, i);
+  }
+  double lo = d[0], hi = d[0];
+  for (int i = 0; i < n; ++i) {
+    const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(e[i]) : 0.0);
+    lo = std::min(lo, d[i] - r);
+    hi = std::max(hi, d[i] + r);
+    tnorm = std::max(tnorm, std::fabs(d[i]) + r);
+  }
+  const double eps = 2.220446049250313e-16;
+  const double pivmin = std::max(tnorm * eps * eps, 1e-300);
+  // number of eigenvalues of T below x
+  auto below = [&](double x) {
+    int cnt = 0;
+    double q = d[0] - x;
+    if (std::fabs(q) < pivmin) q = -pivmin;
+    if (q < 0) ++cnt;
+    for (int i = 1; i < n; ++i) {
+      q = d[i] - x - e[i - 1] * e[i - 1] / q;
+      if (std::fabs(q) < pivmin) q = -pivmin;
+      if (q < 0) ++cnt;
+    }
+    return cnt;
+  };
+  lo -= 2 * eps * tnorm + pivmin;
+  hi += 2 * eps * tnorm + pivmin;
+  for (int it = 0; it < 200 && hi - lo > 2 * eps * std::max(std::fabs(lo), std::fabs(hi)) + pivmin; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (mid <= lo || mid >= hi) break;
+    if (below(mid) >= 1) hi = mid;
+    else lo = mid;
+  }
+  const double lam = 0.5 * (lo + hi);
+  // inverse iteration on the tridiagonal T - lam I (dense, partial pivoting)
+  std::vector<double> z(n, 1.0 / std::sqrt((double)n)), M((size_t)n * n);
+  for (int step = 0; step < 2; ++step) {
+    std::fill(M.begin(), M.end(), 0.0);
+    for (int i = 0; i < n; ++i) {
+      M[(size_t)i * n + i] = d[i] - lam;
+      if (i + 1 < n) M[(size_t)i * n + i + 1] = M[(size_t)(i + 1) * n + i] = e[i];
+    }
+    std::vector<double> b = z;
+    for (int col = 0; col < n; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < n; ++r)
+        if (std::fabs(M[(size_t)r * n + col]) > std::fabs(M[(size_t)piv * n + col])) piv = r;
+      if (piv != col) {
+        for (int c2 = 0; c2 < n; ++c2) std::swap(M[(size_t)col * n + c2], M[(size_t)piv * n + c2]);
+        std::swap(b[col], b[piv]);
+      }
+      double &pv = M[(size_t)col * n + col];
+      if (std::fabs(pv) < eps * tnorm) pv = pv < 0 ? -eps * tnorm : eps * tnorm;
+      if (pv == 0.0) pv = pivmin;
+      for (int r = col + 1; r < n; ++r) {
+        const double f = M[(size_t)r * n + col] / pv;
+        if (f == 0.0) continue;
+        for (int c2 = col; c2 < n; ++c2) M[(size_t)r * n + c2] -= f * M[(size_t)col * n + c2];
+        b[r] -= f * b[col];
+      }
+    }
+    for (int r = n - 1; r >= 0; --r) {
+      double t = b[r];
+      for (int c2 = r + 1; c2 < n; ++c2) t -= M[(size_t)r * n + c2] * b[c2];
+      b[r] = t / M[(size_t)r * n + r];
+    }
+    double nb = 0;
+    for (double x : b) nb += x * x;
+    nb = std::sqrt(nb);
+    if (!(nb > 0) || !std::isfinite(nb)) break;
+    for (int i = 0; i < n; ++i) z[i] = b[i] / nb;
+  }
+  // vec = H_0 H_1 ... H_{n-3} z
+  for (int i = (int)refl.size() - 1; i >= 0; --i) {
+    const std::vector<double> &v = refl[i];
+    const int m = (int)v.size();
+    double t = 0;
+    for (int r = 0; r < m; ++r) t += v[r] * z[i + 1 + r];
+    t *= taus[i];
+    for (int r = 0; r < m; ++r) z[i + 1 + r] -= t * v[r];
+  }
+  vec = z;
+  return lam;
 }
 
 }  // namespace
@@ -238,7 +334,8 @@ double small_sym_lowest(int n, std::vector<double> A, std::vector<double> &vec) 
 // r = h - E_k v_k.  The Krylov space {v_k, r, H~ r, ...} is then extended with
 // the low-precision operator H~ (FP32 operands and accumulation; products on
 // the FP32 matrix cores, or split-BF16x3 on the BF16 ones), every new
-// vector orthogonalised twice in FP64 against the whole basis.  The projected
+// vector orthogonalised in FP64 by the three-term step and then once more
+// against the whole basis.  The projected
 // matrix uses the exact FP64 row of v_k (E_k, |r|, 0, ...) and FP64 dot
 // products of the FP32 images, so the error left by H~ enters the Ritz vector
 // only at second order and each restart contracts it further: the result
@@ -260,7 +357,7 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
   const int64_t n = full->total;
   const int kInnerMax = (int)std::min<int64_t>(std::max<int64_t>(max_iter, 2), 40);
   LzDev dv(c, kInnerMax + 2);
-  double *d_col = dv.a, *d_col2 = dv.b;   // re-orthogonalisation coefficients (device)
+  double *d_col = dv.a;   // orthogonalisation coefficients, norm and alpha (device)
   TenP v = embed(init, full);
   k_dot_to(c, n, v->d(), v->d(), dv.scal);
   lz_rsqrt_kernel<<<1, 1, 0, c->stream>>>(dv.scal, dv.scal + 1, dv.scal + 2);
@@ -301,6 +398,7 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
     T[0][1] = T[1][0] = beta;
     std::vector<double> y;
     double theta_prev = E;
+    double wprev = beta;  // norm that made q_j: beta for q_1, then the previous step's
     int m_in = 0;
     for (int j = 1; j <= kInnerMax; ++j) {
       // w = H~ q_j
@@ -313,18 +411,27 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
       auto w = ten_alloc(c, full);
       k_cast_f2d(c, n, y32->f(), w->d());
       ++total_inner;
-      const int k = (int)Q.size();
-      // Full re-orthogonalisation against Q[0..k-1], classical Gram-Schmidt
-      // twice in FP64 (two batched passes); the first pass's coefficients
-      // <Q_i, w> are the projected matrix's new column.
+      const int k = (int)Q.size();  // j + 1
+      // Orthogonalisation in FP64, two passes: the three-term step removes
+      // the large components (alpha_j = <q_j, w> along q_j, the previous
+      // norm beta_j along q_{j-1}), then one classical Gram-Schmidt pass
+      // against all of Q[0..j] removes what the low-precision operator and
+      // the rounding of that step leave along every q_i.  The projected
+      // matrix's new column is <Q_i, w> = the second pass's coefficient plus
+      // alpha_j (i = j) or beta_j (i = j - 1).
+      const double bprev = wprev;
+      double *d_alpha = d_col + k + 1;
+      k_dot_to(c, n, Q[j]->d(), w->d(), d_alpha);
+      k_axpy_dev(c, n, -1.0, d_alpha, Q[j]->d(), w->d());
+      k_axpby(c, n, -bprev, Q[j - 1]->d(), 1.0, w->d());
       k_multidot(c, n, k, dv.ptrs(), w->d(), d_col, static_cast<double *>(dv.partial->p));
       k_multiaxpy(c, n, k, dv.ptrs(), d_col, w->d());
-      k_multidot(c, n, k, dv.ptrs(), w->d(), d_col2, static_cast<double *>(dv.partial->p));
-      k_multiaxpy(c, n, k, dv.ptrs(), d_col2, w->d());
       k_dot_to(c, n, w->d(), w->d(), d_col + k);
-      const double *hc = dv.read(d_col, k + 1);
+      const double *hc = dv.read(d_col, k + 2);
       std::vector<double> col(hc, hc + k);
       const double wn = std::sqrt(hc[k]);
+      col[j] += hc[k + 1];
+      col[j - 1] += bprev;
       col[0] = (j == 1) ? beta : 0.0;
       for (int i = 1; i <= j; ++i) {
         T[i][j] = col[i];
@@ -342,6 +449,7 @@ LanczosOut lanczos_mixed(const TenP &L, const TenP &W1, const TenP &W2, const Te
       theta_prev = theta;
       if (inner_done) break;
       k_scale(c, n, 1.0 / wn, w->d());
+      wprev = wn;
       Q.push_back(w);
       dv.set_ptr(k, w->d());
       for (auto &row : T) row.push_back(0.0);
