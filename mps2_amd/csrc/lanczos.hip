@@ -11,8 +11,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdlib>
-
 #include <cmath>
 
 #include "runtime.h"
