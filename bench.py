@@ -462,7 +462,8 @@ def main():
                                for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls", "hip_malloc",
                                          "svd_eigh_ms", "svd_refine_ms", "svd_refine_eigh_ms", "svd_refine_rounds",
                                          "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
-                                         "svd_max_sector_sum", "gemm_narrow", "gemm_narrow_rowmajor_a")}
+                                         "svd_max_sector_sum", "gemm_narrow", "gemm_narrow_rowmajor_a",
+                                         "plan_builds", "plan_build_ms")}
                            for k in res},
         "stage_ms": {k: mb[k]["stage_ms"] for k in ("fp64", "mixed", "mixed_bf16x3")},
         "stage_ms_heuristic": {k: heur[k]["stage_ms"] for k in heur},

@@ -160,6 +160,8 @@ class TwoSiteVMPSEngine {
         Contract(&t2, &ad, {{3, 1}, {1, 2}}, &renvs_[i]);
       }
     }
+    // the environments are complete before the first update's phase clocks start
+    CheckMps2(mps2_ctx_sync(DefaultCtx()), "sync");
   }
 
   // two_site_update_finite_vmps_impl.h:143-161

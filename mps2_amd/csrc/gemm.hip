@@ -53,6 +53,23 @@ struct Mfma<float> {
 
 constexpr int kThreads = 256;
 
+// Tile tix of a launch (TileMap, runtime.h): from the list, or by binary
+// search of the launch-order prefix sums for the block (the largest position
+// whose start is <= tix; blocks without tiles are never found) and its
+// row-major tile (i, j).
+__device__ __forceinline__ TileD tile_at(const TileMap &T, const BlkD *__restrict__ blks, int tix) {
+  if (T.list) return T.list[tix];
+  int lo = 0, hi = T.nb;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (T.start[mid] <= tix) lo = mid;
+    else hi = mid;
+  }
+  const int b = T.order[lo], local = tix - T.start[lo];
+  const int tn = (blks[b].N + T.tn - 1) / T.tn;
+  return TileD{b, (local / tn) * T.tm, (local % tn) * T.tn, 0};
+}
+
 // Per-thread share of one BK-deep slab of A (BM x BK) and B (BK x BN),
 // loaded from global memory into registers, then stored to LDS: FP32 as
 // As[m][k], Bs[n][k] with row stride SK (k contiguous, rows 16-byte
@@ -167,7 +184,7 @@ struct Quad {
 // TAG only names the instance (profiling: isolates one stage's launches).
 template <typename T, int BM, int BN, bool TA, bool TB, int TAG, int BK, int NBUF>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
-                                                                 T *__restrict__ C, const TileD *__restrict__ tiles,
+                                                                 T *__restrict__ C, TileMap tiles,
                                                                  const BlkD *__restrict__ blks,
                                                                  const PairD *__restrict__ pairs, int ntiles, int sub) {
   using V4 = typename Mfma<T>::v4;
@@ -183,7 +200,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   const int wm = wave >> 1, wn = wave & 1;
   const int tix = blockIdx.x;
   if (tix >= ntiles) return;
-  const TileD tile = tiles[tix];
+  const TileD tile = tile_at(tiles, blks, tix);
   if (tile.blk < 0) return;  // padding of a shorter per-XCD queue
   const BlkD blk = blks[tile.blk];
   const int M = blk.M, N = blk.N;
@@ -383,7 +400,7 @@ static_assert(kLdB * 4 == 528, "B row offsets in read_frags assume kLdB = 132");
 template <int NST, int TAG>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_dma_kernel(const float *__restrict__ A,
                                                                      const float *__restrict__ B, float *__restrict__ C,
-                                                                     const TileD *__restrict__ tiles,
+                                                                     TileMap tiles,
                                                                      const BlkD *__restrict__ blks,
                                                                      const PairD *__restrict__ pairs, int ntiles,
                                                                      int sub) {
@@ -396,7 +413,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_dma_kernel(const float 
   const int wm = wave >> 1, wn = wave & 1;
   const int tix = blockIdx.x;
   if (tix >= ntiles) return;
-  const TileD tile = tiles[tix];
+  const TileD tile = tile_at(tiles, blks, tix);
   if (tile.blk < 0) return;
   const BlkD blk = blks[tile.blk];
   const int M = blk.M, N = blk.N, m0 = tile.m0, n0 = tile.n0;
@@ -608,7 +625,7 @@ template <int BM, int BN, bool TA, bool TB, int TAG>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float *__restrict__ A,
                                                                      const float *__restrict__ B,
                                                                      float *__restrict__ C,
-                                                                     const TileD *__restrict__ tiles,
+                                                                     TileMap tiles,
                                                                      const BlkD *__restrict__ blks,
                                                                      const PairD *__restrict__ pairs, int ntiles,
                                                                      int sub) {
@@ -626,7 +643,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
   const int wm = wave >> 1, wn = wave & 1;
   const int tix = blockIdx.x;
   if (tix >= ntiles) return;
-  const TileD tile = tiles[tix];
+  const TileD tile = tile_at(tiles, blks, tix);
   if (tile.blk < 0) return;
   const BlkD blk = blks[tile.blk];
   const int M = blk.M, N = blk.N;
@@ -712,7 +729,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
 }
 
 template <int BM, int BN, int TAG>
-void launch_tiles_bf3(Ctx *c, int la, int lb, const float *A, const float *B, float *C, const TileD *tiles,
+void launch_tiles_bf3(Ctx *c, int la, int lb, const float *A, const float *B, float *C, const TileMap &tiles,
                       int ntiles, const BlkD *blks, const PairD *pairs, int sub) {
   dim3 grid(ntiles), block(kThreads);
   if (la == 0 && lb == 0)
@@ -743,12 +760,12 @@ void launch_tiles_bf3(Ctx *c, int la, int lb, const float *A, const float *B, fl
 // is bit-reproducible.  A tile is (block, 256 rows from m0, NC columns from n0).
 template <typename T, bool TA, bool TB, int NC, int TAG>
 __global__ __launch_bounds__(256) void narrow_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
-                                                          T *__restrict__ C, const TileD *__restrict__ tiles,
+                                                          T *__restrict__ C, TileMap tiles,
                                                           const BlkD *__restrict__ blks,
                                                           const PairD *__restrict__ pairs, int ntiles, int sub) {
   const int tix = blockIdx.x;
   if (tix >= ntiles) return;
-  const TileD tile = tiles[tix];
+  const TileD tile = tile_at(tiles, blks, tix);
   if (tile.blk < 0) return;
   const BlkD blk = blks[tile.blk];
   const int M = blk.M, N = blk.N;
@@ -780,7 +797,7 @@ __global__ __launch_bounds__(256) void narrow_gemm_kernel(const T *__restrict__ 
 }
 
 template <typename T, int NC, int TAG>
-void launch_narrow_nc(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_narrow_nc(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                       const BlkD *blks, const PairD *pairs, int sub) {
   dim3 grid(ntiles), block(256);
   if (la == 0 && lb == 0)
@@ -794,7 +811,7 @@ void launch_narrow_nc(Ctx *c, int la, int lb, const T *A, const T *B, T *C, cons
 }
 
 template <typename T, int TAG>
-void launch_narrow_tag(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_narrow_tag(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                        const BlkD *blks, const PairD *pairs, int nc, int sub) {
   if (nc == 4) launch_narrow_nc<T, 4, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
   else if (nc == 8) launch_narrow_nc<T, 8, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
@@ -802,7 +819,7 @@ void launch_narrow_tag(Ctx *c, int la, int lb, const T *A, const T *B, T *C, con
 }
 
 template <typename T, int BM, int BN, int TAG, int BK, int NBUF>
-void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                     const BlkD *blks, const PairD *pairs, int sub) {
   dim3 grid(ntiles), block(kThreads);
   if (la == 0 && lb == 0)
@@ -833,7 +850,7 @@ int forced_variant() {
 }
 
 template <typename T, int BM, int BN, int TAG>
-void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                   const BlkD *blks, const PairD *pairs, int sub, int variant) {
   if constexpr (sizeof(T) == 4) {
     if (c->f32_split) {
@@ -869,17 +886,49 @@ void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const Ti
 // (MPS2_XCDS=1, group 1) is one queue of single tiles, heaviest first: on the
 // D=4096 J1-J2 matvec the grouped order (groups of 2, 4 or 8) made the FP32
 // stage-4 launch 15-19% slower (4.67 -> 5.38..5.58 ms; FP64 unchanged).
-std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts) {
-  static const int kXcds = [] {
+namespace {
+int xcd_queues() {
+  static const int v = [] {
     const char *e = std::getenv("MPS2_XCDS");
     int x = e ? std::atoi(e) : 1;
     return x == 8 ? 8 : 1;
   }();
-  static const int kGroup = [] {
+  return v;
+}
+int tile_group() {
+  static const int v = [] {
     const char *e = std::getenv("MPS2_TILE_GROUP");
-    int g = e ? std::atoi(e) : (kXcds == 8 ? 4 : 1);
+    int g = e ? std::atoi(e) : (xcd_queues() == 8 ? 4 : 1);
     return g < 1 ? 1 : (g > 8 ? 8 : g);
   }();
+  return v;
+}
+}  // namespace
+
+std::vector<int32_t> tile_block_order(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc) {
+  if (xcd_queues() != 1 || tile_group() != 1) return {};
+  std::vector<int32_t> order(blks.size());
+  for (size_t b = 0; b < blks.size(); ++b) order[b] = (int32_t)b;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+    return std::max<int64_t>(kc[x], 1) > std::max<int64_t>(kc[y], 1);
+  });
+  return order;
+}
+
+std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts) {
+  const int kXcds = xcd_queues(), kGroup = tile_group();
+  if (kXcds == 1 && kGroup == 1) {
+    // one queue of single tiles, heaviest first: the tiles of a block share
+    // its cost, so order the blocks (stably) and emit each block's tiles
+    const std::vector<int32_t> order = tile_block_order(blks, kc);
+    std::vector<TileD> out;
+    for (int b : order) {
+      const int tm = (blks[b].M + ts - 1) / ts, tn = (blks[b].N + ts - 1) / ts;
+      for (int i = 0; i < tm; ++i)
+        for (int j = 0; j < tn; ++j) out.push_back(TileD{b, i * ts, j * ts, 0});
+    }
+    return out;
+  }
   struct Group {
     int blk, tm0, tm1, tn0, tn1;
     double cost;
@@ -915,7 +964,7 @@ std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vect
 }
 
 template <typename T>
-void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag,
                          int variant) {
   if (ntiles <= 0) return;
@@ -934,7 +983,7 @@ void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, c
 }
 
 template <typename T>
-void launch_narrow_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_narrow_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                         const BlkD *blks, const PairD *pairs, int nc, int sub, int tag) {
   if (ntiles <= 0) return;
   if (tag == 1) launch_narrow_tag<T, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, nc, sub);
@@ -951,14 +1000,14 @@ std::vector<TileD> narrow_tiles(const std::vector<BlkD> &blks, int nc) {
   return t;
 }
 
-template void launch_narrow_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *, int,
+template void launch_narrow_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileMap &, int,
                                          const BlkD *, const PairD *, int, int, int);
-template void launch_narrow_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
+template void launch_narrow_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileMap &, int,
                                         const BlkD *, const PairD *, int, int, int);
 
-template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileD *,
+template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileMap &,
                                           int, const BlkD *, const PairD *, int, int, int, int, int);
-template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileD *, int,
+template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileMap &, int,
                                          const BlkD *, const PairD *, int, int, int, int, int);
 
 }  // namespace mps2

@@ -35,6 +35,19 @@ void Meta::finalize() {
     lookup[coords[b]] = (int32_t)b;
   }
   total = off;
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t x) { h = (h ^ x) * 1099511628211ull; };
+  mix((uint64_t)qw);
+  for (const Leg &l : legs) {
+    mix((uint64_t)(uint32_t)l.dir);
+    for (const QN &q : l.qn)
+      for (int i = 0; i < kMaxQnWidth; ++i) mix((uint32_t)q.v[i]);
+    for (int64_t d : l.dim) mix((uint64_t)d);
+    mix(0x9e3779b97f4a7c15ull);
+  }
+  for (const Coord &c : coords)
+    for (int32_t x : c) mix((uint32_t)x);
+  digest = h;
 }
 
 bool Meta::same_layout(const Meta &o) const {

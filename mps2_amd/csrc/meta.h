@@ -100,6 +100,7 @@ struct Meta {
   std::vector<int64_t> offset;          // element offset of each block
   std::vector<int64_t> bsize;           // elements in each block
   int64_t total = 0;
+  uint64_t digest = 0;                  // hash of (qw, legs, coords), set by finalize()
   std::unordered_map<Coord, int32_t, CoordHash> lookup;
   mutable DagLink dag;                  // see DagLink
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +26,7 @@ using PlanKey = std::tuple<Ctx *, const Meta *, const Meta *, std::vector<int>, 
 struct PlanEntry {
   std::shared_ptr<DevPlan> plan;
   MetaP keep_out;   // retarget layout kept alive while cached
+  MetaP ka, kb;     // the key's operand layouts, kept alive (their addresses are the key)
 };
 
 // Process-wide plan cache (keys carry the context), FIFO eviction; every
@@ -33,6 +35,21 @@ std::mutex g_plan_mu;
 std::map<PlanKey, PlanEntry> g_plans;
 std::deque<PlanKey> g_plan_order;
 constexpr size_t kPlanCacheCap = 512;
+
+// Second level, keyed by layout content: a tensor made anew with a layout
+// seen before (the same bond in a later sweep, once the kept sector
+// dimensions have settled) finds its plan, tile lists and autotuner state
+// again.  Hits are confirmed by comparing the layouts themselves
+// (Meta::same_layout); the digests only index.  MPS2_PLAN_CONTENT=0 turns
+// this level off (every new layout object then builds its plan).
+using ContentKey = std::tuple<Ctx *, uint64_t, uint64_t, std::vector<int>, std::vector<int>, uint64_t, bool>;
+struct ContentEntry {
+  std::shared_ptr<DevPlan> plan;
+  MetaP target;
+};
+std::map<ContentKey, ContentEntry> g_cplans;
+std::deque<ContentKey> g_cplan_order;
+constexpr size_t kContentCacheCap = 4096;
 
 void perm_descs(const Meta &m, const std::vector<int> &perm, std::vector<PermDesc> &out, std::vector<int64_t> &starts) {
   int r = m.rank();
@@ -158,12 +175,6 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
     blks[i].pe = b.pair_end;
     for (int p = b.pair_begin; p < b.pair_end; ++p) kcs[i] += hp->pairs[p].K;
   }
-  // Tile lists ordered for the XCDs (schedule_tiles, gemm.hip).
-  auto upload_tiles = [&](int edge, int &count) {
-    std::vector<TileD> t = schedule_tiles(blks, kcs, edge);
-    count = (int)t.size();
-    return upload_bytes(c, t.data(), t.size() * sizeof(TileD));
-  };
   int max_n = 0;
   for (auto &b : blks) max_n = std::max(max_n, (int)b.N);
   // MPS2_NARROW=0 keeps narrow contractions on the matrix-core kernels.
@@ -171,16 +182,63 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
     const char *e = std::getenv("MPS2_NARROW");
     return !(e && std::string(e) == "0");
   }();
-  if (narrow_on && max_n <= kNarrowMaxN && !hp->blks.empty()) {
+  if (narrow_on && max_n <= kNarrowMaxN && !hp->blks.empty())
     dp->narrow_nc = max_n <= 4 ? 4 : (max_n <= 8 ? 8 : 16);
-    std::vector<TileD> t = narrow_tiles(blks, dp->narrow_nc);
-    dp->ntiles_narrow = (int)t.size();
-    dp->tiles_narrow = upload_bytes(c, t.data(), t.size() * sizeof(TileD));
+  // Tiles: the blocks in launch order (heaviest first) and, per tile shape,
+  // the prefix sums of their tile counts, decoded on the device; explicit
+  // lists only for an XCD-grouped order (schedule_tiles, gemm.hip).
+  const std::vector<int32_t> order = tile_block_order(blks, kcs);
+  if (!order.empty() || blks.empty()) {
+    const int nb = (int)blks.size();
+    std::vector<int32_t> h((size_t)nb + 3 * ((size_t)nb + 1), 0);
+    for (int i = 0; i < nb; ++i) h[i] = order[i];
+    int32_t *s64 = h.data() + nb, *s128 = s64 + nb + 1, *sn = s128 + nb + 1;
+    const int nc = dp->narrow_nc ? dp->narrow_nc : 1;
+    for (int i = 0; i < nb; ++i) {
+      const BlkD &bk = blks[order[i]];
+      auto cnt = [&](int tm, int tn) {
+        return (int64_t)((bk.M + tm - 1) / tm) * ((bk.N + tn - 1) / tn);
+      };
+      const int64_t n64 = s64[i] + cnt(64, 64), n128 = s128[i] + cnt(128, 128), nn = sn[i] + cnt(kNarrowRows, nc);
+      if (n64 > INT32_MAX || n128 > INT32_MAX || nn > INT32_MAX) throw Error(kErrArg, "contraction: too many tiles");
+      s64[i + 1] = (int32_t)n64;
+      s128[i + 1] = (int32_t)n128;
+      sn[i + 1] = (int32_t)nn;
+    }
+    dp->tilemap = upload_bytes(c, h.data(), h.size() * sizeof(int32_t));
+    const int32_t *d = static_cast<const int32_t *>(dp->tilemap->p);
+    auto mk = [&](const int32_t *start, int tm, int tn) {
+      TileMap m;
+      m.order = d;
+      m.start = start;
+      m.nb = nb;
+      m.tm = tm;
+      m.tn = tn;
+      return m;
+    };
+    dp->map64 = mk(d + nb, 64, 64);
+    dp->map128 = mk(d + 2 * nb + 1, 128, 128);
+    dp->map_narrow = mk(d + 3 * nb + 2, kNarrowRows, nc);
+    dp->ntiles64 = s64[nb];
+    dp->ntiles128 = s128[nb];
+    dp->ntiles_narrow = sn[nb];
+  } else {
+    auto upload_tiles = [&](int edge, int &count, TileMap &m) {
+      std::vector<TileD> t = schedule_tiles(blks, kcs, edge);
+      count = (int)t.size();
+      BufP buf = upload_bytes(c, t.data(), t.size() * sizeof(TileD));
+      m.list = static_cast<const TileD *>(buf->p);
+      return buf;
+    };
+    dp->tiles64 = upload_tiles(64, dp->ntiles64, dp->map64);
+    dp->tiles128 = upload_tiles(128, dp->ntiles128, dp->map128);
+    if (dp->narrow_nc) {
+      std::vector<TileD> t = narrow_tiles(blks, dp->narrow_nc);
+      dp->ntiles_narrow = (int)t.size();
+      dp->tilemap = upload_bytes(c, t.data(), t.size() * sizeof(TileD));
+      dp->map_narrow.list = static_cast<const TileD *>(dp->tilemap->p);
+    }
   }
-  dp->tiles64 = upload_tiles(64, dp->ntiles64);
-  dp->tiles128 = upload_tiles(128, dp->ntiles128);
-  if (ts == 128) dp->tiles = dp->tiles128, dp->ntiles = dp->ntiles128;
-  else dp->tiles = dp->tiles64, dp->ntiles = dp->ntiles64;
   dp->blks = upload_bytes(c, blks.data(), blks.size() * sizeof(BlkD));
   dp->pairs = upload_bytes(c, pairs.data(), pairs.size() * sizeof(PairD));
   if (hp->permute_a) {
@@ -207,19 +265,52 @@ std::shared_ptr<DevPlan> get_plan_t(Ctx *c, const MetaP &a, const MetaP &b, cons
   PlanKey key{c, a.get(), b.get(), ax, bx, target.get(), cplx};
   std::lock_guard<std::mutex> lock(g_plan_mu);
   auto it = g_plans.find(key);
-  if (it != g_plans.end() && it->second.plan->hp->a == a && it->second.plan->hp->b == b) return it->second.plan;
-  auto hp = plan_contract(a, b, ax, bx);
-  auto dp = build_dev_plan(c, hp, target.get(), cplx);
+  if (it != g_plans.end() && it->second.ka == a && it->second.kb == b) return it->second.plan;
+  const ContentKey ck{c, a->digest, b->digest, ax, bx, target ? target->digest : 0, cplx};
+  std::shared_ptr<DevPlan> dp;
+  static const bool content_on = [] {
+    const char *e = std::getenv("MPS2_PLAN_CONTENT");
+    return !(e && std::string(e) == "0");
+  }();
+  auto ci = content_on ? g_cplans.find(ck) : g_cplans.end();
+  if (ci != g_cplans.end()) {
+    const ContractPlan &h = *ci->second.plan->hp;
+    const bool same_target = target ? (ci->second.target && ci->second.target->same_layout(*target))
+                                    : !ci->second.target;
+    if (h.a->same_layout(*a) && h.b->same_layout(*b) && same_target) {
+      dp = ci->second.plan;
+      c->count("plan_content_hits");
+    }
+  }
+  if (!dp) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto hp = plan_contract(a, b, ax, bx);
+    dp = build_dev_plan(c, hp, target.get(), cplx);
+    c->count("plan_builds");
+    c->count("plan_build_ms",
+             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    if (!content_on) {
+    } else if (ci != g_cplans.end()) {
+      ci->second = ContentEntry{dp, target};
+    } else {
+      if (g_cplans.size() >= kContentCacheCap) {
+        g_cplans.erase(g_cplan_order.front());
+        g_cplan_order.pop_front();
+      }
+      g_cplans[ck] = ContentEntry{dp, target};
+      g_cplan_order.push_back(ck);
+    }
+  }
   if (it != g_plans.end()) {
     // same addresses, new layouts (the old ones were freed): replace in place
-    it->second = PlanEntry{dp, target};
+    it->second = PlanEntry{dp, target, a, b};
     return dp;
   }
   if (g_plans.size() >= kPlanCacheCap) {
     g_plans.erase(g_plan_order.front());
     g_plan_order.pop_front();
   }
-  g_plans[key] = PlanEntry{dp, target};
+  g_plans[key] = PlanEntry{dp, target, a, b};
   g_plan_order.push_back(key);
   return dp;
 }
@@ -317,8 +408,8 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   const bool fp64 = sizeof(T) == sizeof(double);
   if (dp.narrow_nc && tag != 2) {
     c->count(dp.hp->la == kTrans ? "gemm_narrow" : "gemm_narrow_rowmajor_a");
-    launch_narrow_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, static_cast<const TileD *>(dp.tiles_narrow->p),
-                          dp.ntiles_narrow, static_cast<const BlkD *>(dp.blks->p),
+    launch_narrow_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, dp.map_narrow, dp.ntiles_narrow,
+                          static_cast<const BlkD *>(dp.blks->p),
                           static_cast<const PairD *>(dp.pairs->p), dp.narrow_nc, 0, tag);
     if (c->time_gemms) {
       MPS2_HIP(hipEventRecord(tm.stop, c->stream));
@@ -331,12 +422,13 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   }
   PlanTune::Probe probe;
   const GemmChoice ch = tag == 2 ? GemmChoice{dp.tile_m, 0} : tune_select(c, dp, fp64, &probe);
-  const TileD *tiles = static_cast<const TileD *>((ch.tile == 128 ? dp.tiles128 : dp.tiles64)->p);
+  TileMap tiles = ch.tile == 128 ? dp.map128 : dp.map64;
   int ntiles = ch.tile == 128 ? dp.ntiles128 : dp.ntiles64;
   int tm_ = ch.tile, tn_ = ch.tile;
   if (tag == 2) {
     const DevPlan::Share &sh = plan_share(c, dp, c->shard_g, c->shard_r);
-    tiles = sh.ntiles ? static_cast<const TileD *>(sh.tiles->p) : nullptr;
+    tiles = TileMap{};
+    tiles.list = sh.ntiles ? static_cast<const TileD *>(sh.tiles->p) : nullptr;
     ntiles = sh.ntiles;
     tm_ = dp.tile_m, tn_ = dp.tile_n;  // the share's list is built with the plan's tile size
     c->shard_flops = sh.flops;
@@ -377,6 +469,19 @@ void plan_cache_clear(Ctx *c) {
   for (auto &k : g_plan_order)
     if (std::get<0>(k) != c) keep.push_back(k);
   g_plan_order.swap(keep);
+  std::vector<ContentEntry> cdropped;
+  for (auto it = g_cplans.begin(); it != g_cplans.end();) {
+    if (std::get<0>(it->first) == c) {
+      cdropped.push_back(std::move(it->second));
+      it = g_cplans.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  std::deque<ContentKey> ckeep;
+  for (auto &k : g_cplan_order)
+    if (std::get<0>(k) != c) ckeep.push_back(k);
+  g_cplan_order.swap(ckeep);
 }
 
 std::shared_ptr<DevPlan> get_plan(Ctx *c, const MetaP &a, const MetaP &b, const std::vector<int> &ax,

@@ -277,23 +277,46 @@ struct PairD {
   int64_t a_off, b_off;
   int32_t K, pad;
 };
+// The tiles of one grouped launch: an explicit list (list != nullptr), or
+// decoded on the device from the blocks in launch order (order[0..nb)) and
+// the prefix sums of their tile counts (start[0..nb], start[0] = 0,
+// start[nb] = the launch's tile count) for tm x tn tiles, each block's tiles
+// row-major.  The decoded form keeps a plan's device footprint to a few
+// integers per block.
+struct TileMap {
+  const TileD *list = nullptr;
+  const int32_t *order = nullptr;
+  const int32_t *start = nullptr;
+  int nb = 0, tm = 0, tn = 0;
+};
 // variant (FP32 128x128 tiles only): 0 = register-staged slabs, 4 = direct-to-LDS
 // slabs with four stages (row-major A and B only).
 template <typename T>
-void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0,
                          int variant = 0);
+template <typename T>
+void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0,
+                         int variant = 0) {
+  TileMap m;
+  m.list = tiles;
+  launch_grouped_gemm<T>(c, la, lb, A, B, C, m, ntiles, blks, pairs, tile_m, tile_n, sub, tag, variant);
+}
 // Narrow contractions (kNarrowMaxN, meta.h) run on the vector ALUs
 // (narrow_gemm_kernel, gemm.hip): one thread per output row and nc (4, 8 or
 // 16) columns, kNarrowRows rows per workgroup.
 constexpr int kNarrowRows = 256;
 template <typename T>
-void launch_narrow_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
+void launch_narrow_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                         const BlkD *blks, const PairD *pairs, int nc, int sub = 0, int tag = 0);
 std::vector<TileD> narrow_tiles(const std::vector<BlkD> &blks, int nc);
 // Tile list of one grouped-GEMM launch (tile edge ts) ordered for the XCDs.
 // kc[i] is block i's summed K extent (its per-tile cost).  See gemm.hip.
 std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc, int ts);
+// The single-queue order of schedule_tiles (blocks heaviest first, stable),
+// or an empty vector when an XCD-grouped order is selected (explicit lists).
+std::vector<int32_t> tile_block_order(const std::vector<BlkD> &blks, const std::vector<int64_t> &kc);
 
 // ------------------------------------------------------------------ autotuning (tune.hip)
 // A grouped-GEMM kernel choice: tile edge (64 or 128) and, for FP32 128x128
@@ -327,10 +350,13 @@ struct PlanTune {
 // Device-side contraction plan (descriptors uploaded once, cached by layout).
 struct DevPlan {
   std::shared_ptr<ContractPlan> hp;
-  BufP tiles, blks, pairs;
-  int ntiles = 0;
-  int tile_m = 64, tile_n = 64;   // heuristic FP32 tile (tiles holds its list)
-  BufP tiles64, tiles128;         // the output tiled 64x64 and 128x128
+  BufP blks, pairs;
+  int tile_m = 64, tile_n = 64;   // heuristic FP32 tile
+  // the output tiled 64x64 and 128x128: decoded on the device from the block
+  // order and prefix sums in tilemap (or explicit lists in tiles64 /
+  // tiles128 for the XCD-grouped order)
+  BufP tilemap, tiles64, tiles128;
+  TileMap map64, map128;
   int ntiles64 = 0, ntiles128 = 0;
   // complex plans (hp in real terms, complex_view in ops.hip): the expansion
   // descriptors of B's blocks and the largest K x N among them
@@ -341,7 +367,7 @@ struct DevPlan {
   // narrow plans (every output block at most kNarrowMaxN wide): the vector-ALU
   // kernel's column chunk (0 = matrix-core kernels) and its tile list
   int narrow_nc = 0;
-  BufP tiles_narrow;
+  TileMap map_narrow;
   int ntiles_narrow = 0;
   mutable PlanTune tune[3];       // [0] FP64, [1] FP32, [2] split-BF16x3
   BufP perm_a, perm_b, perm_a_start, perm_b_start;   // permutation descriptors
