@@ -575,6 +575,47 @@ def test_autotuned_launches_are_bit_identical():
         c.close()
 
 
+def test_plan_found_by_layout_content_is_exact():
+    """Contraction plans are cached by layout object and by layout content: a
+    tensor uploaded anew with a layout seen before (new layout objects, new
+    data) finds the earlier plan (counted as a content hit, no plan built),
+    and the contraction through it equals the oracle and is bit-identical to
+    the same data contracted through a freshly built plan.  The blocks are
+    large enough for tiles of several blocks, edges and K steps, so the
+    device-decoded tile order is exercised."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(31)
+    m = Leg(IN, [((q,), 150 + 37 * q) for q in range(4)])
+    k = Leg(OUT, [((q,), 90 + 11 * q) for q in range(4)])
+    n = Leg(OUT, [((q,), 70 + 29 * q) for q in range(5)])
+    A1 = BTensor.random([m, k], (0,), rng)
+    B1 = BTensor.random([k.inverse(), n], (0,), rng)
+    A2 = BTensor.random([m, k], (0,), rng)
+    B2 = BTensor.random([k.inverse(), n], (0,), rng)
+    c = E.Context(0)
+    try:
+        got1 = E.contract(c.put(A1), c.put(B1), [[1], [0]]).get()
+        before = E.ctx_counters(c.h, ("plan_builds", "plan_content_hits"))
+        got2 = E.contract(c.put(A2), c.put(B2), [[1], [0]]).get()
+        after = E.ctx_counters(c.h, ("plan_builds", "plan_content_hits"))
+        assert after["plan_builds"] == before["plan_builds"], (before, after)
+        assert after["plan_content_hits"] == before["plan_content_hits"] + 1, (before, after)
+        for got, (A, B) in ((got1, (A1, B1)), (got2, (A2, B2))):
+            ref = O.contract(A, B, [[1], [0]])
+            assert same_blocks(got, ref)
+            assert H.max_rel_diff(got, ref) < TOL
+    finally:
+        c.close()
+    c2 = E.Context(0)   # a new context starts with no plans: this one is built
+    try:
+        fresh = E.contract(c2.put(A2), c2.put(B2), [[1], [0]]).get()
+        assert E.ctx_counters(c2.h, ("plan_builds",))["plan_builds"] >= 1
+        for cc in fresh.blocks:
+            assert np.array_equal(fresh.blocks[cc], got2.blocks[cc]), cc
+    finally:
+        c2.close()
+
+
 def test_bf16x3_error_is_fp32_class(ctx):
     """Split-BF16x3 against FP32 MFMA on one large grouped launch (8 output
     blocks of about 500 x 500, K = 330): the relative Frobenius error against
