@@ -193,11 +193,14 @@ def svd_workload(args):
         if rep >= args.warmup:
             calls.append(dict(wall_ms=round(wall, 3), **{k: round(after[k] - before[k], 3) for k in names}))
         log(f"svd call {rep}: {wall:.1f} ms")
+    copy_bytes = 1 << 30
+    copy_ms = E.copy_bench(ctx.h, copy_bytes, reps=2)   # calibration kernel for counter runs
     out = {"metric": "truncated SVD wall-time per call, centre-bond sector structure", "value": min(c["wall_ms"] for c in calls),
            "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "dtype": "f64", "data": "synthetic (seeded random orthogonal factors, graded spectra)",
            "config": {"workload": "truncated SVD, block-diagonal sectors", "sectors": dims,
-                      "decades": args.svd_decades, "dmax": dmax}, "calls": calls}
+                      "decades": args.svd_decades, "dmax": dmax}, "calls": calls,
+           "hbm_copy_gbs": round(2 * copy_bytes / (copy_ms * 1e-3) / 1e9, 1)}
     print(json.dumps(out))
 
 
