@@ -41,17 +41,19 @@ def test_heisenberg_chain_per_update_parity_exact_bond():
 
 
 def test_heisenberg_chain_truncated_sweep_parity():
-    """Binding Dmax: the two SVD algorithms (Gram eigen-decomposition on the
-    GPU, one-sided Jacobi in the oracle) may keep subspaces that differ inside
-    the discarded weight, so intermediate energies agree to the truncation
-    scale and the converged energies to 1e-9."""
+    """Binding Dmax (24 on L=16): kept dimensions identical at every update.
+    In the first sweep the state is far from converged and the two Lanczos
+    solvers (1e-10 energy tolerance) return vectors that differ at about the
+    square root of that tolerance; the truncation carries the difference into
+    the next environments, so mid-sweep energies agree to 1.4e-7 relative
+    (measured; bar 1e-6), and the converged energies to 1.3e-13 (bar 1e-11)."""
     m = HM.heisenberg_chain(16)
     g, o = run_pair(m, 2, 1, 24, 0.0, 1e-10)
     e_g, e_o = g["energies"], o["energies"]
-    scale = max(1e-9, 10 * float(np.max(o["trunc_errs"])))
-    assert np.abs(e_g - e_o).max() < max(scale, 1e-4), np.abs(e_g - e_o).max()
-    assert abs(g["energy"] - o["energy"]) < 1e-9
-    assert (g["dims"][-10:] == o["dims"][-10:]).all()
+    assert (g["dims"] == o["dims"]).all()
+    rel = np.abs(e_g - e_o) / np.abs(e_o)
+    assert rel.max() < 1e-6, rel.max()
+    assert abs(g["energy"] - o["energy"]) < 1e-11 * abs(o["energy"])
 
 
 def test_j1j2_small_cylinder_per_update_parity():
@@ -76,8 +78,9 @@ def test_heisenberg_l16_config1_matches_ed():
 @pytest.mark.parametrize("precision", [1, 2])
 def test_mixed_sweep_matches_fp64(precision):
     """Mixed-precision sweeps (1: FP32-MFMA matvec, 2: split-BF16x3) against
-    FP64 sweeps from one checkpoint: every update's energy within 1e-9
-    relative, final within 1e-10."""
+    FP64 sweeps from one checkpoint (L=20, Dmax 48, Lanczos tolerance 1e-11):
+    every update's energy within 1e-10 relative (measured 1.2e-13 for both),
+    the final within 1e-12, kept dimensions identical at every update."""
     m = HM.heisenberg_chain(20)
     m.init_mps([[i % 2 for i in range(20)], [(i + 1) % 2 for i in range(20)]], enlarged_dim=8, seed=3)
     m.dmrg(2, 1, 32, 0.0, 1e-10, 200)
@@ -86,8 +89,9 @@ def test_mixed_sweep_matches_fp64(precision):
     m.restore()
     gmx = m.dmrg(2, 1, 48, 0.0, 1e-11, 200, precision=precision)
     rel = np.abs(gmx["energies"] - g64["energies"]) / np.abs(g64["energies"])
-    assert rel.max() < 1e-9, rel.max()
-    assert abs(gmx["energy"] - g64["energy"]) <= 1e-10 * abs(g64["energy"])
+    assert rel.max() < 1e-10, rel.max()
+    assert abs(gmx["energy"] - g64["energy"]) <= 1e-12 * abs(g64["energy"])
+    assert np.array_equal(gmx["dims"], g64["dims"])
 
 
 def test_heisenberg_l100_config2_ramp():
