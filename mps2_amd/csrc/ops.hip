@@ -38,7 +38,7 @@ constexpr size_t kPlanCacheCap = 512;
 
 // Second level, keyed by layout content: a tensor made anew with a layout
 // seen before (the same bond in a later sweep, once the kept sector
-// dimensions have settled) finds its plan, tile lists and autotuner state
+// dimensions have settled) finds its plan, tile map and autotuner state
 // again.  Hits are confirmed by comparing the layouts themselves
 // (Meta::same_layout); the digests only index.  MPS2_PLAN_CONTENT=0 turns
 // this level off (every new layout object then builds its plan).

@@ -365,7 +365,7 @@ struct DevPlan {
   int nexpand = 0;
   int64_t expand_max_kn = 0;
   // narrow plans (every output block at most kNarrowMaxN wide): the vector-ALU
-  // kernel's column chunk (0 = matrix-core kernels) and its tile list
+  // kernel's column chunk (0 = matrix-core kernels) and its tile map
   int narrow_nc = 0;
   TileMap map_narrow;
   int ntiles_narrow = 0;
