@@ -182,11 +182,16 @@ struct Quad {
 // MFMA steps walk that range four k at a time (one 16-byte LDS read per
 // fragment per four steps).
 // TAG only names the instance (profiling: isolates one stage's launches).
-template <typename T, int BM, int BN, bool TA, bool TB, int TAG, int BK, int NBUF>
+// SPLIT > 1 (split-K): workgroup w runs part w % SPLIT of tile w / SPLIT, the
+// part-th of SPLIT equal runs of the tile's (block pair, slab) sequence;
+// part 0 writes C, part s > 0 the scratch copy s - 1 (stride sstride, same
+// offsets), and splitk_reduce_kernel adds them in part order.
+template <typename T, int BM, int BN, bool TA, bool TB, int TAG, int BK, int NBUF, int SPLIT>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restrict__ A, const T *__restrict__ B,
                                                                  T *__restrict__ C, TileMap tiles,
                                                                  const BlkD *__restrict__ blks,
-                                                                 const PairD *__restrict__ pairs, int ntiles, int sub) {
+                                                                 const PairD *__restrict__ pairs, int ntiles, int sub,
+                                                                 T *__restrict__ scratch, int64_t sstride) {
   using V4 = typename Mfma<T>::v4;
   using SlabT = Slab<T, BM, BN, TA, TB, BK>;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -198,7 +203,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int tix = blockIdx.x;
+  int tix = blockIdx.x, part = 0;
+  if constexpr (SPLIT > 1) {
+    part = tix % SPLIT;
+    tix /= SPLIT;
+  }
   if (tix >= ntiles) return;
   const TileD tile = tile_at(tiles, blks, tix);
   if (tile.blk < 0) return;  // padding of a shorter per-XCD queue
@@ -207,6 +216,24 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   const int m0 = tile.m0, n0 = tile.n0;
   // Tiles listed for an upper bound of a block whose size is set on the device.
   if (m0 >= M || n0 >= N || blk.pb >= blk.pe) return;
+
+  // The slabs this workgroup multiplies: all of the tile's, or one part's
+  // (first slab at pair p, depth k0; `left` of them).
+  int p = blk.pb, k0 = 0, left = 0x7fffffff;
+  if constexpr (SPLIT > 1) {
+    int total = 0;
+    for (int q = blk.pb; q < blk.pe; ++q) total += (pairs[q].K + BK - 1) / BK;
+    const int j0 = (int)((int64_t)total * part / SPLIT);
+    left = (int)((int64_t)total * (part + 1) / SPLIT) - j0;
+    int j = j0;
+    while (p < blk.pe) {
+      const int ns = (pairs[p].K + BK - 1) / BK;
+      if (j < ns) break;
+      j -= ns;
+      ++p;
+    }
+    k0 = j * BK;
+  }
 
   V4 acc[FM][FN];
 #pragma unroll
@@ -221,11 +248,6 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
   const int fnv = kSkip ? live_frags<FN>(N - n0 - wn * WN) : FN;
 
   SlabT slab;
-  int p = blk.pb, k0 = 0;
-  PairD pr = pairs[p];
-  slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, 0, tid);
-  slab.store(As[0], Bs[0], tid);
-  __syncthreads();
   int cur = 0;
   // One slab's products.
   auto multiply = [&]() {
@@ -269,34 +291,47 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
       }
     }
   };
-  while (true) {
-    int np = p, nk0 = k0 + BK;
-    PairD npr = pr;
-    if (nk0 >= pr.K) {
-      np = p + 1;
-      nk0 = 0;
-      if (np < blk.pe) npr = pairs[np];
+  if (left > 0) {  // uniform: a split part without slabs writes zeros
+    PairD pr = pairs[p];
+    slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, k0, tid);
+    slab.store(As[0], Bs[0], tid);
+    __syncthreads();
+    while (true) {
+      int np = p, nk0 = k0 + BK;
+      PairD npr = pr;
+      if (nk0 >= pr.K) {
+        np = p + 1;
+        nk0 = 0;
+        if (np < blk.pe) npr = pairs[np];
+      }
+      const bool has_next = np < blk.pe && (SPLIT == 1 || left > 1);
+      if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
+      multiply();
+      if constexpr (NBUF == 2) {
+        if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
+        __syncthreads();
+        if (!has_next) break;
+        cur ^= 1;
+      } else {
+        if (!has_next) break;
+        __syncthreads();
+        slab.store(As[0], Bs[0], tid);
+        __syncthreads();
+      }
+      p = np;
+      k0 = nk0;
+      pr = npr;
+      if constexpr (SPLIT > 1) --left;
     }
-    const bool has_next = np < blk.pe;
-    if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
-    multiply();
-    if constexpr (NBUF == 2) {
-      if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
-      __syncthreads();
-      if (!has_next) break;
-      cur ^= 1;
-    } else {
-      if (!has_next) break;
-      __syncthreads();
-      slab.store(As[0], Bs[0], tid);
-      __syncthreads();
-    }
-    p = np;
-    k0 = nk0;
-    pr = npr;
   }
   // ---- epilogue: element r of each 16x16 fragment is row Mfma<T>::row(lane, r), column lane%16
   T *Cb = C + blk.c_off;
+  if constexpr (SPLIT > 1) {
+    if (part > 0) {
+      Cb = scratch + (int64_t)(part - 1) * sstride + blk.c_off;
+      sub = 0;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -818,40 +853,69 @@ void launch_narrow_tag(Ctx *c, int la, int lb, const T *A, const T *B, T *C, con
   else launch_narrow_nc<T, 16, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
 }
 
-template <typename T, int BM, int BN, int TAG, int BK, int NBUF>
+template <typename T, int BM, int BN, int TAG, int BK, int NBUF, int SPLIT = 1>
 void launch_tiles_v(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
-                    const BlkD *blks, const PairD *pairs, int sub) {
-  dim3 grid(ntiles), block(kThreads);
+                    const BlkD *blks, const PairD *pairs, int sub, T *scratch = nullptr, int64_t sstride = 0) {
+  dim3 grid((unsigned)((int64_t)ntiles * SPLIT)), block(kThreads);
   if (la == 0 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, false, false, TAG, BK, NBUF>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, false, false, TAG, BK, NBUF, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
   else if (la == 0 && lb == 1)
-    grouped_gemm_kernel<T, BM, BN, false, true, TAG, BK, NBUF>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, false, true, TAG, BK, NBUF, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
   else if (la == 1 && lb == 0)
-    grouped_gemm_kernel<T, BM, BN, true, false, TAG, BK, NBUF>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, true, false, TAG, BK, NBUF, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
   else
-    grouped_gemm_kernel<T, BM, BN, true, true, TAG, BK, NBUF>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_kernel<T, BM, BN, true, true, TAG, BK, NBUF, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
+}
+
+// C += scratch part 1 + ... + part SPLIT - 1 on every output block, in part
+// order (fixed, so split launches are reproducible).
+template <typename T, int SPLIT>
+__global__ void splitk_reduce_kernel(T *__restrict__ C, const T *__restrict__ scratch, int64_t sstride,
+                                     const BlkD *__restrict__ blks) {
+  const BlkD b = blks[blockIdx.y];
+  const int64_t n = (int64_t)b.M * b.N;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T v = C[b.c_off + i];
+#pragma unroll
+    for (int s = 0; s < SPLIT - 1; ++s) v += scratch[(int64_t)s * sstride + b.c_off + i];
+    C[b.c_off + i] = v;
+  }
+}
+
+// Split-K launch (FP32 128x128 tiles, plain writes): SPLIT parts per tile,
+// then the reduction.  out_span: one past the largest output element the
+// launch writes (c_off + M N over its blocks); nblk: its output blocks.
+template <typename T, int BM, int BN, int TAG, int SPLIT>
+void launch_split(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
+                  const BlkD *blks, const PairD *pairs, int64_t out_span, int nblk, int64_t max_mn) {
+  BufP scratch = c->alloc((size_t)(SPLIT - 1) * (size_t)out_span * sizeof(T));
+  T *sp = static_cast<T *>(scratch->p);
+  launch_tiles_v<T, BM, BN, TAG, 16, 2, SPLIT>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, 0, sp, out_span);
+  const unsigned gx = (unsigned)std::min<int64_t>((max_mn + 255) / 256, 1024);
+  splitk_reduce_kernel<T, SPLIT><<<dim3(std::max(gx, 1u), (unsigned)nblk), 256, 0, c->stream>>>(C, sp, out_span, blks);
 }
 
 // FP32 128x128 slab loading: 0 = registers -> LDS double buffer, 4 = direct-to-
-// LDS with four stages (row-major A and B).  MPS2_GEMM_VARIANT (0 or 4)
-// forces one for every such launch; otherwise the caller's choice runs.
+// LDS with four stages (row-major A and B), 8 / 9 = split-K in 2 / 4 parts
+// (launches that pass their output extent).  MPS2_GEMM_VARIANT (0, 4, 8 or
+// 9) forces one for every such launch; otherwise the caller's choice runs.
 int forced_variant() {
   static const int v = [] {
     const char *e = std::getenv("MPS2_GEMM_VARIANT");
     if (!e) return -1;
     const int x = std::atoi(e);
-    return x == 0 || x == 4 ? x : -1;
+    return x == 0 || x == 4 || x == 8 || x == 9 ? x : -1;
   }();
   return v;
 }
 
 template <typename T, int BM, int BN, int TAG>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
-                  const BlkD *blks, const PairD *pairs, int sub, int variant) {
+                  const BlkD *blks, const PairD *pairs, int sub, int variant, const SplitInfo &si) {
   if constexpr (sizeof(T) == 4) {
     if (c->f32_split) {
       launch_tiles_bf3<BM, BN, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
@@ -864,6 +928,11 @@ void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const Ti
     if (v == 4 && la == 0 && lb == 0) {
       grouped_gemm_dma_kernel<4, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,
                                                                                      ntiles, sub);
+      return;
+    }
+    if ((v == 8 || v == 9) && sub == 0 && si.out_span > 0 && si.nblk > 0 && si.nblk <= 65535) {
+      if (v == 8) launch_split<T, BM, BN, TAG, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, si.out_span, si.nblk, si.max_mn);
+      else launch_split<T, BM, BN, TAG, 4>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, si.out_span, si.nblk, si.max_mn);
       return;
     }
   }
@@ -966,18 +1035,18 @@ std::vector<TileD> schedule_tiles(const std::vector<BlkD> &blks, const std::vect
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag,
-                         int variant) {
+                         int variant, const SplitInfo &si) {
   if (ntiles <= 0) return;
   const bool big = tile_m == 128 && tile_n == 128;
   if (tag == 1) {
-    if (big) launch_tiles<T, 128, 128, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
-    else launch_tiles<T, 64, 64, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
+    if (big) launch_tiles<T, 128, 128, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant, si);
+    else launch_tiles<T, 64, 64, 1>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant, si);
   } else if (tag == 2) {
-    if (big) launch_tiles<T, 128, 128, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
-    else launch_tiles<T, 64, 64, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
+    if (big) launch_tiles<T, 128, 128, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant, si);
+    else launch_tiles<T, 64, 64, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant, si);
   } else {
-    if (big) launch_tiles<T, 128, 128, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
-    else launch_tiles<T, 64, 64, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant);
+    if (big) launch_tiles<T, 128, 128, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant, si);
+    else launch_tiles<T, 64, 64, 0>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub, variant, si);
   }
   c->gemm_launches++;
 }
@@ -1006,8 +1075,8 @@ template void launch_narrow_gemm<float>(Ctx *, int, int, const float *, const fl
                                         const BlkD *, const PairD *, int, int, int);
 
 template void launch_grouped_gemm<double>(Ctx *, int, int, const double *, const double *, double *, const TileMap &,
-                                          int, const BlkD *, const PairD *, int, int, int, int, int);
+                                          int, const BlkD *, const PairD *, int, int, int, int, int, const SplitInfo &);
 template void launch_grouped_gemm<float>(Ctx *, int, int, const float *, const float *, float *, const TileMap &, int,
-                                         const BlkD *, const PairD *, int, int, int, int, int);
+                                         const BlkD *, const PairD *, int, int, int, int, int, const SplitInfo &);
 
 }  // namespace mps2

@@ -239,6 +239,17 @@ std::shared_ptr<DevPlan> build_dev_plan(Ctx *c, const std::shared_ptr<ContractPl
       dp->map_narrow.list = static_cast<const TileD *>(dp->tilemap->p);
     }
   }
+  for (const BlkD &bk : blks) {
+    const int64_t mn = (int64_t)bk.M * bk.N;
+    dp->split.out_span = std::max(dp->split.out_span, bk.c_off + mn);
+    dp->split.max_mn = std::max(dp->split.max_mn, mn);
+  }
+  dp->split.nblk = (int)blks.size();
+  {
+    double mn = 0;
+    for (const BlkD &bk : blks) mn += (double)bk.M * bk.N;
+    dp->split.avg_k = mn > 0 ? hp->flops / (2.0 * mn) : 0.0;
+  }
   dp->blks = upload_bytes(c, blks.data(), blks.size() * sizeof(BlkD));
   dp->pairs = upload_bytes(c, pairs.data(), pairs.size() * sizeof(PairD));
   if (hp->permute_a) {
@@ -435,7 +446,8 @@ void run_plan(Ctx *c, const DevPlan &dp, const TenP &a, const TenP &b, T *out) {
   }
   tune_probe_begin(c, dp, fp64, &probe);
   launch_grouped_gemm<T>(c, dp.hp->la, dp.hp->lb, A, B, out, tiles, ntiles, static_cast<const BlkD *>(dp.blks->p),
-                         static_cast<const PairD *>(dp.pairs->p), tm_, tn_, 0, tag, ch.variant);
+                         static_cast<const PairD *>(dp.pairs->p), tm_, tn_, 0, tag, tag == 2 ? 0 : ch.variant,
+                         dp.split);
   tune_probe_end(c, dp, fp64, &probe);
   if (c->time_gemms) {
     MPS2_HIP(hipEventRecord(tm.stop, c->stream));

@@ -289,19 +289,28 @@ struct TileMap {
   const int32_t *start = nullptr;
   int nb = 0, tm = 0, tn = 0;
 };
+// Output extent of a launch, for split-K: one past the largest element it
+// writes, its output blocks, the largest block's M N.
+struct SplitInfo {
+  int64_t out_span = 0;
+  int nblk = 0;
+  int64_t max_mn = 0;
+  double avg_k = 0;   // flops / (2 sum M N): the mean K of an output element
+};
 // variant (FP32 128x128 tiles only): 0 = register-staged slabs, 4 = direct-to-LDS
-// slabs with four stages (row-major A and B only).
+// slabs with four stages (row-major A and B only), 8 / 9 = split-K in 2 / 4
+// parts per tile (plain writes; needs si).
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
-                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0,
-                         int variant = 0);
+                         const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag,
+                         int variant, const SplitInfo &si);
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileD *tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub = 0, int tag = 0,
                          int variant = 0) {
   TileMap m;
   m.list = tiles;
-  launch_grouped_gemm<T>(c, la, lb, A, B, C, m, ntiles, blks, pairs, tile_m, tile_n, sub, tag, variant);
+  launch_grouped_gemm<T>(c, la, lb, A, B, C, m, ntiles, blks, pairs, tile_m, tile_n, sub, tag, variant, SplitInfo{});
 }
 // Narrow contractions (kNarrowMaxN, meta.h) run on the vector ALUs
 // (narrow_gemm_kernel, gemm.hip): one thread per output row and nc (4, 8 or
@@ -358,6 +367,7 @@ struct DevPlan {
   BufP tilemap, tiles64, tiles128;
   TileMap map64, map128;
   int ntiles64 = 0, ntiles128 = 0;
+  SplitInfo split;                // output extent (split-K launches)
   // complex plans (hp in real terms, complex_view in ops.hip): the expansion
   // descriptors of B's blocks and the largest K x N among them
   bool cplx = false;

@@ -74,7 +74,7 @@ void load_cache_locked() {
     int code = 0, tile = 0, variant = 0;
     double best = 0, heur = 0;
     if (!(s >> key >> code >> tile >> variant >> best >> heur)) continue;
-    if ((tile != 64 && tile != 128) || (variant != 0 && variant != 4) || code < 0 || code > 2) continue;
+    if ((tile != 64 && tile != 128) || (variant != 0 && variant != 4 && variant != 9) || code < 0 || code > 2) continue;
     CacheEntry ce;
     ce.choice = GemmChoice{tile, variant};
     ce.best_ms = best;
@@ -154,8 +154,18 @@ PlanTune::~PlanTune() {
   }
 }
 
-std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split) {
+std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split, int num_cu) {
   std::vector<GemmChoice> c;
+  // FP32 launches whose 128x128 tiles fill fewer than two rounds of the
+  // compute units (three workgroups each) with long K run split-K in four
+  // parts, untuned: a fixed rule, so the result does not depend on timing
+  // (the centre-bond stage 4 at D=4096, 978 tiles of K ~ 16 thousand:
+  // 4.64 -> 4.14 ms; 2 parts: 4.50 ms).
+  if (!fp64 && !split && dp.tile_m == 128 && dp.ntiles128 < 6 * num_cu && dp.split.nblk <= 65535 &&
+      dp.split.avg_k >= 2048) {
+    c.push_back(GemmChoice{128, 9});
+    return c;
+  }
   const GemmChoice h = fp64 ? GemmChoice{64, 0} : GemmChoice{dp.tile_m, 0};
   c.push_back(h);
   if (split) {
@@ -177,7 +187,7 @@ GemmChoice tune_select(Ctx *c, const DevPlan &dp, bool fp64, PlanTune::Probe *pr
   const int idx = tune_index(c, fp64), code = class_code(idx);
   PlanTune &t = dp.tune[idx];
   probe->active = false;
-  if (t.cand.empty()) t.cand = tune_candidates(dp, fp64, idx == 2);
+  if (t.cand.empty()) t.cand = tune_candidates(dp, fp64, idx == 2, c->num_cu);
   if (!tuning_enabled_env() || !c->tune_mode) return t.cand[0];
   if (t.decided >= 0) return t.cand[t.decided];
   if (dp.hp->flops < kTuneMinFlops || t.cand.size() < 2) return t.cand[0];

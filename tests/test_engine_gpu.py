@@ -541,6 +541,36 @@ def test_fp32_large_tile_launch_vs_numpy(ctx, prec):
         assert (np.abs(got.blocks[c] - ref.blocks[c]) <= low_prec_bound(prec, 60) * bound.blocks[c] + 1e-30).all(), c
 
 
+def test_fp32_split_k_launch_vs_oracle(ctx):
+    """A contraction with few 128x128 tiles (4 output blocks of about 300
+    square: 36 tiles) and long K (3 block pairs of about 1200 per output
+    block) takes the FP32 split-K launch (four parts per tile, added in part
+    order by the reduction): every element within the worst-case bound of
+    FP32 accumulation, (K + 2) 2^-24 sum_k |a||b| (operands rounded to FP32,
+    K additions), of the oracle, and bit-identical on a second run."""
+    from mps2_amd.flat import Leg
+    rng = np.random.default_rng(41)
+    m = Leg(IN, [((q,), 290 + 7 * q) for q in range(4)])
+    n = Leg(OUT, [((q,), 301 - 5 * q) for q in range(4)])
+    k1 = Leg(OUT, [((q,), 40 + q) for q in range(-3, 4)])
+    k2 = Leg(OUT, [((q,), 28 + 2 * q) for q in range(3)])
+    A = BTensor.random([m, k1, k2], (0,), rng)
+    B = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
+    dA, dB = ctx.put(A), ctx.put(B)
+    got = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
+    again = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED).get()
+    ref = O.contract(A, B, [[1, 2], [0, 1]])
+    assert sorted(got.blocks) == sorted(ref.blocks) and len(ref.blocks) >= 3
+    absA = BTensor(A.legs, A.qw, {c: np.abs(b) for c, b in A.blocks.items()})
+    absB = BTensor(B.legs, B.qw, {c: np.abs(b) for c, b in B.blocks.items()})
+    bound = O.contract(absA, absB, [[1, 2], [0, 1]])
+    kmax = sum(d for _, d in k1.sectors) * sum(d for _, d in k2.sectors)
+    for c in ref.blocks:
+        assert np.array_equal(got.blocks[c], again.blocks[c]), c
+        tol = (kmax + 2) * 2.0 ** -24
+        assert (np.abs(got.blocks[c] - ref.blocks[c]) <= tol * bound.blocks[c] + 1e-30).all(), c
+
+
 def test_autotuned_launches_are_bit_identical():
     """The GEMM autotuner times its candidate kernels on the first launches of
     a large plan (>= 1e9 flops) and keeps the fastest; every candidate has
