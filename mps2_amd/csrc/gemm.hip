@@ -899,10 +899,11 @@ void launch_split(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const Ti
   splitk_reduce_kernel<T, SPLIT><<<dim3(std::max(gx, 1u), (unsigned)nblk), 256, 0, c->stream>>>(C, sp, out_span, blks);
 }
 
-// FP32 128x128 slab loading: 0 = registers -> LDS double buffer, 4 = direct-to-
-// LDS with four stages (row-major A and B), 8 / 9 = split-K in 2 / 4 parts
-// (launches that pass their output extent).  MPS2_GEMM_VARIANT (0, 4, 8 or
-// 9) forces one for every such launch; otherwise the caller's choice runs.
+// Variants: 0 = registers -> LDS double buffer; 4 = direct-to-LDS with four
+// stages (FP32 128x128 tiles of row-major A and B); 8 / 9 = split-K in 2 / 4
+// parts (FP32 128x128 and FP64 64x64 tiles, launches that pass their output
+// extent).  MPS2_GEMM_VARIANT (0, 4, 8 or 9) forces one for every such
+// launch; otherwise the caller's choice runs.
 int forced_variant() {
   static const int v = [] {
     const char *e = std::getenv("MPS2_GEMM_VARIANT");
@@ -922,14 +923,17 @@ void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const Ti
       return;
     }
   }
+  const int f = forced_variant();
+  const int v = f >= 0 ? f : variant;
   if constexpr (sizeof(T) == 4 && BM == 128) {
-    const int f = forced_variant();
-    const int v = f >= 0 ? f : variant;
     if (v == 4 && la == 0 && lb == 0) {
       grouped_gemm_dma_kernel<4, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,
                                                                                      ntiles, sub);
       return;
     }
+  }
+  // split-K: FP32 128x128 and FP64 64x64 tiles
+  if constexpr ((sizeof(T) == 4 && BM == 128) || (sizeof(T) == 8 && BM == 64)) {
     if ((v == 8 || v == 9) && sub == 0 && si.out_span > 0 && si.nblk > 0 && si.nblk <= 65535) {
       if (v == 8) launch_split<T, BM, BN, TAG, 2>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, si.out_span, si.nblk, si.max_mn);
       else launch_split<T, BM, BN, TAG, 4>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, si.out_span, si.nblk, si.max_mn);

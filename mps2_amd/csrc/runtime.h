@@ -297,9 +297,10 @@ struct SplitInfo {
   int64_t max_mn = 0;
   double avg_k = 0;   // flops / (2 sum M N): the mean K of an output element
 };
-// variant (FP32 128x128 tiles only): 0 = register-staged slabs, 4 = direct-to-LDS
-// slabs with four stages (row-major A and B only), 8 / 9 = split-K in 2 / 4
-// parts per tile (plain writes; needs si).
+// variant: 0 = register-staged slabs; 4 = direct-to-LDS slabs with four
+// stages (FP32 128x128 tiles, row-major A and B only); 8 / 9 = split-K in
+// 2 / 4 parts per tile (FP32 128x128 or FP64 64x64 tiles, plain writes;
+// needs si).
 template <typename T>
 void launch_grouped_gemm(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                          const BlkD *blks, const PairD *pairs, int tile_m, int tile_n, int sub, int tag,

@@ -156,14 +156,24 @@ PlanTune::~PlanTune() {
 
 std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split, int num_cu) {
   std::vector<GemmChoice> c;
-  // FP32 launches whose 128x128 tiles fill fewer than two rounds of the
-  // compute units (three workgroups each) with long K run split-K in four
-  // parts, untuned: a fixed rule, so the result does not depend on timing
-  // (the centre-bond stage 4 at D=4096, 978 tiles of K ~ 16 thousand:
-  // 4.64 -> 4.14 ms; 2 parts: 4.50 ms).
-  if (!fp64 && !split && dp.tile_m == 128 && dp.ntiles128 < 6 * num_cu && dp.split.nblk <= 65535 &&
-      dp.split.avg_k >= 2048) {
+  // Split-K in four parts, untuned (a fixed rule, so the result does not
+  // depend on timing), for launches with long K (mean 2048 or more) whose
+  // last round of workgroups would leave the compute units idle: four parts
+  // per tile when that shortens the rounds of resident workgroups (slots:
+  // three FP32 128x128 or four FP64 64x64 workgroups per compute unit) by
+  // 10% or more.  The centre-bond stage 4 at D=4096 (978 FP32 tiles, 3318
+  // FP64 tiles, K ~ 16 thousand): FP32 4.64 -> 4.14 ms (2 parts: 4.50 ms).
+  auto split_pays = [&](int64_t tiles, int64_t slots) {
+    if (tiles <= 0 || slots <= 0 || dp.split.avg_k < 2048 || dp.split.nblk > 65535) return false;
+    const double r1 = std::ceil((double)tiles / slots), r4 = std::ceil(4.0 * tiles / slots) / 4.0;
+    return r4 < 0.9 * r1;
+  };
+  if (!fp64 && !split && dp.tile_m == 128 && split_pays(dp.ntiles128, 3 * (int64_t)num_cu)) {
     c.push_back(GemmChoice{128, 9});
+    return c;
+  }
+  if (fp64 && split_pays(dp.ntiles64, 4 * (int64_t)num_cu)) {
+    c.push_back(GemmChoice{64, 9});
     return c;
   }
   const GemmChoice h = fp64 ? GemmChoice{64, 0} : GemmChoice{dp.tile_m, 0};

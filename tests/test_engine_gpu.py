@@ -541,7 +541,7 @@ def test_fp32_large_tile_launch_vs_numpy(ctx, prec):
         assert (np.abs(got.blocks[c] - ref.blocks[c]) <= low_prec_bound(prec, 60) * bound.blocks[c] + 1e-30).all(), c
 
 
-def test_fp32_split_k_launch_vs_oracle(ctx):
+def test_split_k_launch_vs_oracle(ctx):
     """A contraction with few 128x128 tiles (4 output blocks of about 300
     square: 36 tiles) and long K (3 block pairs of about 1200 per output
     block) takes the FP32 split-K launch (four parts per tile, added in part
@@ -569,6 +569,15 @@ def test_fp32_split_k_launch_vs_oracle(ctx):
         assert np.array_equal(got.blocks[c], again.blocks[c]), c
         tol = (kmax + 2) * 2.0 ** -24
         assert (np.abs(got.blocks[c] - ref.blocks[c]) <= tol * bound.blocks[c] + 1e-30).all(), c
+    # FP64: the same shape runs split-K on 64x64 tiles (100 tiles, fewer than
+    # one round of the compute units); the engine's FP64 bar, bit-identical
+    # on a second run
+    got64 = E.contract(dA, dB, [[1, 2], [0, 1]]).get()
+    again64 = E.contract(dA, dB, [[1, 2], [0, 1]]).get()
+    assert same_blocks(got64, ref)
+    assert H.max_rel_diff(got64, ref) < TOL
+    for c in ref.blocks:
+        assert np.array_equal(got64.blocks[c], again64.blocks[c]), c
 
 
 def test_autotuned_launches_are_bit_identical():
