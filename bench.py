@@ -337,8 +337,8 @@ def main():
     dom_ach = dom["dom_flops"] / (dom["dom_ms"] * 1e-3) / 1e12 if dom["dom_ms"] > 0 else 0.0
     roofline = dict(bound="mfma", achieved=round(dom_ach, 3), peak=PEAK["fp32"], unit="TFLOP/s",
                     frac=round(dom_ach / PEAK["fp32"], 4), traffic=measured_traffic(args),
-                    kernel=f"grouped_gemm_kernel<float, ..., 1> (stage {dom['dom_stage'] + 1} of the centre-bond "
-                           f"matvec, run as the tagged instance)",
+                    kernel=f"grouped_gemm_kernel<float, ..., 1, ...> (stage {dom['dom_stage'] + 1} of the centre-bond "
+                           f"matvec, run as the tagged instance; a split-K launch with its reduction)",
                     flops_per_launch=dom["dom_flops"], ms_per_launch=dom["dom_ms"],
                     algorithmic_bytes_per_launch=dom.get("dom_bytes"))
     d64 = mb["fp64"]
