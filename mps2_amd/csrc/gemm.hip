@@ -656,14 +656,16 @@ struct Bf3Slab {
   }
 };
 
-template <int BM, int BN, bool TA, bool TB, int TAG>
+// SPLIT > 1: split-K as in grouped_gemm_kernel.
+template <int BM, int BN, bool TA, bool TB, int TAG, int SPLIT>
 __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float *__restrict__ A,
                                                                      const float *__restrict__ B,
                                                                      float *__restrict__ C,
                                                                      TileMap tiles,
                                                                      const BlkD *__restrict__ blks,
                                                                      const PairD *__restrict__ pairs, int ntiles,
-                                                                     int sub) {
+                                                                     int sub, float *__restrict__ scratch,
+                                                                     int64_t sstride) {
   using V4 = Mfma<float>::v4;
   using SlabT = Bf3Slab<BM, BN, TA, TB>;
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -676,7 +678,11 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int tix = blockIdx.x;
+  int tix = blockIdx.x, part = 0;
+  if constexpr (SPLIT > 1) {
+    part = tix % SPLIT;
+    tix /= SPLIT;
+  }
   if (tix >= ntiles) return;
   const TileD tile = tile_at(tiles, blks, tix);
   if (tile.blk < 0) return;
@@ -684,6 +690,21 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
   const int M = blk.M, N = blk.N;
   const int m0 = tile.m0, n0 = tile.n0;
   if (m0 >= M || n0 >= N || blk.pb >= blk.pe) return;
+  int p = blk.pb, k0 = 0, left = 0x7fffffff;
+  if constexpr (SPLIT > 1) {
+    int total = 0;
+    for (int q = blk.pb; q < blk.pe; ++q) total += (pairs[q].K + BK - 1) / BK;
+    const int j0 = (int)((int64_t)total * part / SPLIT);
+    left = (int)((int64_t)total * (part + 1) / SPLIT) - j0;
+    int j = j0;
+    while (p < blk.pe) {
+      const int ns = (pairs[p].K + BK - 1) / BK;
+      if (j < ns) break;
+      j -= ns;
+      ++p;
+    }
+    k0 = j * BK;
+  }
 
   V4 acc[FM][FN];
 #pragma unroll
@@ -692,13 +713,13 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
     for (int j = 0; j < FN; ++j) acc[i][j] = V4{0, 0, 0, 0};
 
   SlabT slab;
-  int p = blk.pb, k0 = 0;
-  PairD pr = pairs[p];
-  slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, 0, tid);
-  slab.store(As[0], Bs[0], tid);
-  __syncthreads();
   int cur = 0;
   const int frag_row = (lane & 15) * RS + 8 * (lane >> 4);
+  if (left > 0) {  // uniform: a split part without slabs writes zeros
+  PairD pr = pairs[p];
+  slab.load(A + pr.a_off, B + pr.b_off, M, N, pr.K, m0, n0, k0, tid);
+  slab.store(As[0], Bs[0], tid);
+  __syncthreads();
   while (true) {
     int np = p, nk0 = k0 + BK;
     PairD npr = pr;
@@ -707,7 +728,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
       nk0 = 0;
       if (np < blk.pe) npr = pairs[np];
     }
-    const bool has_next = np < blk.pe;
+    const bool has_next = np < blk.pe && (SPLIT == 1 || left > 1);
     if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
     {
       const __bf16 *as = As[cur] + wm * WM * RS + frag_row;
@@ -745,8 +766,16 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
     p = np;
     k0 = nk0;
     pr = npr;
+    if constexpr (SPLIT > 1) --left;
+  }
   }
   float *Cb = C + blk.c_off;
+  if constexpr (SPLIT > 1) {
+    if (part > 0) {
+      Cb = scratch + (int64_t)(part - 1) * sstride + blk.c_off;
+      sub = 0;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -763,22 +792,23 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_bf3_kernel(const float 
     }
 }
 
-template <int BM, int BN, int TAG>
+template <int BM, int BN, int TAG, int SPLIT = 1>
 void launch_tiles_bf3(Ctx *c, int la, int lb, const float *A, const float *B, float *C, const TileMap &tiles,
-                      int ntiles, const BlkD *blks, const PairD *pairs, int sub) {
-  dim3 grid(ntiles), block(kThreads);
+                      int ntiles, const BlkD *blks, const PairD *pairs, int sub, float *scratch = nullptr,
+                      int64_t sstride = 0) {
+  dim3 grid((unsigned)((int64_t)ntiles * SPLIT)), block(kThreads);
   if (la == 0 && lb == 0)
-    grouped_gemm_bf3_kernel<BM, BN, false, false, TAG>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_bf3_kernel<BM, BN, false, false, TAG, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
   else if (la == 0 && lb == 1)
-    grouped_gemm_bf3_kernel<BM, BN, false, true, TAG>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_bf3_kernel<BM, BN, false, true, TAG, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
   else if (la == 1 && lb == 0)
-    grouped_gemm_bf3_kernel<BM, BN, true, false, TAG>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_bf3_kernel<BM, BN, true, false, TAG, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
   else
-    grouped_gemm_bf3_kernel<BM, BN, true, true, TAG>
-        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub);
+    grouped_gemm_bf3_kernel<BM, BN, true, true, TAG, SPLIT>
+        <<<grid, block, 0, c->stream>>>(A, B, C, tiles, blks, pairs, ntiles, sub, scratch, sstride);
 }
 
 // ---------------------------------------------------------------- narrow outputs (vector ALU)
@@ -917,14 +947,23 @@ int forced_variant() {
 template <typename T, int BM, int BN, int TAG>
 void launch_tiles(Ctx *c, int la, int lb, const T *A, const T *B, T *C, const TileMap &tiles, int ntiles,
                   const BlkD *blks, const PairD *pairs, int sub, int variant, const SplitInfo &si) {
+  const int f = forced_variant();
+  const int v = f >= 0 ? f : variant;
   if constexpr (sizeof(T) == 4) {
     if (c->f32_split) {
-      launch_tiles_bf3<BM, BN, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+      if (v == 9 && sub == 0 && si.out_span > 0 && si.nblk > 0 && si.nblk <= 65535) {
+        BufP scratch = c->alloc((size_t)3 * (size_t)si.out_span * sizeof(float));
+        float *sp = static_cast<float *>(scratch->p);
+        launch_tiles_bf3<BM, BN, TAG, 4>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, 0, sp, si.out_span);
+        const unsigned gx = (unsigned)std::min<int64_t>((si.max_mn + 255) / 256, 1024);
+        splitk_reduce_kernel<float, 4><<<dim3(std::max(gx, 1u), (unsigned)si.nblk), 256, 0, c->stream>>>(
+            C, sp, si.out_span, blks);
+      } else {
+        launch_tiles_bf3<BM, BN, TAG>(c, la, lb, A, B, C, tiles, ntiles, blks, pairs, sub);
+      }
       return;
     }
   }
-  const int f = forced_variant();
-  const int v = f >= 0 ? f : variant;
   if constexpr (sizeof(T) == 4 && BM == 128) {
     if (v == 4 && la == 0 && lb == 0) {
       grouped_gemm_dma_kernel<4, TAG><<<dim3(ntiles), dim3(kThreads), 0, c->stream>>>(A, B, C, tiles, blks, pairs,

@@ -176,6 +176,15 @@ std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split
     c.push_back(GemmChoice{64, 9});
     return c;
   }
+  // Split-BF16x3 (two workgroups per compute unit) is bound by the operand
+  // panels' L2 misses; four parts per tile put the same part of neighbouring
+  // tiles on one XCD at once, so very long K pays there by locality alone
+  // (the centre-bond stage 4: 2.46 -> 2.21 ms).
+  if (split && dp.tile_m == 128 && dp.split.avg_k >= 4096 && dp.ntiles128 < 8 * (int64_t)num_cu &&
+      dp.split.nblk <= 65535) {
+    c.push_back(GemmChoice{128, 9});
+    return c;
+  }
   const GemmChoice h = fp64 ? GemmChoice{64, 0} : GemmChoice{dp.tile_m, 0};
   c.push_back(h);
   if (split) {

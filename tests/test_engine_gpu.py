@@ -543,16 +543,18 @@ def test_fp32_large_tile_launch_vs_numpy(ctx, prec):
 
 def test_split_k_launch_vs_oracle(ctx):
     """A contraction with few 128x128 tiles (4 output blocks of about 300
-    square: 36 tiles) and long K (3 block pairs of about 1200 per output
-    block) takes the FP32 split-K launch (four parts per tile, added in part
-    order by the reduction): every element within the worst-case bound of
-    FP32 accumulation, (K + 2) 2^-24 sum_k |a||b| (operands rounded to FP32,
-    K additions), of the oracle, and bit-identical on a second run."""
+    square: 36 tiles) and long K (3 block pairs of about 1500 per output
+    block) takes the split-K launches (four parts per tile, added in part
+    order by the reduction) in all three arithmetic forms.  FP32: every
+    element within the worst-case bound of FP32 accumulation, (K + 2) 2^-24
+    sum_k |a||b| (operands rounded to FP32, K additions); split-BF16x3 within
+    its bound (low_prec_bound); FP64 to the engine's FP64 bar; each
+    bit-identical on a second run."""
     from mps2_amd.flat import Leg
     rng = np.random.default_rng(41)
     m = Leg(IN, [((q,), 290 + 7 * q) for q in range(4)])
     n = Leg(OUT, [((q,), 301 - 5 * q) for q in range(4)])
-    k1 = Leg(OUT, [((q,), 40 + q) for q in range(-3, 4)])
+    k1 = Leg(OUT, [((q,), 50 + q) for q in range(-3, 4)])
     k2 = Leg(OUT, [((q,), 28 + 2 * q) for q in range(3)])
     A = BTensor.random([m, k1, k2], (0,), rng)
     B = BTensor.random([k1.inverse(), k2.inverse(), n], (0,), rng)
@@ -569,6 +571,12 @@ def test_split_k_launch_vs_oracle(ctx):
         assert np.array_equal(got.blocks[c], again.blocks[c]), c
         tol = (kmax + 2) * 2.0 ** -24
         assert (np.abs(got.blocks[c] - ref.blocks[c]) <= tol * bound.blocks[c] + 1e-30).all(), c
+    bf3 = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED_BF16X3).get()
+    bf3_again = E.contract_ex(dA, dB, [[1, 2], [0, 1]], E.PREC_MIXED_BF16X3).get()
+    for c in ref.blocks:
+        assert np.array_equal(bf3.blocks[c], bf3_again.blocks[c]), c
+        tol = low_prec_bound(E.PREC_MIXED_BF16X3, kmax)
+        assert (np.abs(bf3.blocks[c] - ref.blocks[c]) <= tol * bound.blocks[c] + 1e-30).all(), c
     # FP64: the same shape runs split-K on 64x64 tiles (100 tiles, fewer than
     # one round of the compute units); the engine's FP64 bar, bit-identical
     # on a second run
