@@ -163,8 +163,10 @@ std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split
   // three FP32 128x128 or four FP64 64x64 workgroups per compute unit) by
   // 10% or more.  The centre-bond stage 4 at D=4096 (978 FP32 tiles, 3318
   // FP64 tiles, K ~ 16 thousand): FP32 4.64 -> 4.14 ms (2 parts: 4.50 ms).
+  // (the three partial copies stay under 2 GiB of scratch)
+  const bool split_fits = dp.split.nblk <= 65535 && 3 * dp.split.out_span * (fp64 ? 8 : 4) <= (int64_t(1) << 31);
   auto split_pays = [&](int64_t tiles, int64_t slots) {
-    if (tiles <= 0 || slots <= 0 || dp.split.avg_k < 2048 || dp.split.nblk > 65535) return false;
+    if (tiles <= 0 || slots <= 0 || dp.split.avg_k < 2048 || !split_fits) return false;
     const double r1 = std::ceil((double)tiles / slots), r4 = std::ceil(4.0 * tiles / slots) / 4.0;
     return r4 < 0.9 * r1;
   };
@@ -180,8 +182,7 @@ std::vector<GemmChoice> tune_candidates(const DevPlan &dp, bool fp64, bool split
   // panels' L2 misses; four parts per tile put the same part of neighbouring
   // tiles on one XCD at once, so very long K pays there by locality alone
   // (the centre-bond stage 4: 2.46 -> 2.21 ms).
-  if (split && dp.tile_m == 128 && dp.split.avg_k >= 4096 && dp.ntiles128 < 8 * (int64_t)num_cu &&
-      dp.split.nblk <= 65535) {
+  if (split && dp.tile_m == 128 && dp.split.avg_k >= 4096 && dp.ntiles128 < 8 * (int64_t)num_cu && split_fits) {
     c.push_back(GemmChoice{128, 9});
     return c;
   }
