@@ -5,10 +5,10 @@
 //     Q = H_0 H_1 ... H_{n-2}, H_c = I - tau_c v_c v_c^T, v_c[c+1] = 1), one
 //     column per step for every sector at once, with the trailing update
 //     deferred over panels of kPB reflectors (see the tridiagonalisation
-//     section): per column one workgroup per sector finishes w_{c-1} and
-//     builds the reflector v_c, and many workgroups form y = S v_c from the
-//     stored trailing square without writing it; every kPB columns a tiled
-//     pass applies the panel's rank-2kPB update.
+//     section): per column a row-parallel kernel finishes w_{c-1} and forms
+//     column c, a second one builds the reflector v_c and forms y = S v_c
+//     from the stored trailing square without writing it; every kPB columns
+//     a tiled pass applies the panel's rank-2kPB update.
 //  2. Tridiagonal divide and conquer (Cuppen), bottom-up from 1x1 blocks,
 //     every merge of one level in the same launches.  A merge of adjacent
 //     blocks diag(T1', T2') + rho u u^T (u = e_last + sign(beta) e_first,
@@ -39,7 +39,6 @@ namespace {
 
 constexpr int kNB = 128;  // compact-WY panel width of the back-transformation
 constexpr int kT = 256;   // threads of the per-sector kernels
-constexpr int kSymvRows = 4;  // rows per product workgroup (one per wavefront)
 // strides of 64 columns whose loads the product kernel keeps in flight at
 // once (16 against 4: 0.3% faster on a 1700-column sector batch)
 constexpr int kSymvChunk = 16;
@@ -54,10 +53,10 @@ __device__ __forceinline__ double okey(double v) { return v == v ? v : INFINITY;
 struct ESec {
   int64_t nn;   // offset in n*n buffers
   int64_t nv;   // offset in n-vectors
-  int64_t np;   // unused
+  int64_t ps;   // first product-partial workgroup of the tridiagonalisation
   int64_t tb;   // offset in T blocks (npanels * kNB * kNB)
   int n;
-  int pad;
+  int pe;       // first reflector-partial workgroup of the tridiagonalisation
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -99,103 +98,83 @@ __device__ double block_max(double v, double *red) {
 // the HBM traffic of a rank-2 update per column) and the deferred update is
 // one tiled read-modify-write pass per kPB columns.
 //
-// Per column c, every sector of the batch in the same launches:
-//   row kernel (one workgroup per sector): from y = S v_{c-1} (previous
-//     product) and the panel projections p_i = w_i.v_{c-1}, s_i = v_i.v_{c-1}
-//     (i in the panel of c - 1, before it; reduced here together with
-//     y.v_{c-1}), finish w_{c-1} = tau (y - sum_i (v_i p_i + w_i s_i)) + alpha v_{c-1}
-//     with alpha = -tau^2/2 (y.v_{c-1} - 2 p.s); form column c of A_true from
-//     row c of S and the pending (v_i, w_i); build the reflector v_c (d[c],
-//     e[c], tau[c]);
-//   update kernel (first column c = k > 0 of a panel): S -= the previous
+// Per column c, every sector of the batch in the same phase, no phase on a
+// single workgroup:
+//   reflector phase (one thread per row r >= c): sums the product phase's
+//     per-workgroup partials of y.v_{c-1} and of the panel projections
+//     p_i = w_i.v_{c-1}, s_i = v_i.v_{c-1} (i in the panel of c - 1, before
+//     it), finishes w_{c-1} = tau (y - sum_i (v_i p_i + w_i s_i)) + alpha v_{c-1}
+//     with alpha = -tau^2/2 (y.v_{c-1} - 2 p.s), forms column c of A_true,
+//     a = row c of S - sum_{i pending} (v_i w_i[c] + w_i v_i[c]), sets
+//     d[c] = a[c] and writes per-workgroup partial sums of a[r]^2 (r >= c + 2);
+//   update phase (first column c = k > 0 of a panel): S -= the previous
 //     panel's terms on rows and columns > c;
-//   product kernel: y = S v_c on rows > c (S only read).
+//   product phase (one wavefront per row r > c): every workgroup forms the
+//     same reflector of column c from a[c + 1] and the partials of |a|^2
+//     (e[c], tau[c]; v_c = a / (alpha - beta) below row c + 1, formed on the
+//     fly from a and written by the wavefront of each row), y = S v_c on its
+//     rows (S only read), and per-workgroup partials of the next column's
+//     y.v_c, w_i.v_c, v_i.v_c.
+// Partials are summed in a fixed order (within a workgroup, then over the
+// workgroups in index order), so the results do not depend on scheduling.
 // S is kept as a full square (both triangles) on rows and columns > c.
-constexpr int kPB = 8;     // reflectors per deferred update
-constexpr int kTR = 512;   // threads of the row kernel
-constexpr int kUT = 64;    // update tile
+// One launch per phase (two per column, three at a panel start); per-launch
+// latency, not the work of a column, bounds the column loop at these sizes
+// (about 15 us per column).
+constexpr int kPB = 8;          // reflectors per deferred update
+constexpr int kNS = 2 * kPB;    // partials per product item: y.v, p_i (kPB - 1), s_i (kPB - 1), one unused
+constexpr int kTB = 512;        // threads of every tridiagonalisation workgroup
+constexpr int kER = kTB;        // rows per reflector item
+constexpr int kPR = kTB / 64;   // rows per product item (one per wavefront)
+constexpr int kUT = 64;         // update tile
+constexpr int kUR = kUT * 64 / kTB;  // rows per thread of an update tile
 
 struct TrdArgs {
   double *S;        // Gram matrices, n x n row-major per sector (overwritten)
   double *VH;       // reflectors, column-major per sector (column c = v_c)
   double *W0, *W1;  // w_i of even / odd panels, n x kPB column-major per sector (offset nv * kPB)
-  double *y;        // result of the last product kernel (n per sector)
-  double *a;        // column c for rows beyond the row kernel's registers (n per sector)
+  double *y;        // result of the last product phase (n per sector)
+  double *a;        // column c of A_true (n per sector)
   double *d, *e, *tau;
+  double *ps;       // product partials, kNS per item (sector base ESec::ps items)
+  double *pe;       // reflector partials of |a|^2, one per item (sector base ESec::pe)
 };
 
 __device__ __forceinline__ double *wcol(const TrdArgs &A, const ESec &S, int i) {
   return ((i / kPB) & 1 ? A.W1 : A.W0) + S.nv * kPB + (int64_t)(i % kPB) * S.n;
 }
 
-// Sums over the workgroup of v[0..16) (all threads receive them), fixed
-// order.  Within a wavefront a halving butterfly: at distance o = 32, 16, 8,
-// 4 each lane keeps half of its values (the upper half when lane & o) and
-// adds the partner's copy of that half, so after four steps a lane holds one
-// value, (lane >> 2) & 15, summed over 16 lanes; distances 2 and 1 complete
-// the wavefront sum.  17 shuffles of a double instead of 96.  red holds
-// (blockDim.x / 64 + 1) * 16 doubles.
-template <int O, int H>
-__device__ __forceinline__ void halve16(double (&v)[16], int lane) {
-  const bool up = lane & O;
-#pragma unroll
-  for (int j = 0; j < H; ++j) {
-    const double send = up ? v[j] : v[j + H];
-    const double keep = up ? v[j + H] : v[j];
-    v[j] = keep + __shfl_xor(send, O);
-  }
-}
-
-__device__ void block_sum16(double (&v)[16], double *red) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  halve16<32, 8>(v, lane);
-  halve16<16, 4>(v, lane);
-  halve16<8, 2>(v, lane);
-  halve16<4, 1>(v, lane);
-  double x = v[0];
-  x += __shfl_xor(x, 2);
-  x += __shfl_xor(x, 1);
-  __syncthreads();  // red may still be read by a previous sum
-  if ((lane & 3) == 0) red[w * 16 + (lane >> 2)] = x;
-  __syncthreads();
-  if (threadIdx.x < 16) {
-    double t = 0;
-    for (int i = 0; i < nw; ++i) t += red[i * 16 + threadIdx.x];
-    red[nw * 16 + threadIdx.x] = t;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 16; ++j) v[j] = red[nw * 16 + j];
-}
-
-// Row kernel for column c (grid: one workgroup per sector).  Thread tid owns
-// rows r = c + tid + j * kTR.  With P > 0 (all rows fit P per thread) every
-// value the step reads is loaded into registers first, in one batch (rows
-// past the end read row n - 1; unused pending slots read reflector 0 with
-// zero weight; the W buffers are zero-filled), so the kernel waits on memory
-// once; with P = 0 the rows are looped over and re-read.  The entries at row
-// c that every thread needs are staged in LDS.
-template <int P>
-__global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ secs, int c, TrdArgs A) {
-  __shared__ double red[(kTR / 64 + 1) * 16];
-  __shared__ double sh[2];
+// Reflector item: thread tid of item b of a sector owns row
+// r = c + b * kER + tid; the items of the sectors with c < n follow each
+// other.
+__device__ void reflect_item(const ESec *__restrict__ secs, int nsec, int c, int item, const TrdArgs &A) {
+  constexpr int kG = kTB / kNS;  // thread groups summing the product partials
+  __shared__ double red[kTB / 64];
+  __shared__ double grp[kG][kNS];
+  __shared__ double fin[kNS];
   __shared__ double un[2][kPB];  // v_i[c], w_i[c] of the pending slots
-  const ESec S = secs[blockIdx.x];
-  const int n = S.n;
-  if (c >= n) return;
-  constexpr int nt = kTR;
-  constexpr int PR = P > 0 ? P : 1;
-  const int tid = threadIdx.x;
+  int b = item, q = 0;
+  for (; q < nsec; ++q) {
+    const int n = secs[q].n;
+    if (c < n) {
+      const int nb = (n - c + kER - 1) / kER;
+      if (b < nb) break;
+      b -= nb;
+    }
+  }
+  if (q >= nsec) return;
+  __syncthreads();  // the shared arrays may still be read for a previous item
+  const ESec S = secs[q];
+  const int n = S.n, tid = threadIdx.x;
+  const int r = c + b * kER + tid;
+  const bool act = r < n;
+  const int rc = act ? r : n - 1;
   const int k = c - c % kPB;                             // first column of c's panel
   const int pb = c == k ? (c >= kPB ? c - kPB : c) : k;  // pending reflectors [pb, c)
   const int npend = c - pb;
   const int ncorr = npend > 0 ? npend - 1 : 0;           // [pb, c - 1): panel of c - 1, before it
   const int slast = c - 1 - pb;                          // slot of reflector c - 1 (w_{c-1} is formed here)
   const double *V = A.VH + S.nn;
-  const double *srow = A.S + S.nn + (int64_t)c * n;
-  const double *vp = V + (int64_t)(c > 0 ? c - 1 : 0) * n;
-  const double *y = A.y + S.nv;
-  double *ascr = A.a + S.nv;
   // slot i: reflector pb + i if i < npend, else reflector 0 with weight 0
   auto vcol = [&](int i) { return V + (int64_t)(i < npend ? pb + i : 0) * n; };
   auto wcl = [&](int i) { return wcol(A, S, i < npend ? pb + i : 0); };
@@ -203,144 +182,170 @@ __global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ s
     const int i = tid % kPB;
     un[tid / kPB][i] = (i < npend ? 1.0 : 0.0) * (tid < kPB ? vcol(i) : wcl(i))[c];
   }
-  // ---- loads: row c of S, y, v_{c-1} and the pending reflectors' rows
-  double sr[PR], yr[PR], vr[PR], vv[PR][kPB], ww[PR][kPB];
-  if (P > 0) {
+  // ---- loads: this row of S's row c, y, v_{c-1}, the pending reflectors;
+  // the product partials of column c - 1
+  const double sr = A.S[S.nn + (int64_t)c * n + rc];
+  const double yr = A.y[S.nv + rc];
+  const double vr = (c > 0 && act) ? V[(int64_t)(c - 1) * n + rc] : 0.0;
+  double vv[kPB], ww[kPB];
 #pragma unroll
-    for (int j = 0; j < PR; ++j) {
-      const int r = c + tid + j * nt, rc = r < n ? r : n - 1;
-      sr[j] = srow[rc];
-      yr[j] = y[rc];
-      vr[j] = (r < n ? 1.0 : 0.0) * vp[rc];
-#pragma unroll
-      for (int i = 0; i < kPB; ++i) {
-        vv[j][i] = vcol(i)[rc];
-        ww[j][i] = wcl(i)[rc];
-      }
-    }
+  for (int i = 0; i < kPB; ++i) {
+    vv[i] = vcol(i)[rc];
+    ww[i] = wcl(i)[rc];
   }
-  const double tp = c > 0 ? A.tau[S.nv + c - 1] : 0.0;
-  const double yc = y[c];
-  double wr[PR];
-#pragma unroll
-  for (int j = 0; j < PR; ++j) wr[j] = 0.0;
-  double wc = 0.0;  // w_{c-1}[c]
+  double part = 0.0;
   if (c > 0) {
-    // 1. y.v_{c-1}, p_i = w_i.v_{c-1}, s_i = v_i.v_{c-1} (i < ncorr) over rows >= c
-    double sums[16];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) sums[j] = 0.0;
-    if (P > 0) {
-#pragma unroll
-      for (int j = 0; j < PR; ++j) {
-        sums[0] = fma(yr[j], vr[j], sums[0]);
-#pragma unroll
-        for (int i = 0; i < kPB - 1; ++i) {
-          sums[1 + i] = fma(ww[j][i], vr[j], sums[1 + i]);
-          sums[8 + i] = fma(vv[j][i], vr[j], sums[8 + i]);
-        }
-      }
-    } else {
-      for (int r = c + tid; r < n; r += nt) {
-        const double x = vp[r];
-        sums[0] = fma(y[r], x, sums[0]);
-#pragma unroll
-        for (int i = 0; i < kPB - 1; ++i) {
-          sums[1 + i] = fma(wcl(i)[r], x, sums[1 + i]);
-          sums[8 + i] = fma(vcol(i)[r], x, sums[8 + i]);
-        }
-      }
-    }
-    block_sum16(sums, red);  // also makes un[] visible
+    const int nps = (n - c + kPR - 1) / kPR;  // product items of column c - 1
+    const double *P = A.ps + S.ps * kNS;
+    for (int g = tid / kNS; g < nps; g += kG) part += P[(int64_t)g * kNS + tid % kNS];
+  }
+  grp[tid / kNS][tid % kNS] = part;
+  __syncthreads();  // grp, un
+  if (tid < kNS) {
+    double t = 0.0;
+    for (int g = 0; g < kG; ++g) t += grp[g][tid];
+    fin[tid] = t;
+  }
+  __syncthreads();
+  double wr = 0.0, wc = 0.0;  // w_{c-1}[r], w_{c-1}[c]
+  if (c > 0) {
     double pv[kPB - 1], sv[kPB - 1], ps = 0.0;
 #pragma unroll
     for (int i = 0; i < kPB - 1; ++i) {
       const double m = i < ncorr ? 1.0 : 0.0;
-      pv[i] = m * sums[1 + i];
-      sv[i] = m * sums[8 + i];
+      pv[i] = m * fin[1 + i];
+      sv[i] = m * fin[kPB + i];
       ps = fma(pv[i], sv[i], ps);
     }
-    const double al = -0.5 * tp * tp * (sums[0] - 2.0 * ps);
-    // 2. w_{c-1} on rows >= c: tau (y - sum_i (v_i p_i + w_i s_i)) + alpha v_{c-1}
-    double *w = wcol(A, S, c - 1);
-    {
-      double t = yc;
+    const double tp = A.tau[S.nv + c - 1];
+    const double al = -0.5 * tp * tp * (fin[0] - 2.0 * ps);
+    double tc = A.y[S.nv + c], t = yr;
 #pragma unroll
-      for (int i = 0; i < kPB - 1; ++i) {
-        t = fma(-un[0][i], pv[i], t);
-        t = fma(-un[1][i], sv[i], t);
-      }
-      wc = fma(al, 1.0, tp * t);  // v_{c-1}[c] = 1
+    for (int i = 0; i < kPB - 1; ++i) {
+      tc = fma(-un[0][i], pv[i], tc);
+      tc = fma(-un[1][i], sv[i], tc);
+      t = fma(-vv[i], pv[i], t);
+      t = fma(-ww[i], sv[i], t);
     }
-    if (P > 0) {
-#pragma unroll
-      for (int j = 0; j < PR; ++j) {
-        double t = yr[j];
-#pragma unroll
-        for (int i = 0; i < kPB - 1; ++i) {
-          t = fma(-vv[j][i], pv[i], t);
-          t = fma(-ww[j][i], sv[i], t);
-        }
-        wr[j] = fma(al, vr[j], tp * t);
-        const int r = c + tid + j * nt;
-        if (r < n) w[r] = wr[j];
-      }
-    } else {
-      for (int r = c + tid; r < n; r += nt) {
-        double t = y[r];
-#pragma unroll
-        for (int i = 0; i < kPB - 1; ++i) {
-          t = fma(-vcol(i)[r], pv[i], t);
-          t = fma(-wcl(i)[r], sv[i], t);
-        }
-        w[r] = fma(al, vp[r], tp * t);
-      }
-    }
-  } else {
-    __syncthreads();  // un[] (the sums above carry this barrier otherwise)
+    wc = fma(al, 1.0, tp * tc);  // v_{c-1}[c] = 1
+    wr = fma(al, vr, tp * t);
+    if (act) wcol(A, S, c - 1)[r] = wr;
   }
-  // 3. column c of A_true, a[r] = S[c][r] - sum_{i in [pb, c)} (v_i[r] w_i[c] + w_i[r] v_i[c])
-  double vic[kPB], wic[kPB];
+  // ---- column c of A_true on this row
+  double a = sr;
 #pragma unroll
   for (int i = 0; i < kPB; ++i) {
-    vic[i] = un[0][i];
-    wic[i] = i == slast ? wc : un[1][i];
+    a = fma(-vv[i], i == slast ? wc : un[1][i], a);
+    a = fma(-(i == slast ? wr : ww[i]), un[0][i], a);
   }
-  double ss = 0.0, ar[PR];
-  if (P > 0) {
-#pragma unroll
-    for (int j = 0; j < PR; ++j) {
-      const int r = c + tid + j * nt;
-      double a = sr[j];
-#pragma unroll
-      for (int i = 0; i < kPB; ++i) {
-        a = fma(-vv[j][i], wic[i], a);
-        a = fma(-(i == slast ? wr[j] : ww[j][i]), vic[i], a);
-      }
-      ar[j] = a;
-      const double am = (r < n && r >= c + 2) ? a : 0.0;
-      ss = fma(am, am, ss);
-    }
-  } else {
-    ar[0] = 0.0;
-    for (int r = c + tid; r < n; r += nt) {
-      double a = srow[r];
-#pragma unroll
-      for (int i = 0; i < kPB; ++i) {
-        const double wm = wcl(i)[r];
-        a = fma(-vcol(i)[r], wic[i], a);
-        a = fma(-wm, vic[i], a);  // slot c - 1: w_{c-1}[r], written above by this thread
-      }
-      ascr[r] = a;
-      if (r >= c + 2) ss = fma(a, a, ss);
+  if (act) A.a[S.nv + r] = a;
+  if (r == c) A.d[S.nv + c] = a;
+  const double am = (act && r >= c + 2) ? a : 0.0;
+  const double ss = block_sum(am * am, red);
+  if (tid == 0) A.pe[S.pe + b] = ss;
+}
+
+// Update item (first column c of a panel):
+//   S[r][q] -= sum_{i in [c - kPB, c)} (v_i[r] w_i[q] + w_i[r] v_i[q])   for r, q > c,
+// one 64 x 64 tile (the full square), the panel entries of the tile's rows
+// and columns staged in LDS; thread t updates column t % 64 of kUR rows.
+__device__ void update_item(const ESec *__restrict__ secs, int nsec, int c, int item, const TrdArgs &A) {
+  __shared__ double pv[4][kPB][kUT];  // v_i, w_i on the tile's rows; v_i, w_i on its columns
+  int b = item, q = 0, nb = 0;
+  for (; q < nsec; ++q) {
+    const int n = secs[q].n;
+    if (c < n - 1) {
+      nb = (n - c - 1 + kUT - 1) / kUT;
+      if (b < nb * nb) break;
+      b -= nb * nb;
     }
   }
-  if (tid == 0) A.d[S.nv + c] = P > 0 ? ar[0] : ascr[c];  // row c is thread 0's first
-  if (c >= n - 1) return;
-  // 4. reflector of column c: alpha = a[c+1], x = a[c+2:]
-  if (tid == 1) sh[0] = P > 0 ? ar[0] : ascr[c + 1];  // read after the barriers of the sum
-  const double xn = sqrt(block_sum(ss, red));
-  const double alpha = sh[0];
+  if (q >= nsec) return;
+  __syncthreads();  // pv may still be read for a previous item
+  const ESec S = secs[q];
+  const int n = S.n, t = threadIdx.x;
+  const int r0 = c + 1 + (b / nb) * kUT, q0 = c + 1 + (b % nb) * kUT;
+  const int i0 = c - kPB;
+  for (int idx = t; idx < 4 * kPB * kUT; idx += kTB) {
+    const int which = idx / (kPB * kUT), i = (idx / kUT) % kPB, x = idx % kUT;
+    const int rr = (which < 2 ? r0 : q0) + x;
+    double val = 0.0;
+    if (rr < n) val = (which & 1) ? wcol(A, S, i0 + i)[rr] : A.VH[S.nn + (int64_t)(i0 + i) * n + rr];
+    pv[which][i][x] = val;
+  }
+  __syncthreads();
+  const int x = t & 63, ry = (t >> 6) * kUR;
+  const int qq = q0 + x;
+  if (qq >= n) return;
+  double vq[kPB], wq[kPB];
+#pragma unroll
+  for (int i = 0; i < kPB; ++i) {
+    vq[i] = pv[2][i][x];
+    wq[i] = pv[3][i][x];
+  }
+  double *Sm = A.S + S.nn;
+  // all kUR loads in flight first, then the rows one at a time (the panel
+  // entries of a row read from LDS at one address per wavefront: broadcast)
+  double sv[kUR];
+#pragma unroll
+  for (int j = 0; j < kUR; ++j) {
+    const int r = r0 + ry + j;
+    sv[j] = r < n ? Sm[(int64_t)r * n + qq] : 0.0;
+  }
+#pragma unroll 2
+  for (int j = 0; j < kUR; ++j) {
+    double s = sv[j];
+#pragma unroll
+    for (int i = 0; i < kPB; ++i) {
+      s = fma(-pv[0][i][ry + j], wq[i], s);
+      s = fma(-pv[1][i][ry + j], vq[i], s);
+    }
+    sv[j] = s;
+  }
+#pragma unroll
+  for (int j = 0; j < kUR; ++j) {
+    const int r = r0 + ry + j;
+    if (r < n) Sm[(int64_t)r * n + qq] = sv[j];
+  }
+}
+
+// Product item: wavefront wv of item b of a sector owns row
+// r = c + 1 + b * kPR + wv (lanes along the row); the items of the sectors
+// with c < n - 1 follow each other.
+__device__ void product_item(const ESec *__restrict__ secs, int nsec, int c, int item, const TrdArgs &A) {
+  __shared__ double pt[kPR][kNS];
+  int b = item, q = 0;
+  for (; q < nsec; ++q) {
+    const int n = secs[q].n;
+    if (c < n - 1) {
+      const int nwg = (n - c - 1 + kPR - 1) / kPR;
+      if (b < nwg) break;
+      b -= nwg;
+    }
+  }
+  if (q >= nsec) return;
+  const ESec S = secs[q];
+  const int n = S.n, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int r = c + 1 + b * kPR + wv;
+  const bool act = r < n;
+  const double *a = A.a + S.nv;
+  double *V = A.VH + S.nn;
+  // this lane's value for the next column's partials (lane 1 + i: w_i, lane
+  // kPB + i: v_i of slot i of column c + 1's step; slots below c), loaded now
+  const int c1 = c + 1, k1 = c1 - c1 % kPB;
+  const int pb1 = c1 == k1 ? (c1 >= kPB ? c1 - kPB : c1) : k1;
+  const int ncorr1 = c1 - pb1 > 0 ? c1 - pb1 - 1 : 0;
+  double sl = 0.0;
+  if (act && lane >= 1 && lane < 2 * kPB - 1) {
+    const int i = lane < kPB ? lane - 1 : lane - kPB;
+    if (i < ncorr1) sl = lane < kPB ? wcol(A, S, pb1 + i)[r] : V[(int64_t)(pb1 + i) * n + r];
+  }
+  // reflector of column c: alpha = a[c + 1], |x|^2 = the reflector phase's
+  // partials summed in item order
+  const int ne = (n - c + kER - 1) / kER;
+  double x2 = 0.0;
+  for (int g = 0; g < ne; ++g) x2 += A.pe[S.pe + g];
+  const double xn = sqrt(x2), alpha = a[c + 1];
   double beta, tau, sc;
   if (xn == 0.0) {
     beta = alpha;
@@ -351,127 +356,56 @@ __global__ __launch_bounds__(kTR) void trd_row_kernel(const ESec *__restrict__ s
     tau = (beta - alpha) / beta;
     sc = 1.0 / (alpha - beta);
   }
-  double *vh = A.VH + S.nn + (int64_t)c * n;
-  if (P > 0) {
-#pragma unroll
-    for (int j = 0; j < PR; ++j) {
-      const int r = c + tid + j * nt;
-      if (r > c && r < n) vh[r] = r == c + 1 ? 1.0 : ar[j] * sc;
-    }
-  } else {
-    for (int r = c + tid; r < n; r += nt)
-      if (r > c) vh[r] = r == c + 1 ? 1.0 : ascr[r] * sc;
-  }
-  if (tid == 0) {
+  if (b == 0 && threadIdx.x == 0) {
     A.e[S.nv + c] = beta;
     A.tau[S.nv + c] = tau;
   }
-}
-
-// Deferred update at the first column c of a panel:
-//   S[r][q] -= sum_{i in [c - kPB, c)} (v_i[r] w_i[q] + w_i[r] v_i[q])   for r, q > c,
-// 64 x 64 tiles (the full square), the panel entries of the tile's rows and
-// columns staged in LDS; thread t updates column t % 64 of 16 rows.
-__global__ __launch_bounds__(256) void trd_update_kernel(const ESec *__restrict__ secs, int nsec, int c,
-                                                         TrdArgs A) {
-  int b = blockIdx.x, q = 0, nb = 0;
-  for (; q < nsec; ++q) {
-    const int n = secs[q].n;
-    if (c < n - 1) {
-      nb = (n - c - 1 + kUT - 1) / kUT;
-      if (b < nb * nb) break;
-      b -= nb * nb;
-    }
-  }
-  if (q >= nsec) return;
-  const ESec S = secs[q];
-  const int n = S.n, t = threadIdx.x;
-  const int r0 = c + 1 + (b / nb) * kUT, q0 = c + 1 + (b % nb) * kUT;
-  __shared__ double pv[4][kPB][kUT];  // v_i, w_i on the tile's rows; v_i, w_i on its columns
-  const int i0 = c - kPB;
-  for (int idx = t; idx < 4 * kPB * kUT; idx += 256) {
-    const int which = idx / (kPB * kUT), i = (idx / kUT) % kPB, x = idx % kUT;
-    const int rr = (which < 2 ? r0 : q0) + x;
-    double val = 0.0;
-    if (rr < n) val = (which & 1) ? wcol(A, S, i0 + i)[rr] : A.VH[S.nn + (int64_t)(i0 + i) * n + rr];
-    pv[which][i][x] = val;
-  }
-  __syncthreads();
-  const int x = t & 63, ry = (t >> 6) * 16;
-  const int qq = q0 + x;
-  if (qq >= n) return;
-  double vq[kPB], wq[kPB];
-#pragma unroll
-  for (int i = 0; i < kPB; ++i) {
-    vq[i] = pv[2][i][x];
-    wq[i] = pv[3][i][x];
-  }
-  double *Sm = A.S + S.nn;
-  // all 16 loads in flight first, then the rows one at a time (the panel
-  // entries of a row read from LDS at one address per wavefront: broadcast)
-  double sv[16];
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int r = r0 + ry + j;
-    sv[j] = r < n ? Sm[(int64_t)r * n + qq] : 0.0;
-  }
-#pragma unroll 2
-  for (int j = 0; j < 16; ++j) {
-    double s = sv[j];
-#pragma unroll
-    for (int i = 0; i < kPB; ++i) {
-      s = fma(-pv[0][i][ry + j], wq[i], s);
-      s = fma(-pv[1][i][ry + j], vq[i], s);
-    }
-    sv[j] = s;
-  }
-#pragma unroll
-  for (int j = 0; j < 16; ++j) {
-    const int r = r0 + ry + j;
-    if (r < n) Sm[(int64_t)r * n + qq] = sv[j];
-  }
-}
-
-// Product kernel for column c, rows r > c (kSymvRows per workgroup, one per
-// wavefront, lanes along the row): y[r] = sum_{q > c} S[r][q] v_c[q].
-// S is only read.
-__global__ __launch_bounds__(kT) void trd_symv_kernel(const ESec *__restrict__ secs, int nsec, int c, TrdArgs A) {
-  int b = blockIdx.x, q = 0;
-  for (; q < nsec; ++q) {
-    const int n = secs[q].n;
-    if (c < n - 1) {
-      const int nwg = (n - c - 1 + kSymvRows - 1) / kSymvRows;
-      if (b < nwg) break;
-      b -= nwg;
-    }
-  }
-  if (q >= nsec) return;
-  const ESec S = secs[q];
-  const int n = S.n, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r = c + 1 + b * kSymvRows + wv;
-  if (r >= n) return;  // whole wavefront; no barrier follows
-  const double *row = A.S + S.nn + (int64_t)r * n;
-  const double *v = A.VH + S.nn + (int64_t)c * n;
+  // y[r] = sum_{q > c} S[r][q] v_c[q], v_c[q] = 1 (q = c + 1) or a[q] sc.
   // Columns in chunks of U 64-wide strides, every load of a chunk issued
-  // before its arithmetic (a row of up to 64 U columns costs one memory
-  // round trip; the last chunk reads zeros past the row end, which leave the
-  // sum unchanged); each lane sums its columns in increasing order.
+  // before its arithmetic (the last chunk reads zeros past the row end, which
+  // leave the sum unchanged); each lane sums its columns in increasing order.
   constexpr int U = kSymvChunk;
-  double acc = 0;
-  for (int col = c + 1 + lane; col < n; col += U * 64) {
-    double x[U], t[U];
+  double acc = 0.0, vr = 0.0;
+  if (act) {
+    const double *row = A.S + S.nn + (int64_t)r * n;
+    for (int col = c + 1 + lane; col < n; col += U * 64) {
+      double x[U], t[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int cc = col + 64 * u;
-      const bool in = cc < n;
-      x[u] = in ? v[cc] : 0.0;
-      t[u] = in ? row[cc] : 0.0;
+      for (int u = 0; u < U; ++u) {
+        const int cc = col + 64 * u;
+        const bool in = cc < n;
+        x[u] = in ? a[cc] : 0.0;
+        t[u] = in ? row[cc] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) acc = fma(t[u], col + 64 * u == c + 1 ? 1.0 : x[u] * sc, acc);
     }
-#pragma unroll
-    for (int u = 0; u < U; ++u) acc = fma(t[u], x[u], acc);
+    vr = r == c + 1 ? 1.0 : a[r] * sc;
+    if (lane == 0) V[(int64_t)c * n + r] = vr;
   }
   const double y = wave_sum(acc);
-  if (lane == 0) A.y[S.nv + r] = y;
+  if (act && lane == 0) A.y[S.nv + r] = y;
+  // partials of column c + 1's step over this item's rows: y.v_c, then
+  // w_i.v_c and v_i.v_c of its slots
+  __syncthreads();  // pt may still be read for a previous item
+  if (lane < kNS) pt[wv][lane] = act ? (lane == 0 ? y : sl) * vr : 0.0;
+  __syncthreads();
+  if (threadIdx.x < kNS) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kPR; ++w) t += pt[w][threadIdx.x];
+    A.ps[(S.ps + b) * kNS + threadIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kTB) void trd_reflect_kernel(const ESec *__restrict__ secs, int nsec, int c, TrdArgs A) {
+  reflect_item(secs, nsec, c, blockIdx.x, A);
+}
+__global__ __launch_bounds__(kTB) void trd_update_kernel(const ESec *__restrict__ secs, int nsec, int c, TrdArgs A) {
+  update_item(secs, nsec, c, blockIdx.x, A);
+}
+__global__ __launch_bounds__(kTB) void trd_product_kernel(const ESec *__restrict__ secs, int nsec, int c, TrdArgs A) {
+  product_item(secs, nsec, c, blockIdx.x, A);
 }
 
 // T block of panel p (compact WY, forward columnwise) from the Gram of its
@@ -948,17 +882,19 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
   st->nsec = nsec;
   st->n = ns;
   st->secs.resize(nsec);
-  int64_t nn = 0, nv = 0, tb = 0;
-  int maxn = 0;
+  int64_t nn = 0, nv = 0, tb = 0, nps = 0;
+  int maxn = 0, npe = 0;
   for (int q = 0; q < nsec; ++q) {
     MPS2_REQUIRE(goffs[q] == nn, "eigh: Gram matrices must be packed back to back");
     ESec &S = st->secs[q];
     S.n = ns[q];
     S.nn = nn;
     S.nv = nv;
-    S.np = 0;
+    S.ps = nps;
     S.tb = tb;
-    S.pad = 0;
+    S.pe = npe;
+    nps += (ns[q] + kPR - 1) / kPR;
+    npe += (ns[q] + kER - 1) / kER;
     nn += (int64_t)ns[q] * ns[q];
     nv += ns[q];
     tb += (int64_t)((ns[q] + kNB - 1) / kNB) * kNB * kNB;
@@ -975,7 +911,9 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
   st->Z = c->alloc(nn * sizeof(double));
   auto bd = c->alloc(nv * sizeof(double)), be = c->alloc(nv * sizeof(double)), btau = c->alloc(nv * sizeof(double)),
        by = c->alloc(nv * sizeof(double)), ba = c->alloc(nv * sizeof(double)),
-       bw0 = c->alloc(nv * kPB * sizeof(double)), bw1 = c->alloc(nv * kPB * sizeof(double));
+       bw0 = c->alloc(nv * kPB * sizeof(double)), bw1 = c->alloc(nv * kPB * sizeof(double)),
+       bps = c->alloc(std::max<int64_t>(nps, 1) * kNS * sizeof(double)),
+       bpe = c->alloc(std::max<int64_t>(npe, 1) * sizeof(double));
   double *VH = dp<double>(st->VH), *Tb = dp<double>(st->Tb);
   double *dv = dp<double>(bd), *ev = dp<double>(be), *tauv = dp<double>(btau);
   k_fill(c, VH, nn, 0.0);
@@ -990,30 +928,23 @@ EighResult eigh_batched(Ctx *c, double *G, const std::vector<int64_t> &goffs, co
     c->count("eigh_n3_sum_g", n3 * 1e-9);   // sum of n^3 over sectors, 1e9 units
   }
   // ---- 1. tridiagonalisation, one column per step for every sector
-  const TrdArgs TA{G, VH, dp<double>(bw0), dp<double>(bw1), dp<double>(by), dp<double>(ba), dv, ev, tauv};
-  auto launch_col = [&](int col) {
-    int rows = 0;
-    int64_t nwg = 0, nut = 0;
+  const TrdArgs TA{G,  VH,   dp<double>(bw0), dp<double>(bw1), dp<double>(by), dp<double>(ba),
+                   dv, ev, tauv, dp<double>(bps), dp<double>(bpe)};
+  for (int col = 0; col < maxn; ++col) {
+    int64_t ner = 0, nwg = 0, nut = 0;
     for (int q = 0; q < nsec; ++q) {
-      if (col < ns[q]) rows = std::max(rows, ns[q] - col);
+      if (col < ns[q]) ner += (ns[q] - col + kER - 1) / kER;
       if (col < ns[q] - 1) {
-        nwg += (ns[q] - col - 1 + kSymvRows - 1) / kSymvRows;
+        nwg += (ns[q] - col - 1 + kPR - 1) / kPR;
         const int64_t nb = (ns[q] - col - 1 + kUT - 1) / kUT;
         nut += nb * nb;
       }
     }
-    switch ((rows + kTR - 1) / kTR) {
-      case 1: trd_row_kernel<1><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
-      case 2: trd_row_kernel<2><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
-      case 3:
-      case 4: trd_row_kernel<4><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
-      default: trd_row_kernel<0><<<nsec, kTR, 0, c->stream>>>(SD, col, TA); break;
-    }
+    if (ner > 0) trd_reflect_kernel<<<(unsigned)ner, kTB, 0, c->stream>>>(SD, nsec, col, TA);
     if (col % kPB == 0 && col >= kPB && nut > 0)
-      trd_update_kernel<<<(unsigned)nut, 256, 0, c->stream>>>(SD, nsec, col, TA);
-    if (nwg > 0) trd_symv_kernel<<<(unsigned)nwg, kT, 0, c->stream>>>(SD, nsec, col, TA);
-  };
-  for (int col = 0; col < maxn; ++col) launch_col(col);
+      trd_update_kernel<<<(unsigned)nut, kTB, 0, c->stream>>>(SD, nsec, col, TA);
+    if (nwg > 0) trd_product_kernel<<<(unsigned)nwg, kTB, 0, c->stream>>>(SD, nsec, col, TA);
+  }
   // WY blocks: Gram of each panel's reflectors (MFMA), then the T recurrence
   {
     std::vector<PanelRef> prs;
