@@ -345,7 +345,28 @@ __device__ void product_item(const ESec *__restrict__ secs, int nsec, int c, int
   const int ne = (n - c + kER - 1) / kER;
   double x2 = 0.0;
   for (int g = 0; g < ne; ++g) x2 += A.pe[S.pe + g];
-  const double xn = sqrt(x2), alpha = a[c + 1];
+  const double alpha = a[c + 1];
+  // y[r] = sum_{q > c} S[r][q] v_c[q], v_c[q] = 1 (q = c + 1) or a[q] sc.
+  // Columns in chunks of U 64-wide strides, every load of a chunk issued
+  // before its arithmetic (the last chunk reads zeros past the row end, which
+  // leave the sum unchanged); each lane sums its columns in increasing order.
+  // The first chunk is loaded before the reflector's scalars are formed, so
+  // its round trip overlaps theirs.
+  constexpr int U = kSymvChunk / 2;
+  const double *row = A.S + S.nn + (int64_t)r * n;
+  double x[U], t[U];
+  auto load = [&](int col) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int cc = col + 64 * u;
+      const bool in = act && cc < n;
+      x[u] = in ? a[cc] : 0.0;
+      t[u] = in ? row[cc] : 0.0;
+    }
+  };
+  int col = c + 1 + lane;
+  load(col);
+  const double xn = sqrt(x2);
   double beta, tau, sc;
   if (xn == 0.0) {
     beta = alpha;
@@ -360,25 +381,13 @@ __device__ void product_item(const ESec *__restrict__ secs, int nsec, int c, int
     A.e[S.nv + c] = beta;
     A.tau[S.nv + c] = tau;
   }
-  // y[r] = sum_{q > c} S[r][q] v_c[q], v_c[q] = 1 (q = c + 1) or a[q] sc.
-  // Columns in chunks of U 64-wide strides, every load of a chunk issued
-  // before its arithmetic (the last chunk reads zeros past the row end, which
-  // leave the sum unchanged); each lane sums its columns in increasing order.
-  constexpr int U = kSymvChunk;
   double acc = 0.0, vr = 0.0;
   if (act) {
-    const double *row = A.S + S.nn + (int64_t)r * n;
-    for (int col = c + 1 + lane; col < n; col += U * 64) {
-      double x[U], t[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int cc = col + 64 * u;
-        const bool in = cc < n;
-        x[u] = in ? a[cc] : 0.0;
-        t[u] = in ? row[cc] : 0.0;
-      }
+    while (col < n) {
 #pragma unroll
       for (int u = 0; u < U; ++u) acc = fma(t[u], col + 64 * u == c + 1 ? 1.0 : x[u] * sc, acc);
+      col += U * 64;
+      if (col < n) load(col);
     }
     vr = r == c + 1 ? 1.0 : a[r] * sc;
     if (lane == 0) V[(int64_t)c * n + r] = vr;

@@ -249,8 +249,10 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
 
   SlabT slab;
   int cur = 0;
-  // One slab's products.
-  auto multiply = [&]() {
+  // One slab's products; mid() runs halfway through them (the next slab's
+  // LDS store, so that it and the wait for its global loads overlap the
+  // matrix-core work).
+  auto multiply = [&](auto &&mid) {
     if constexpr (SlabT::kQ) {
       constexpr int SK = SlabT::SK;
       constexpr int KPL = BK / 4;   // k values per lane group and slab
@@ -264,12 +266,14 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
 #pragma unroll
         for (int j = 0; j < FN; ++j) bf[j].load(bs + j * 16 * SK + 4 * h);
 #pragma unroll
-        for (int st = 0; st < 4; ++st)
+        for (int st = 0; st < 4; ++st) {
+          if (4 * h + st == KPL / 2) mid();
 #pragma unroll
           for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j)
               if (!kSkip || (i < fmv && j < fnv)) acc[i][j] = Mfma<T>::op(af[i].v[st], bf[j].v[st], acc[i][j]);
+        }
       }
     } else {
       constexpr int PA = SlabT::PA, PB = SlabT::PB;
@@ -277,6 +281,7 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
       const T *bs = Bs[cur];
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
+        if (ks == BK / 2) mid();
         const int kr = ks + (lane >> 4);
         T af[FM], bf[FN];
 #pragma unroll
@@ -306,13 +311,15 @@ __global__ __launch_bounds__(kThreads) void grouped_gemm_kernel(const T *__restr
       }
       const bool has_next = np < blk.pe && (SPLIT == 1 || left > 1);
       if (has_next) slab.load(A + npr.a_off, B + npr.b_off, M, N, npr.K, m0, n0, nk0, tid);
-      multiply();
       if constexpr (NBUF == 2) {
-        if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
+        multiply([&] {
+          if (has_next) slab.store(As[cur ^ 1], Bs[cur ^ 1], tid);
+        });
         __syncthreads();
         if (!has_next) break;
         cur ^= 1;
       } else {
+        multiply([] {});
         if (!has_next) break;
         __syncthreads();
         slab.store(As[0], Bs[0], tid);
