@@ -464,7 +464,7 @@ def main():
         "sweep_counters": {k: {c: round(res[k]["counters"][c], 1)
                                for c in ("matvec_fp64", "matvec_fp32", "syncs", "svd_calls", "hip_malloc",
                                          "svd_eigh_ms", "svd_refine_ms", "svd_refine_eigh_ms", "svd_refine_rounds",
-                                         "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
+                                         "svd_refine_settled", "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
                                          "svd_max_sector_sum", "gemm_narrow", "gemm_narrow_rowmajor_a",
                                          "plan_builds", "plan_build_ms")}
                            for k in res},

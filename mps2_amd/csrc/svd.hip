@@ -59,6 +59,20 @@ constexpr double kDegTol = 1e-6;
 // values s_r >= 3e-4 s_max just above the refinement threshold into the tail
 // basis).  The truncation treats them as tied.
 constexpr double kResolve = kSvdResolve;
+constexpr double kEps = 2.220446049250313e-16;
+
+// Multiple of eps in the error bound of a value (MPS2_SVD_SETTLE_BAND,
+// default 2, twelve times the error measured on graded spectra, 0.16 eps
+// times the problem's largest eigenvalue; 0 keeps refining until the
+// uniform accuracy rule is met).
+double settle_band() {
+  static const double v = [] {
+    const char *e = std::getenv("MPS2_SVD_SETTLE_BAND");
+    const double x = e ? std::atof(e) : 2.0;
+    return x > 0.0 ? x : 0.0;
+  }();
+  return v;
+}
 
 // Jacobi sweeps per 2b x 2b subproblem and per outer round (MPS2_JACOBI_INNER,
 // default 1), and the rotation threshold relative to trace(G_q)
@@ -578,7 +592,7 @@ struct SvInfo {
 };
 
 void truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err, int64_t dmin, int64_t dmax,
-                   std::vector<int64_t> &kept, double *err, int64_t *D) {
+                   std::vector<int64_t> &kept, double *err, int64_t *D, int64_t *derr_out = nullptr) {
   std::vector<SvInfo> all;
   for (int q = 0; q < (int)sv.size(); ++q)
     for (int j = 0; j < (int)sv[q].size(); ++j) all.push_back({sv[q][j], q, j});
@@ -599,6 +613,7 @@ void truncate_rule(const std::vector<std::vector<double>> &sv, double trunc_err,
       else break;
     }
   }
+  if (derr_out) *derr_out = derr;
   int64_t d = std::max(derr, dmin);
   d = std::min(d, dmax);
   d = std::min(d, cnt);
@@ -901,6 +916,56 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     std::vector<double> lam_top(ns, 0.0);  // scale of each sector's current problem
     for (int q = 0; q < ns; ++q)
       for (double x : evals[q]) lam_top[q] = std::max(lam_top[q], x);
+    // Error scale of each value: the largest eigenvalue of the problem it
+    // was last computed from (its eigensolver's error is a few eps times that).
+    std::vector<std::vector<double>> esc(ns);
+    for (int q = 0; q < ns; ++q) esc[q].assign(evals[q].size(), lam_top[q]);
+    // True when no value's place relative to the cut's tie window (relative
+    // kDegTol around the D-th largest value) can change within its error
+    // bound, band eps (scale + 2 s s_max) (the eigensolver's error and that of
+    // the projected tail), and D does not hang on the trunc_err tail sum: a
+    // further round would then leave the kept counts as they are.  The kept
+    // values near the cut then carry that bound, which for every value above
+    // kResolve s_max is smaller than a direct SVD's eps s_max.
+    auto settled = [&](int64_t derr_now) -> bool {
+      const double band = settle_band();
+      if (band <= 0.0) return false;
+      if (!(trunc_err <= 0.0 || derr_now >= out.D + 2 || derr_now + 2 <= out.D)) return false;
+      double top = 0.0;
+      for (int q = 0; q < ns; ++q)
+        for (double x : evals[q]) top = std::max(top, x);
+      const double smax = std::sqrt(top), thr = kResolve * smax;
+      auto rule = [&](double x) { return x > 0.0 ? std::max(std::sqrt(x), thr) : thr; };
+      auto err = [&](int q, int j) {
+        return band * kEps * (esc[q][j] + 2.0 * std::sqrt(std::max(evals[q][j], 0.0)) * smax);
+      };
+      struct Ent {
+        double v;
+        int q, j, pos;
+      };
+      std::vector<Ent> es;
+      for (int q = 0; q < ns; ++q)
+        for (int t = 0; t < (int)order[q].size(); ++t) es.push_back({svr[q][t], q, order[q][t], t});
+      if (out.D <= 0 || out.D >= (int64_t)es.size()) return false;
+      std::nth_element(es.begin(), es.begin() + (out.D - 1), es.end(), [](const Ent &x, const Ent &y) {
+        if (x.v != y.v) return x.v > y.v;
+        if (x.q != y.q) return x.q < y.q;
+        return x.pos < y.pos;
+      });
+      const Ent cut = es[out.D - 1];
+      const double lc = evals[cut.q][cut.j], ec = err(cut.q, cut.j);
+      const double vlo = rule(lc - ec), vhi = rule(lc + ec);
+      for (const Ent &x : es) {
+        const double l = evals[x.q][x.j];
+        if (l == 0.0) continue;  // a structural zero is always cut
+        const double e = err(x.q, x.j), lo = rule(l - e), hi = rule(l + e);
+        const bool below = hi < vlo * (1.0 - kDegTol);
+        const bool above = lo > vhi * (1.0 + kDegTol);
+        const bool inside = lo > vhi * (1.0 - kDegTol) && hi < vlo * (1.0 + kDegTol);
+        if (!(below || above || inside)) return false;
+      }
+      return true;
+    };
     for (int round = 0; round < 3; ++round) {
       std::vector<double> all;
       for (auto &v : svr) all.insert(all.end(), v.begin(), v.end());
@@ -1113,14 +1178,20 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
         for (int j = 0; j < r; ++j) evals[q][j] = er2.evals[i][j];
         lam_top[q] = 0.0;
         for (int j = 0; j < r; ++j) lam_top[q] = std::max(lam_top[q], er2.evals[i][j]);
+        for (int j = 0; j < r; ++j) esc[q][j] = lam_top[q];
       }
       singular_values();
-      truncate_rule(svr, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D);
+      int64_t derr_now = 0;
+      truncate_rule(svr, trunc_err, dmin, dmax, out.kept, &out.trunc_err, &out.D, &derr_now);
       // The kept vectors stay the tridiagonal solution's (the products S Vt =
       // U^T M, or U S = M V, are exact projections for any basis of the kept
       // span); refined vectors are formed only for a tail that a further
       // round must resolve.
       if (round == 2) break;
+      if (settled(derr_now)) {
+        c->count("svd_refine_settled");
+        break;
+      }
       all.clear();
       for (auto &v : svr) all.insert(all.end(), v.begin(), v.end());
       if (out.D <= 0 || out.D >= (int64_t)all.size()) break;

@@ -284,7 +284,8 @@ COUNTERS = ("svd_calls", "svd_ms", "svd_eigh_ms", "svd_sectors", "svd_max_sector
             "matvec_fp64", "matvec_fp32", "hip_malloc", "alloc_trim", "svd_refine_rounds", "svd_refine_sectors",
             "svd_refine_ms", "svd_refine_vec_ms", "svd_refine_eigh_ms", "tune_probes", "tune_decisions",
             "tune_changed", "tune_cache_hits", "eigh_trd_ms", "eigh_dc_ms", "eigh_back_ms", "eigh_trd_columns",
-            "eigh_n3_sum_g", "gemm_narrow", "gemm_narrow_rowmajor_a", "plan_builds", "plan_build_ms")
+            "eigh_n3_sum_g", "gemm_narrow", "gemm_narrow_rowmajor_a", "plan_builds", "plan_build_ms",
+            "svd_refine_settled")
 
 
 def ctx_counters(ctx_handle, names=COUNTERS):
