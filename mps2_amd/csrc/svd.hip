@@ -925,8 +925,9 @@ SvdOut svd_impl(const TenP &t, int ldims, const QN &ldiv, double trunc_err, int6
     // bound, band eps (scale + 2 s s_max) (the eigensolver's error and that of
     // the projected tail), and D does not hang on the trunc_err tail sum: a
     // further round would then leave the kept counts as they are.  The kept
-    // values near the cut then carry that bound, which for every value above
-    // kResolve s_max is smaller than a direct SVD's eps s_max.
+    // values near the cut then carry that bound, in s eps (scale / s + 2 s_max)
+    // at band 2: at most 12 eps s_max at kResolve s_max, within a direct SVD's
+    // p(n) eps s_max.
     auto settled = [&](int64_t derr_now) -> bool {
       const double band = settle_band();
       if (band <= 0.0) return false;

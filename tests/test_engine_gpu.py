@@ -156,6 +156,43 @@ def test_svd_deep_cut_parity(ctx, orth):
 
 
 @pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
+def test_svd_settled_cut_parity(ctx, orth):
+    """Cut at the resolution floor (Dmax above the number of values over
+    1e-11 s_max; spectra over 16 decades with exact +-q multiplets), the case
+    of the D=4096 benchmark: the refinement stops after one round once every
+    value's place relative to the cut's tie window is settled within its error
+    bound (2 eps (scale + 2 s s_max)), and the kept counts per sector, D and
+    the kept values above the floor still match the oracle's direct SVD to
+    within that bound, at most 12 eps s_max at the floor (asserted with the
+    oracle's own error added)."""
+    rng = np.random.default_rng(11 + orth)
+    psi = graded_block_matrix(rng, (300, 420, 300), (-2, 0, 2), 16.0)
+    dpsi = ctx.put(psi)
+    before = E.ctx_counters(ctx.h, ("svd_refine_settled", "svd_refine_rounds"))
+    Uo, So, Vo, eo, Do = O.svd(psi, 1, (0,), 0.0, 1, 900)
+    u, s, vt, eg, Dg = E.svd(dpsi, 1, (0,), 0.0, 1, 900, orth)
+    after = E.ctx_counters(ctx.h, ("svd_refine_settled", "svd_refine_rounds"))
+    assert after["svd_refine_settled"] - before["svd_refine_settled"] == 1
+    assert after["svd_refine_rounds"] - before["svd_refine_rounds"] == 1
+    assert Dg == Do == 900
+    assert u.get().legs[-1].key() == Uo.legs[-1].key()
+    so, sg = np.diag(So.dense()), np.diag(s.get().dense())
+    smax = so.max()
+    assert np.sum(so > 1e-11 * smax) < 900  # the cut lies at the floor
+    assert np.abs(so - sg).max() < 1e-13 * smax, np.abs(so - sg).max() / smax
+    big = so > 1e-8 * smax
+    np.testing.assert_allclose(sg[big], so[big], rtol=1e-8)
+    # refined values between the floor and 1e-8 s_max: within 12 eps s_max
+    # plus the oracle's own error
+    res = so > 1e-11 * smax
+    mid = res & ~big
+    assert np.abs(so - sg)[mid].max() <= 1e-14 * smax, np.abs(so - sg)[mid].max() / smax
+    # the kept values below it are tied by the truncation rule; after one
+    # round they are resolved only to about sqrt(eps 1e-10) s_max = 2e-13 s_max
+    assert np.all(sg[~res] < 2e-11 * smax)
+
+
+@pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
 @pytest.mark.parametrize("dims", [(3, 2, 4), (40, 70, 25), (150, 260, 120)])
 def test_svd_parity(ctx, orth, dims):
     rng = np.random.default_rng(sum(dims) + orth)
