@@ -158,10 +158,11 @@ def test_svd_deep_cut_parity(ctx, orth):
 @pytest.mark.parametrize("orth", [E.ORTH_LEFT, E.ORTH_RIGHT])
 def test_svd_settled_cut_parity(ctx, orth):
     """Cut at the resolution floor (Dmax above the number of values over
-    1e-11 s_max; spectra over 16 decades with exact +-q multiplets), the case
-    of the D=4096 benchmark: the refinement stops after one round once every
-    value's place relative to the cut's tie window is settled within its error
-    bound (2 eps (scale + 2 s s_max)), and the kept counts per sector, D and
+    1e-11 s_max; spectra over 16 decades with exact +-q multiplets), a cut
+    that needed a second refinement round before: the refinement now stops
+    after one round once every value's place relative to the cut's tie window
+    is settled within its error bound (2 eps (scale + 2 s s_max)), and the
+    kept counts per sector, D and
     the kept values above the floor still match the oracle's direct SVD to
     within that bound, at most 12 eps s_max at the floor (asserted with the
     oracle's own error added)."""
